@@ -1,0 +1,135 @@
+"""Line table and spectral-region configuration.
+
+Mirrors the reference's FORWARD_PARAM line arrays (forward_param.f90:16-20)
+and the LINES: control-file rows (input_file.f90:253-367): a region has a
+main line, optional blends (offset = 1e3*(lambda0_blend - lambda0_main) mA),
+and a uniform wavelength grid WAVE(j) = start + step*j in mA relative to the
+main line (pre_post_duties.f90:592-621).
+
+The per-line constants come from firtez_dz_amd/data/lines_database.json,
+derived from the reference's default/lines_database.dat (including the
+Anstee-Barklem-O'Mara collisional parameters for rows that do not list them).
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass, field
+from typing import List, Sequence
+
+import numpy as np
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "lines_database.json")
+
+MAXBLEND = 4
+
+
+@dataclass
+class Line:
+    id: int
+    zn: int
+    ion: int
+    lambda0: float     # Angstrom
+    loggf: float
+    eplow: float       # eV
+    sigma: float       # ABO cross-section (Bohr radii^2), 0 -> Unsold
+    alpha: float       # ABO velocity exponent
+    sl: float
+    ll: float
+    jl: float
+    su: float
+    lu: float
+    ju: float
+
+
+@dataclass
+class Region:
+    lines: List[int]          # indices into the configured line list; [0] = main line
+    nwave: int
+    wave_start: float         # mA relative to the main line
+    wave_step: float          # mA
+
+
+@dataclass
+class LineSetup:
+    lines: List[Line]
+    regions: List[Region] = field(default_factory=list)
+
+    @property
+    def numw(self) -> int:
+        return sum(r.nwave for r in self.regions)
+
+    def wave(self) -> np.ndarray:
+        return np.concatenate([r.wave_start + r.wave_step * np.arange(r.nwave) for r in self.regions])
+
+
+def database() -> dict:
+    with open(_DATA) as fh:
+        rows = json.load(fh)["lines"]
+    out = {}
+    for r in rows:
+        # the reference takes the first row with a given id (input_file.f90:305-310)
+        out.setdefault(r["id"], r)
+    return out
+
+
+def line_from_db(line_id: int) -> Line:
+    r = database()[line_id]
+    return Line(**{k: r[k] for k in Line.__dataclass_fields__})
+
+
+def setup(rows: Sequence[tuple]) -> LineSetup:
+    """rows: LINES:-style tuples ((main_id, blend_id, ...), nwave, start_mA, step_mA)."""
+    lines: List[Line] = []
+    ids: List[int] = []
+    regions = []
+    for ids_row, nw, start, step in rows:
+        idx = []
+        for lid in ids_row:
+            if lid not in ids:
+                ids.append(lid)
+                lines.append(line_from_db(lid))
+            idx.append(ids.index(lid))
+        if len(idx) > MAXBLEND:
+            raise ValueError("at most %d lines per region" % MAXBLEND)
+        regions.append(Region(idx, int(nw), float(start), float(step)))
+    return LineSetup(lines, regions)
+
+
+def fe6301_6302(nwave: int = 112, start: float = -624.0, step: float = 20.0) -> LineSetup:
+    """Fe I 6301.5 with the Fe I 6302.5 blend in one region (LINES: 4,5 ...)."""
+    return setup([((4, 5), nwave, start, step)])
+
+
+# ---------------------------------------------------------------- C structs
+class CLine(ctypes.Structure):
+    _fields_ = [("zn", ctypes.c_int32), ("ion", ctypes.c_int32),
+                ("lambda0", ctypes.c_double), ("loggf", ctypes.c_double),
+                ("eplow", ctypes.c_double), ("sigma", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("sl", ctypes.c_double),
+                ("ll", ctypes.c_double), ("jl", ctypes.c_double),
+                ("su", ctypes.c_double), ("lu", ctypes.c_double),
+                ("ju", ctypes.c_double)]
+
+
+class CRegion(ctypes.Structure):
+    _fields_ = [("nblend", ctypes.c_int32), ("line", ctypes.c_int32 * MAXBLEND),
+                ("nwave", ctypes.c_int32), ("wave_start", ctypes.c_double),
+                ("wave_step", ctypes.c_double)]
+
+
+def c_arrays(ls: LineSetup):
+    lines = (CLine * len(ls.lines))()
+    for i, ln in enumerate(ls.lines):
+        for f, _ in CLine._fields_:
+            setattr(lines[i], f, getattr(ln, f))
+    regs = (CRegion * len(ls.regions))()
+    for i, r in enumerate(ls.regions):
+        regs[i].nblend = len(r.lines)
+        for b, li in enumerate(r.lines):
+            regs[i].line[b] = li
+        regs[i].nwave = r.nwave
+        regs[i].wave_start = r.wave_start
+        regs[i].wave_step = r.wave_step
+    return lines, regs

@@ -1,0 +1,328 @@
+"""The oracle: a CPU restatement of the FIRTEZ-dz column path, and (where it
+was built) the reference's own Fortran modules behind a small harness.
+
+TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py import this package, always as the checker.
+The product (firtez_dz_amd) never imports it.
+
+  Cpu   -> oracle/libfzcpu.so  (fzcpu.c, FP64, OpenMP over columns)
+  Ref   -> oracle/_ref/libfzref.so (reference Fortran + harness; built only
+           where the reference tree exists: make -C oracle/ref)
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from firtez_dz_amd import lines as _lines
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CPU_LIB = os.path.join(HERE, "libfzcpu.so")
+REF_LIB = os.path.join(HERE, "_ref", "libfzref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def build_cpu():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def build_ref():
+    """Build oracle/_ref from the reference sources (only where they exist)."""
+    src = os.environ.get("FZ_REFERENCE", "/root/reference") + "/src"
+    if not os.path.isdir(src):
+        return False
+    subprocess.check_call(["make", "-s", "-C", os.path.join(HERE, "ref"), "REF=" + src])
+    return True
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+class Cpu:
+    """ctypes view of fzcpu.c."""
+
+    def __init__(self):
+        if not os.path.exists(CPU_LIB):
+            build_cpu()
+        self.lib = ctypes.CDLL(CPU_LIB)
+        L = self.lib
+        L.fzc_setup.restype = ctypes.c_int
+        L.fzc_partition.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.fzc_eos.argtypes = [ctypes.c_float, ctypes.c_float, _dp, _dp, _dp]
+        L.fzc_opac_cont.argtypes = [ctypes.c_float, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.fzc_opac_line.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.fzc_damping.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_double, _dp, _dp]
+        L.fzc_voigt.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        L.fzc_zeeman.argtypes = [ctypes.c_int, _dp, _dp, _ip]
+        L.fzc_gain.argtypes = [_fp]
+        L.fzc_set_gain.argtypes = [_fp]
+        L.fzc_forward1d.argtypes = [ctypes.c_int, _fp, _dp, _dp, _dp]
+        L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
+        L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
+        self.numw = 0
+        self.resp = False
+
+    def setup(self, ls: _lines.LineSetup, resp=True, atmpar=(1, 1, 0, 1, 1, 1, 1, 0)):
+        cl, cr = _lines.c_arrays(ls)
+        ap = (ctypes.c_int32 * 8)(*atmpar)
+        n = self.lib.fzc_setup(cl, len(ls.lines), cr, len(ls.regions), int(resp), ap)
+        if n < 0:
+            raise RuntimeError("fzc_setup failed %d" % n)
+        self.numw, self.resp, self.atmpar = n, bool(resp), tuple(atmpar)
+        return n
+
+    def partition(self, elem, T):
+        u = np.zeros(3)
+        du = np.zeros(3)
+        self.lib.fzc_partition(elem, T, _ptr(u, _dp), _ptr(du, _dp))
+        return u, du
+
+    def eos(self, T, P, ne=0.0):
+        ne_ = ctypes.c_double(ne)
+        out = np.zeros(5)
+        d = np.zeros(25)
+        e = self.lib.fzc_eos(T, P, ctypes.byref(ne_), _ptr(out, _dp), _ptr(d, _dp))
+        return e, out, d
+
+    def opac_cont(self, T, lam, ne, nh, d):
+        kc = ctypes.c_double()
+        d = np.array(d, dtype=np.float64)
+        e = self.lib.fzc_opac_cont(T, lam, ne, nh, ctypes.byref(kc), _ptr(d, _dp))
+        return e, kc.value, d
+
+    def opac_line(self, line, T, ne, nh, d):
+        kl = ctypes.c_double()
+        d = np.array(d, dtype=np.float64)
+        e = self.lib.fzc_opac_line(line, T, ne, nh, ctypes.byref(kl), _ptr(d, _dp))
+        return e, kl.value, d
+
+    def damping(self, line, T, nh, d):
+        dm = ctypes.c_double()
+        d = np.array(d, dtype=np.float64)
+        self.lib.fzc_damping(line, T, nh, ctypes.byref(dm), _ptr(d, _dp))
+        return dm.value, d
+
+    def voigt(self, a, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        h = np.zeros_like(v)
+        f = np.zeros_like(v)
+        self.lib.fzc_voigt(len(v), a, _ptr(v, _dp), _ptr(h, _dp), _ptr(f, _dp))
+        return h, f
+
+    def zeeman(self, line):
+        s = np.zeros(64)
+        st = np.zeros(64)
+        ty = np.zeros(64, dtype=np.int32)
+        n = self.lib.fzc_zeeman(line, _ptr(s, _dp), _ptr(st, _dp), _ptr(ty, _ip))
+        return s[:n], st[:n], ty[:n]
+
+    def gain(self):
+        g = np.zeros((self.numw, 4), dtype=np.float32)
+        e = self.lib.fzc_gain(_ptr(g, _fp))
+        if e:
+            raise RuntimeError("fzc_gain error %d" % e)
+        return g.T.copy()            # (4, numw)
+
+    def set_gain(self, g):
+        g = np.ascontiguousarray(np.asarray(g, dtype=np.float32).T)
+        self.lib.fzc_set_gain(_ptr(g, _fp))
+
+    def forward1d(self, model):
+        """model: (13, nz) float32 [T,Pg,rho,Pe,mu,Bx,By,Bz,vx,vy,vz,tau5,z];
+        returns (model_out, syn (4,numw), dsyn (nz,numw,8,4) or None, kc5)."""
+        m = np.ascontiguousarray(model, dtype=np.float32).copy()
+        nz = m.shape[1]
+        syn = np.zeros((self.numw, 4))
+        dsyn = np.zeros((nz, self.numw, 8, 4)) if self.resp else np.zeros(1)
+        kc5 = np.zeros(nz)
+        e = self.lib.fzc_forward1d(nz, _ptr(m, _fp), _ptr(syn, _dp), _ptr(dsyn, _dp), _ptr(kc5, _dp))
+        if e:
+            raise RuntimeError("fzc_forward1d error %d" % e)
+        return m, syn.T.copy(), (dsyn if self.resp else None), kc5
+
+    def forward_batch(self, models, nthreads=0):
+        """models: (ncol, 13, nz) float32 (updated copy returned)."""
+        m = np.ascontiguousarray(models, dtype=np.float32).copy()
+        ncol, _, nz = m.shape
+        syn = np.zeros((ncol, self.numw, 4))
+        dsyn = np.zeros((ncol, nz, self.numw, 8, 4)) if self.resp else None
+        e = self.lib.fzc_forward_batch(ncol, nz, _ptr(m, _fp), _ptr(syn, _dp),
+                                       _ptr(dsyn, _dp) if dsyn is not None else None, nthreads)
+        if e:
+            raise RuntimeError("fzc_forward_batch error %d" % e)
+        return m, syn, dsyn
+
+    def he(self, tem, pg_top, z_km, pel_top=0.0):
+        nz = len(tem)
+        t = np.ascontiguousarray(tem, dtype=np.float32).copy()
+        pg = np.zeros(nz, np.float32)
+        pg[-1] = pg_top
+        rho = np.zeros(nz, np.float32)
+        pel = np.zeros(nz, np.float32)
+        pel[-1] = pel_top
+        mw = np.zeros(nz, np.float32)
+        nh = np.zeros(nz)
+        zz = np.ascontiguousarray(z_km, dtype=np.float64)
+        e = self.lib.fzc_he(nz, _ptr(t, _fp), _ptr(pg, _fp), _ptr(rho, _fp), _ptr(pel, _fp), _ptr(mw, _fp),
+                            _ptr(nh, _dp), _ptr(zz, _dp))
+        if e:
+            raise RuntimeError("fzc_he error %d" % e)
+        return pg, rho, pel, mw, nh
+
+
+class Ref:
+    """ctypes view of the reference Fortran modules (oracle/_ref/libfzref.so)."""
+
+    def __init__(self):
+        if not ref_available():
+            raise RuntimeError("oracle/_ref not built (needs the reference tree)")
+        self.lib = ctypes.CDLL(REF_LIB)
+        self.numw = 0
+
+    def setup(self, ls: _lines.LineSetup, resp=True, atmpar=(1, 1, 0, 1, 1, 1, 1, 0)):
+        n = len(ls.lines)
+        zn = np.array([l.zn for l in ls.lines], np.int32)
+        ion = np.array([l.ion for l in ls.lines], np.int32)
+        l0 = np.array([l.lambda0 for l in ls.lines])
+        lg = np.array([l.loggf for l in ls.lines])
+        ep = np.array([l.eplow for l in ls.lines])
+        sg = np.array([l.sigma for l in ls.lines])
+        al = np.array([l.alpha for l in ls.lines])
+        slj = np.array([[l.sl, l.ll, l.jl, l.su, l.lu, l.ju] for l in ls.lines])
+        nr = len(ls.regions)
+        nbl = np.array([len(r.lines) for r in ls.regions], np.int32)
+        blid = np.zeros((nr, 4), np.int32)
+        for i, r in enumerate(ls.regions):
+            blid[i, :len(r.lines)] = r.lines
+        nw = np.array([r.nwave for r in ls.regions], np.int32)
+        wi = np.array([r.wave_start for r in ls.regions])
+        dw = np.array([r.wave_step for r in ls.regions])
+        ap = np.array(atmpar, np.int32)
+        self._keep = (zn, ion, l0, lg, ep, sg, al, slj, nbl, blid, nw, wi, dw, ap)
+        f = self.lib.fzref_setup
+        f.argtypes = [ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, _ip, _ip, _ip,
+                      _dp, _dp, ctypes.c_int, _ip]
+        f(n, _ptr(zn, _ip), _ptr(ion, _ip), _ptr(l0, _dp), _ptr(lg, _dp), _ptr(ep, _dp), _ptr(sg, _dp),
+          _ptr(al, _dp), _ptr(slj, _dp), nr, _ptr(nbl, _ip), _ptr(blid, _ip), _ptr(nw, _ip), _ptr(wi, _dp),
+          _ptr(dw, _dp), int(resp), _ptr(ap, _ip))
+        self.numw = int(nw.sum())
+        self.resp = bool(resp)
+        return self.numw
+
+    def partition(self, elem, T):
+        u = np.zeros(3)
+        du = np.zeros(3)
+        f = self.lib.fzref_partition
+        f.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        f(elem, T, _ptr(u, _dp), _ptr(du, _dp))
+        return u, du
+
+    def atomdata(self):
+        xi, xii, ab, m = (np.zeros(92) for _ in range(4))
+        den = np.zeros(2)
+        self.lib.fzref_atomdata(*[_ptr(a, _dp) for a in (xi, xii, ab, m, den)])
+        return xi, xii, ab, m, den
+
+    def eos(self, T, P, ne=0.0):
+        t = ctypes.c_float(T)
+        p = ctypes.c_float(P)
+        ne_ = ctypes.c_double(ne)
+        out = np.zeros(5)
+        d = np.zeros(25)
+        self.lib.fzref_eos(ctypes.byref(t), ctypes.byref(p), ctypes.byref(ne_), _ptr(out, _dp), _ptr(d, _dp))
+        return 0, out, d
+
+    def opac_cont(self, T, lam, ne, nh):
+        t = ctypes.c_float(T)
+        kc = ctypes.c_double()
+        d = np.zeros(25)
+        f = self.lib.fzref_opac_cont
+        f.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                      _dp, _dp]
+        f(ctypes.byref(t), lam, ne, nh, ctypes.byref(kc), _ptr(d, _dp))
+        return 0, kc.value, d
+
+    def opac_line(self, line, T, ne, nh):
+        t = ctypes.c_float(T)
+        kl = ctypes.c_double()
+        d = np.zeros(25)
+        f = self.lib.fzref_opac_line
+        f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_double, ctypes.c_double, _dp, _dp]
+        f(line, ctypes.byref(t), ne, nh, ctypes.byref(kl), _ptr(d, _dp))
+        return 0, kl.value, d
+
+    def damping(self, line, T, nh):
+        t = ctypes.c_float(T)
+        dm = ctypes.c_double()
+        d = np.zeros(25)
+        f = self.lib.fzref_damping
+        f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_double, _dp, _dp]
+        f(line, ctypes.byref(t), nh, ctypes.byref(dm), _ptr(d, _dp))
+        return dm.value, d
+
+    def voigt(self, a, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        h = np.zeros_like(v)
+        fo = np.zeros_like(v)
+        f = self.lib.fzref_voigt
+        f.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp]
+        f(len(v), a, _ptr(v, _dp), _ptr(h, _dp), _ptr(fo, _dp))
+        return h, fo
+
+    def zeeman(self, line):
+        s = np.zeros(64)
+        st = np.zeros(64)
+        ty = np.zeros(64, dtype=np.int32)
+        n = ctypes.c_int32()
+        f = self.lib.fzref_zeeman
+        f.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int32), _dp, _dp, _ip]
+        f(line, ctypes.byref(n), _ptr(s, _dp), _ptr(st, _dp), _ptr(ty, _ip))
+        return s[:n.value], st[:n.value], ty[:n.value]
+
+    def gain(self):
+        g = np.zeros((self.numw, 4), dtype=np.float32)
+        self.lib.fzref_gain(_ptr(g, _fp))
+        return g.T.copy()
+
+    def set_gain(self, g):
+        g = np.ascontiguousarray(np.asarray(g, dtype=np.float32).T)
+        self.lib.fzref_set_gain(_ptr(g, _fp))
+
+    def forward1d(self, model):
+        m = np.asfortranarray(np.asarray(model, dtype=np.float32).T.copy())   # (nz,13) Fortran
+        nz = m.shape[0]
+        syn = np.zeros((self.numw, 4))
+        dsyn = np.zeros((nz, self.numw, 8, 4))
+        kc5 = np.zeros(nz)
+        f = self.lib.fzref_forward1d
+        f.argtypes = [ctypes.c_int, _fp, _dp, _dp, _dp]
+        f(nz, _ptr(m, _fp), _ptr(syn, _dp), _ptr(dsyn, _dp), _ptr(kc5, _dp))
+        return np.ascontiguousarray(m.T), syn.T.copy(), (dsyn if self.resp else None), kc5
+
+    def he(self, tem, pg_top, z_km, pel_top=0.0):
+        nz = len(tem)
+        t = np.ascontiguousarray(tem, dtype=np.float32).copy()
+        pg = np.zeros(nz, np.float32)
+        pg[-1] = pg_top
+        rho = np.zeros(nz, np.float32)
+        pel = np.zeros(nz, np.float32)
+        pel[-1] = pel_top
+        mw = np.zeros(nz, np.float32)
+        nh = np.zeros(nz)
+        zz = np.ascontiguousarray(z_km, dtype=np.float64).copy()
+        f = self.lib.fzref_he
+        f.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
+        f(nz, _ptr(t, _fp), _ptr(pg, _fp), _ptr(rho, _fp), _ptr(pel, _fp), _ptr(mw, _fp), _ptr(nh, _dp),
+          _ptr(zz, _dp))
+        return pg, rho, pel, mw, nh

@@ -1,0 +1,73 @@
+/* FP64 CPU restatement of the FIRTEZ-dz per-column forward + response-function
+ * path.  TEST INFRASTRUCTURE ONLY (the oracle): tests, the smoke check and the
+ * CPU-baseline leg of the benchmark call it as the checker; the product path
+ * never links it. */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FZC_MAXLINES 16
+#define FZC_MAXREG 8
+#define FZC_MAXBLEND 4
+#define FZC_MAXCOMP 64
+
+/* line table row: forward_param.f90:16-20 (LINE_ZN, LINE_ION, LINE_L0, LOGGF,
+ * EPLOW, ALPHA, SIGMA, SL..JU) */
+typedef struct {
+  int32_t zn, ion;
+  double lambda0, loggf, eplow, sigma, alpha;
+  double sl, ll, jl, su, lu, ju;
+} fzc_line;
+
+/* spectral region: one LINES: row (input_file.f90:253-367) */
+typedef struct {
+  int32_t nblend;            /* 1 + number of blends */
+  int32_t line[FZC_MAXBLEND];/* 0-based line-table indices; [0] = main line */
+  int32_t nwave;
+  double wave_start, wave_step;   /* mA relative to the main line */
+} fzc_region;
+
+/* Configure lines/regions; resp != 0 turns the response functions on;
+ * atmpar[8] mirrors INV_ATMPAR (T,Pg,rho,Bx,By,Bz,vz,P0). Returns numw. */
+int fzc_setup(const fzc_line* lines, int nlines, const fzc_region* regs, int nreg,
+              int resp, const int32_t* atmpar);
+int fzc_numw(void);
+void fzc_wave(double* wave);
+
+/* atomic data and partition functions (atom_database.f90) */
+void fzc_atomdata(double* xi, double* xii, double* ab, double* m, double* deno);
+void fzc_partition(int elem, double T, double* u, double* du);
+
+/* equation of state at one height; out = NH, NE, RHO, MU, PEL; d = 25 DERIVVAR */
+int fzc_eos(float T, float P, double* ne_inout, double* out, double* d);
+int fzc_opac_cont(float T, double lam, double ne, double nh, double* kc, double* d);
+int fzc_opac_line(int line, float T, double ne, double nh, double* kl, double* d);
+int fzc_damping(int line, float T, double nh, double* damp, double* d);
+void fzc_voigt(int n, double a, const double* v, double* h, double* f);
+int fzc_zeeman(int line, double* shift, double* strength, int32_t* type);
+
+/* GAIN1D from the HSRA continuum (normalization.f90:696-754) */
+int fzc_gain(float* gain);
+void fzc_set_gain(const float* gain);
+
+/* One column (forward.f90:857-1029, HYDROSTATIC=no):
+ * model[13*nz] column-major MODEL(NZ,13), updated in place;
+ * syn[4*numw]  = SYN1D*GAIN1D;
+ * dsyn[4*8*numw*nz] = propagated DSYN1D*GAIN1D (only if resp);
+ * kc5[nz]. Returns 0 or an error code. */
+int fzc_forward1d(int nz, float* model, double* syn, double* dsyn, double* kc5);
+
+/* ncol independent columns, OpenMP over columns (nthreads<=0: default). */
+int fzc_forward_batch(int ncol, int nz, float* models, double* syn, double* dsyn,
+                      int nthreads);
+
+/* hydrostatic re-integration from the top (hydrostatic_tools.f90:150-222) */
+int fzc_he(int nz, float* tem, float* pg, float* rho, float* pel, float* mw,
+           double* nh, const double* zz);
+
+#ifdef __cplusplus
+}
+#endif
