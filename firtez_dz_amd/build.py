@@ -1,0 +1,50 @@
+"""Build the HIP library (gfx950) in-tree: firtez_dz_amd/libfirtez_gpu.so.
+
+  python -m firtez_dz_amd.build
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libfirtez_gpu.so")
+SOURCES = [os.path.join(HERE, "csrc", "fz_gpu.hip")]
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # no packed FP32 instructions anywhere in the device code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def build(verbose: bool = False) -> str:
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, *SOURCES, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd))
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the iso_c_binding module (checks that the Fortran boundary builds)."""
+    src = os.path.join(HERE, "fortran", "firtez_gpu_mod.f90")
+    out = os.path.join(HERE, "fortran", "firtez_gpu_mod.o")
+    cmd = ["/opt/rocm/bin/amdflang", "-c", "-O2", "-fPIC", "-J", os.path.join(HERE, "fortran"), src, "-o", out]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    return out
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,1013 @@
+// MI355X (gfx950) implementation of the FIRTEZ-dz per-column forward synthesis
+// and analytic response functions, behind the C ABI of include/firtez_gpu.h.
+//
+// Kernels
+//   micro_kernel : one thread per column, heights bottom->top (the electron
+//                  density warm-starts from the height below, as in FORWARD1D,
+//                  forward.f90:911-960).  FP64: EOS + implicit derivatives,
+//                  continuum/line opacity, damping.  Writes a per-height
+//                  record [field][z][column] (columns fastest).
+//   rt_kernel<RF> : one thread per (column, wavelength).  Pass 1 bottom->top:
+//                  propagation matrix, evolution operator, formal solution
+//                  I_k = (1-O_k) S_k + O_k I_{k-1} (rtesolver.f90:163-265);
+//                  the incoming I of every height goes to a scratch slab.
+//                  Pass 2 top->bottom (response functions): recomputes the
+//                  propagation matrix with derivatives, the local response
+//                  dO(I_in - S) + (1-O) dS, and propagates it to the top with
+//                  the running product O_nz...O_{k+1} (forward.f90:511-548).
+//   hsra_kernel / gain_kernel : continuum normalisation (normalization.f90).
+//   he_kernel    : hydrostatic RK4 from the top (hydrostatic_tools.f90:150-396).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/firtez_gpu.h"
+#include "fz_micro.hpp"
+#include "fz_rt.hpp"
+
+namespace fz {
+
+constexpr int MAXL = 16, MAXR = 8, MAXS = 32, MAXC = 1024, MAXNZ = 512;
+constexpr int NATM = 12;
+enum { A_T = 0, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VX, A_VY, A_VZ, A_TAU };
+enum { F_FLAG = 0, F_KC5 = 1, F_REG = 2 };
+
+struct Cfg {
+  int nlines, nreg, nslot, numw, nz, nfield;
+  LineConst line[MAXL];
+  float dopfac[MAXL];   // Doppler width (mA) = dopfac * sqrt(T)
+  float vfac[MAXL];     // Doppler shift (mA) = vfac * v_z   (absorption_matrix.f90:1896)
+  int reg_nblend[MAXR], reg_slot0[MAXR], reg_pix0[MAXR], reg_nwave[MAXR];
+  double reg_wstart[MAXR], reg_wstep[MAXR], reg_l0main[MAXR];
+  int slot_line[MAXS];
+  float slot_bldiff[MAXS];
+  int comp0[MAXL], ntype[MAXL][3];   // components sorted sigma_r, pi, sigma_b
+  float comp_shift[MAXC], comp_str[MAXC];
+  float dz_cm[MAXNZ];
+  double zz[MAXNZ];    // heights in km, rounded through REAL(4) like MODEL(:,13)
+};
+__constant__ Cfg c_cfg;
+
+__device__ __forceinline__ int fslot(int s, int q) { return F_REG + 4 * c_cfg.nreg + 8 * s + q; }
+__device__ __forceinline__ int freg(int r, int q) { return F_REG + 4 * r + q; }
+
+// ---------------------------------------------------------------- microphysics
+__global__ void __launch_bounds__(64) micro_kernel(int ncol, float* __restrict__ atm, float* __restrict__ micro,
+                                                   int32_t* __restrict__ status, int der) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int nz = c_cfg.nz;
+  const long cs = ncol;                       // stride between heights
+  const long ps = (long)nz * ncol;            // stride between parameters / fields
+  auto A = [&](int p, int k) -> float& { return atm[p * ps + k * cs + col]; };
+  auto M = [&](int f, int k) -> float& { return micro[f * ps + k * cs + col]; };
+  Saha s;
+  Deriv d = {};
+  double ne = 0.0;
+  int err = ERR_OK;
+  for (int k = 0; k < nz; ++k) {
+    const double T = (double)A(A_T, k), P = (double)A(A_PG, k);
+    saha_init(s, T, der != 0);
+    double nh, pel, dens, mu;
+    err = get_rho(s, T, P, ne, nh, pel, dens, mu);
+    if (err) break;
+    const float rhof = (float)dens;
+    A(A_RHO, k) = rhof;
+    A(A_MU, k) = (float)mu;
+    A(A_PEL, k) = (float)pel;
+    if (der) eos_derivatives(s, T, P, (double)rhof, ne, d);
+    double kc[4];
+    err = opac_cont(T, 5000.0, ne, nh, d, false, kc);
+    if (err) break;
+    const double kc5 = kc[0] / nh * (1.0 / MAMU);
+    const bool bnd = (k == 0) || (kc5 >= 2.0e10);     // forward.f90:884,931,971
+    M(F_FLAG, k) = bnd ? 1.0f : 0.0f;
+    M(F_KC5, k) = (float)kc5;
+    if (bnd) continue;
+    for (int r = 0; r < c_cfg.nreg && !err; ++r) {
+      err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, der != 0, kc);
+      for (int q = 0; q < 4; ++q) M(freg(r, q), k) = (float)kc[q];
+      for (int b = 0; b < c_cfg.reg_nblend[r] && !err; ++b) {
+        const int sl = c_cfg.reg_slot0[r] + b;
+        const LineConst& L = c_cfg.line[c_cfg.slot_line[sl]];
+        double kl[4] = {0, 0, 0, 0}, dm[4] = {0, 0, 0, 0};
+        err = opac_line(L, T, ne, nh, s, d, der != 0, kl);
+        damping(L, T, nh, d, der != 0, dm);
+        M(fslot(sl, 0), k) = (float)kl[0];
+        M(fslot(sl, 1), k) = (float)dm[0];
+        M(fslot(sl, 2), k) = (float)kl[1];
+        M(fslot(sl, 3), k) = (float)kl[2];
+        M(fslot(sl, 4), k) = (float)kl[3];
+        M(fslot(sl, 5), k) = (float)dm[1];
+        M(fslot(sl, 6), k) = (float)dm[2];
+        M(fslot(sl, 7), k) = (float)dm[3];
+      }
+    }
+    if (err) break;
+  }
+  if (!err) {
+    // log tau_5000 by the REAL(4) trapezoid of forward.f90:1000-1006 (z in km)
+    float tau = 1e-9f;
+    A(A_TAU, nz - 1) = log10f(tau);
+    for (int k = nz - 2; k >= 0; --k) {
+      const float a = (M(F_KC5, k) * A(A_RHO, k) + M(F_KC5, k + 1) * A(A_RHO, k + 1)) / 2.0f;
+      tau = (float)((double)tau + (double)a * ((c_cfg.zz[k + 1] - c_cfg.zz[k]) * 1000.0 * 100.0));
+      A(A_TAU, k) = log10f(tau);
+    }
+  }
+  if (status) status[col] = err;
+}
+
+// ---------------------------------------------------------------- radiative transfer
+struct RtArgs {
+  int ncol;          // columns in the whole cube (strides of atm/micro)
+  int col0;          // first column of this launch
+  int ntile;         // columns in this launch
+  const float* atm;
+  const float* micro;
+  const float* gain; // [numw][4]
+  const int32_t* status;
+  float* stokes;     // [ncol][4][numw]
+  float* jac;        // [ncol][np][nz][4][numw]
+  float* scratch;    // [nz][4][ntile*numw]
+};
+
+enum { D_T = 0, D_P, D_R, D_V, D_B, D_G, D_F, NDIR };
+
+struct Geo {
+  float B, cg, sg, sgf, cgf, c2p, s2p;
+};
+
+__device__ __forceinline__ Geo geometry(float bx, float by, float bz) {
+  // absorption_matrix.f90:1800-1871
+  Geo g;
+  const float bt2 = bx * bx + by * by, b2 = bt2 + bz * bz;
+  g.B = sqrtf(b2);
+  if (g.B > 0.f) {
+    g.cg = bz / g.B;
+    g.sg = sqrtf(bt2) / g.B;
+    g.cgf = 1.f + bz * bz / b2;
+    g.sgf = 1.f - bz * bz / b2;
+  } else {
+    g.cg = 0.f; g.sg = 0.f; g.cgf = 1.f; g.sgf = 1.f;
+  }
+  if (bt2 > 0.f) {
+    g.c2p = (bx * bx - by * by) / bt2;
+    g.s2p = 2.f * bx * by / bt2;
+  } else {
+    g.c2p = 0.f; g.s2p = 0.f;
+  }
+  return g;
+}
+
+// profile combination of the three component types (sigma_r, pi, sigma_b)
+// into (I,Q,U,V) (absorption_matrix.f90:1946-2230)
+struct Lin {
+  float I, Q, U, V;
+};
+__device__ __forceinline__ Lin lin(const Geo& g, float xr, float xp, float xb) {
+  const float qb = (0.5f * xp - 0.25f * (xr + xb)) * g.sgf;
+  return {0.5f * g.sgf * xp + 0.25f * g.cgf * (xr + xb), qb * g.c2p, qb * g.s2p, 0.5f * g.cg * (xr - xb)};
+}
+__device__ __forceinline__ Lin lin_dgamma(const Geo& g, float xr, float xp, float xb) {
+  const float dsgf = 2.f * g.sg * g.cg;   // d sin^2(gamma) / d gamma
+  const float qb = (0.5f * xp - 0.25f * (xr + xb)) * dsgf;
+  return {0.5f * dsgf * xp - 0.25f * dsgf * (xr + xb), qb * g.c2p, qb * g.s2p, -0.5f * g.sg * (xr - xb)};
+}
+__device__ __forceinline__ Lin lin_dphi(const Geo& g, float xr, float xp, float xb) {
+  const float qb = (0.5f * xp - 0.25f * (xr + xb)) * g.sgf;
+  return {0.f, -2.f * qb * g.s2p, 2.f * qb * g.c2p, 0.f};
+}
+
+struct Layer {
+  float T, vz;
+  Geo g;
+};
+
+// propagation matrix without derivatives
+__device__ __forceinline__ void absorb(const RtArgs& a, int col, int k, int r, float wave, const Layer& ly,
+                                       float x7[7]) {
+  const long ps = (long)c_cfg.nz * a.ncol, cs = a.ncol;
+  const float* mic = a.micro + k * cs + col;
+  x7[0] = mic[freg(r, 0) * ps];
+  for (int c = 1; c < 7; ++c) x7[c] = 0.f;
+  const float sqT = sqrtf(ly.T);
+  for (int b = 0; b < c_cfg.reg_nblend[r]; ++b) {
+    const int sl = c_cfg.reg_slot0[r] + b, li = c_cfg.slot_line[sl];
+    const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
+    const float idld = 1.0f / (c_cfg.dopfac[li] * sqT);
+    const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
+    const float bsc = ly.g.B * idld;
+    float hs[3], fs[3];
+    int m = c_cfg.comp0[li];
+    for (int t = 0; t < 3; ++t) {
+      float h = 0.f, f = 0.f;
+      const int n = c_cfg.ntype[li][t];
+      for (int j = 0; j < n; ++j, ++m) {
+        float H, F;
+        voigt(dmp, base + c_cfg.comp_shift[m] * bsc, H, F);
+        h += c_cfg.comp_str[m] * H;
+        f += c_cfg.comp_str[m] * F;
+      }
+      hs[t] = h;
+      fs[t] = f;
+    }
+    const Lin p = lin(ly.g, hs[0], hs[1], hs[2]), q = lin(ly.g, fs[0], fs[1], fs[2]);
+    x7[0] += kl * p.I;
+    x7[1] += kl * p.Q;
+    x7[2] += kl * p.U;
+    x7[3] += kl * p.V;
+    x7[4] += kl * q.Q;
+    x7[5] += kl * q.U;
+    x7[6] += kl * q.V;
+  }
+}
+
+// propagation matrix and its derivatives along the 7 internal directions
+// T|Pg, Pg|T, rho|T, v_z, |B|, inclination, azimuth (absorption_matrix.f90:1717-2642)
+__device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int r, float wave, const Layer& ly,
+                                           float x7[7], float dx[NDIR][7]) {
+  const long ps = (long)c_cfg.nz * a.ncol, cs = a.ncol;
+  const float* mic = a.micro + k * cs + col;
+  x7[0] = mic[freg(r, 0) * ps];
+  for (int c = 1; c < 7; ++c) x7[c] = 0.f;
+  for (int i = 0; i < NDIR; ++i)
+    for (int c = 0; c < 7; ++c) dx[i][c] = 0.f;
+  dx[D_T][0] = mic[freg(r, 1) * ps];
+  dx[D_P][0] = mic[freg(r, 2) * ps];
+  dx[D_R][0] = mic[freg(r, 3) * ps];
+  const float sqT = sqrtf(ly.T);
+  const float isp = 0.564189583547756f;   // 1/sqrt(pi)
+  for (int b = 0; b < c_cfg.reg_nblend[r]; ++b) {
+    const int sl = c_cfg.reg_slot0[r] + b, li = c_cfg.slot_line[sl];
+    const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
+    const float dkl[3] = {mic[fslot(sl, 2) * ps], mic[fslot(sl, 3) * ps], mic[fslot(sl, 4) * ps]};
+    const float dda[3] = {mic[fslot(sl, 5) * ps], mic[fslot(sl, 6) * ps], mic[fslot(sl, 7) * ps]};
+    const float idld = 1.0f / (c_cfg.dopfac[li] * sqT);
+    const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
+    const float bsc = ly.g.B * idld;
+    // per component type: H, F, sum dH/du, dH/da, u*dH/du, u*dH/da, shift*dH/du, shift*dH/da
+    float S[8][3];
+    int m = c_cfg.comp0[li];
+    for (int t = 0; t < 3; ++t) {
+      float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const int n = c_cfg.ntype[li][t];
+      for (int j = 0; j < n; ++j, ++m) {
+        const float sh = c_cfg.comp_shift[m], st = c_cfg.comp_str[m];
+        const float u = base + sh * bsc;
+        float H, F;
+        voigt(dmp, u, H, F);
+        const float hu = 2.f * (dmp * F - u * H);            // dH/du
+        const float ha = 2.f * (dmp * H + u * F - isp);      // dH/da
+        acc[0] += st * H;
+        acc[1] += st * F;
+        acc[2] += st * hu;
+        acc[3] += st * ha;
+        acc[4] += st * hu * u;
+        acc[5] += st * ha * u;
+        acc[6] += st * hu * sh;
+        acc[7] += st * ha * sh;
+      }
+      for (int q = 0; q < 8; ++q) S[q][t] = acc[q];
+    }
+    const Geo& g = ly.g;
+    const Lin pH = lin(g, S[0][0], S[0][1], S[0][2]);
+    const Lin pF = lin(g, S[1][0], S[1][1], S[1][2]);
+    const Lin pHu = lin(g, S[2][0], S[2][1], S[2][2]);
+    const Lin pHa = lin(g, S[3][0], S[3][1], S[3][2]);
+    const Lin pHuu = lin(g, S[4][0], S[4][1], S[4][2]);
+    const Lin pHau = lin(g, S[5][0], S[5][1], S[5][2]);
+    const Lin pHb = lin(g, S[6][0], S[6][1], S[6][2]);
+    const Lin pHab = lin(g, S[7][0], S[7][1], S[7][2]);
+    const Lin gH = lin_dgamma(g, S[0][0], S[0][1], S[0][2]);
+    const Lin gF = lin_dgamma(g, S[1][0], S[1][1], S[1][2]);
+    const Lin fH = lin_dphi(g, S[0][0], S[0][1], S[0][2]);
+    const Lin fF = lin_dphi(g, S[1][0], S[1][1], S[1][2]);
+    // eta components use the H sums; rho components use the F sums, whose
+    // derivatives are dF/du = -dH/da and dF/da = dH/du for w = H + iF
+    const float e7[7] = {pH.I, pH.Q, pH.U, pH.V, pF.Q, pF.U, pF.V};
+    const float a7[7] = {pHa.I, pHa.Q, pHa.U, pHa.V, pHu.Q, pHu.U, pHu.V};      // d/da
+    const float u7[7] = {pHu.I, pHu.Q, pHu.U, pHu.V, -pHa.Q, -pHa.U, -pHa.V};  // d/du
+    const float uu7[7] = {pHuu.I, pHuu.Q, pHuu.U, pHuu.V, -pHau.Q, -pHau.U, -pHau.V};
+    const float b7[7] = {pHb.I, pHb.Q, pHb.U, pHb.V, -pHab.Q, -pHab.U, -pHab.V};
+    const float g7[7] = {gH.I, gH.Q, gH.U, gH.V, gF.Q, gF.U, gF.V};
+    const float f7[7] = {fH.I, fH.Q, fH.U, fH.V, fF.Q, fF.U, fF.V};
+    const float dudT = -0.5f / ly.T;                     // du/dT = -u/(2T)
+    const float dudv = c_cfg.vfac[li] * idld;
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      x7[c] += kl * e7[c];
+      dx[D_T][c] += dkl[0] * e7[c] + kl * (dudT * uu7[c] + dda[0] * a7[c]);
+      dx[D_P][c] += dkl[1] * e7[c] + kl * dda[1] * a7[c];
+      dx[D_R][c] += dkl[2] * e7[c] + kl * dda[2] * a7[c];
+      dx[D_V][c] += kl * dudv * u7[c];
+      dx[D_B][c] += kl * idld * b7[c];
+      dx[D_G][c] += kl * g7[c];
+      dx[D_F][c] += kl * f7[c];
+    }
+  }
+}
+
+__device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
+  const long ps = (long)c_cfg.nz * a.ncol, cs = a.ncol;
+  const float* at = a.atm + k * cs + col;
+  Layer ly;
+  ly.T = at[A_T * ps];
+  ly.vz = at[A_VZ * ps];
+  ly.g = geometry(at[A_BX * ps], at[A_BY * ps], at[A_BZ * ps]);
+  return ly;
+}
+
+template <int RF>
+__global__ void __launch_bounds__(256) rt_kernel(RtArgs a) {
+  constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : 0);
+  const int numw = c_cfg.numw, nz = c_cfg.nz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nthr = (long)a.ntile * numw;
+  if (t >= nthr) return;
+  const int ct = (int)(t / numw);
+  const int l = (int)(t - (long)ct * numw);
+  const int col = a.col0 + ct;
+  if (a.status && a.status[col] != 0) {
+    for (int s = 0; s < 4; ++s) a.stokes[((long)col * 4 + s) * numw + l] = __builtin_nanf("");
+    return;
+  }
+  int r = 0;
+  while (r + 1 < c_cfg.nreg && l >= c_cfg.reg_pix0[r + 1]) ++r;
+  const double wd = c_cfg.reg_wstart[r] + c_cfg.reg_wstep[r] * (double)(l - c_cfg.reg_pix0[r]);
+  const float wave = (float)wd;
+  // Planck function constants (rtesolver.f90:1008-1035): S = c1 / (exp(c2/T) - 1)
+  const double lam = (c_cfg.reg_l0main[r] + wd / 1.0e3) * 1.0e-8;
+  const float c1 = (float)(2.0 * HPLA * LIGHT * LIGHT / (lam * lam * lam * lam * lam));
+  const float c2 = (float)(HPLA * LIGHT / (lam * KBOL));
+  const long ps = (long)nz * a.ncol, cs = a.ncol;
+  float I[4] = {0.f, 0.f, 0.f, 0.f};
+  float* scr = a.scratch + t;
+  const long sstr = nthr;
+  for (int k = 0; k < nz; ++k) {
+    const Layer ly = load_layer(a, col, k);
+    const float S = c1 / (expf(c2 / ly.T) - 1.0f);
+    if (a.micro[F_FLAG * ps + k * cs + col] != 0.f) {
+      I[0] = S; I[1] = 0.f; I[2] = 0.f; I[3] = 0.f;
+      continue;
+    }
+    float x7[7];
+    absorb(a, col, k, r, wave, ly, x7);
+    Evol E;
+    evol_build(x7, c_cfg.dz_cm[k], E);
+    if (RF) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) scr[(long)(k * 4 + s) * sstr] = I[s];
+    }
+    float In[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      In[i] = ((i == 0 ? 1.f : 0.f) - E.O[i][0]) * S + E.O[i][0] * I[0] + E.O[i][1] * I[1] + E.O[i][2] * I[2] +
+              E.O[i][3] * I[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) I[i] = In[i];
+  }
+  float gn[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    gn[s] = a.gain[l * 4 + s];
+    a.stokes[((long)col * 4 + s) * numw + l] = I[s] * gn[s];
+  }
+  if constexpr (RF != 0) {
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (i == j) ? 1.f : 0.f;
+    float* jb = a.jac + (long)col * NP * nz * 4 * numw + l;
+    for (int k = nz - 1; k >= 0; --k) {
+      float outv[NP][4];
+      const Layer ly = load_layer(a, col, k);
+      if (a.micro[F_FLAG * ps + k * cs + col] != 0.f) {
+        // boundary layer: no local response; layers below cannot reach the top
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+          for (int s = 0; s < 4; ++s) outv[p][s] = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+      } else {
+        const float ex = expf(c2 / ly.T);
+        const float S = c1 / (ex - 1.0f);
+        const float dS = c1 * c2 * ex / (ly.T * ly.T * (ex - 1.f) * (ex - 1.f));
+        float x7[7], dx[NDIR][7];
+        absorb_der(a, col, k, r, wave, ly, x7, dx);
+        Evol E;
+        const float dz = c_cfg.dz_cm[k];
+        evol_build(x7, dz, E);
+        float v[4], m4v[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[s] = scr[(long)(k * 4 + s) * sstr] - (s == 0 ? S : 0.f);
+        m4mul(E.e, E.r, E.s0, E.c, v, m4v);
+        float loc[NDIR][4];
+        constexpr int dirs_sph[5] = {D_T, D_B, D_G, D_F, D_V};
+        constexpr int dirs_car[7] = {D_T, D_P, D_R, D_B, D_G, D_F, D_V};
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          const int dd = RF == FZ_RF_SPHERICAL ? dirs_sph[q] : dirs_car[q];
+          evol_dir(E, dx[dd], dz, v, m4v, loc[dd]);
+        }
+        // source-function term (1 - O) e0 dS/dT for T
+#pragma unroll
+        for (int i = 0; i < 4; ++i) loc[D_T][i] += ((i == 0 ? 1.f : 0.f) - E.O[i][0]) * dS;
+        float lp[NP][4];
+        if constexpr (RF == FZ_RF_SPHERICAL) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            lp[0][i] = loc[D_T][i];
+            lp[1][i] = loc[D_B][i];
+            lp[2][i] = loc[D_G][i];
+            lp[3][i] = loc[D_F][i];
+            lp[4][i] = loc[D_V][i];
+          }
+        } else {
+          // chain rule (|B|, gamma, phi) -> (Bx, By, Bz)
+          const float* at = a.atm + k * cs + col;
+          const float bx = at[A_BX * ps], by = at[A_BY * ps], bz = at[A_BZ * ps];
+          const float bt2 = bx * bx + by * by, bt = sqrtf(bt2), B = ly.g.B, B2 = B * B;
+          const float dB[3] = {bx / B, by / B, bz / B};
+          const float dG[3] = {bz * bx / (B2 * bt), bz * by / (B2 * bt), -bt / B2};
+          const float dF[3] = {-by / bt2, bx / bt2, 0.f};
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            lp[0][i] = loc[D_T][i];
+            lp[1][i] = loc[D_P][i];
+            lp[2][i] = loc[D_R][i];
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+              lp[3 + j][i] = dB[j] * loc[D_B][i] + dG[j] * loc[D_G][i] + dF[j] * loc[D_F][i];
+            lp[6][i] = loc[D_V][i];
+          }
+        }
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            outv[p][i] = acc[i][0] * lp[p][0] + acc[i][1] * lp[p][1] + acc[i][2] * lp[p][2] + acc[i][3] * lp[p][3];
+        // running product O_nz ... O_k
+        float nacc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            nacc[i][j] = acc[i][0] * E.O[0][j] + acc[i][1] * E.O[1][j] + acc[i][2] * E.O[2][j] + acc[i][3] * E.O[3][j];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = nacc[i][j];
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) jb[((long)(p * nz + k) * 4 + s) * numw] = outv[p][s] * gn[s];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- HSRA normalisation
+// Equation of state and continuum opacity along the embedded HSRA model
+// (normalization.f90:696-754); one thread, heights in order (warm start).
+__global__ void hsra_kernel(double* __restrict__ kc_out /*[387][nreg]*/, int32_t* __restrict__ err_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  Saha s;
+  Deriv d = {};
+  double ne = 0.0;
+  int err = 0;
+  for (int k = 0; k < FZ_HSRA_LEN && !err; ++k) {
+    const double T = (double)fz_hsra_t[k], P = (double)fz_hsra_pg[k];
+    saha_init(s, T, false);
+    double nh, pel, dens, mu;
+    err = get_rho(s, T, P, ne, nh, pel, dens, mu);
+    for (int r = 0; r < c_cfg.nreg && !err; ++r) {
+      double kc[4];
+      err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, false, kc);
+      kc_out[k * c_cfg.nreg + r] = kc[0];
+    }
+  }
+  *err_out = err;
+}
+
+// Continuum formal solution through HSRA per wavelength; every step is
+// rounded to REAL(4) like ANALYTICAL_SOLVER_HSRA (normalization.f90:466-480).
+// The source function uses the last region's central wavelength for every
+// pixel, which is what SOURCE_FUNCTION_HSRA's region test evaluates to
+// (normalization.f90:679-682).
+__global__ void gain_kernel(const double* __restrict__ kc, float* __restrict__ gain) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= c_cfg.numw) return;
+  int r = 0;
+  while (r + 1 < c_cfg.nreg && l >= c_cfg.reg_pix0[r + 1]) ++r;
+  const double wave = c_cfg.reg_wstart[r] + c_cfg.reg_wstep[r] * (double)(l - c_cfg.reg_pix0[r]);
+  const double lam = (c_cfg.reg_l0main[c_cfg.nreg - 1] + wave / 1.0e3) * 1.0e-8;
+  const double c1 = 2.0 * HPLA * LIGHT * LIGHT / (lam * lam * lam * lam * lam), c2 = HPLA * LIGHT / (lam * KBOL);
+  double syn = 0.0;
+  for (int k = 0; k < FZ_HSRA_LEN; ++k) {
+    const double sf = (double)(float)(c1 * (1.0 / (exp(c2 / (double)fz_hsra_t[k]) - 1.0)));
+    if (k == 0) {
+      syn = sf;
+      continue;
+    }
+    double eta = kc[k * c_cfg.nreg + r];
+    const bool imat = (eta == eta + 1.0);
+    if (imat) eta = 1.0e15;
+    const double dz = (k == FZ_HSRA_LEN - 1) ? 1.0e5 * fabs((double)fz_hsra_z[k] - (double)fz_hsra_z[k - 1])
+                                             : 1.0e5 * fabs((double)fz_hsra_z[k + 1] - (double)fz_hsra_z[k]);
+    const double ev = imat ? 1.0 : exp(-eta * dz);
+    syn = (double)(float)((1.0 - ev) * sf + ev * syn);
+  }
+  const float g = (float)(1.0 / syn);
+  gain[4 * l + 0] = g;
+  gain[4 * l + 1] = g;
+  gain[4 * l + 2] = g;
+  gain[4 * l + 3] = -g;   // V sign convention (normalization.f90:743,751)
+}
+
+// ---------------------------------------------------------------- hydrostatic equilibrium
+__device__ double he_rhs(Saha& s, double lnp, double tem, double& nel, int& err) {
+  double nh, pel, dens, mw;
+  saha_init(s, tem, false);
+  const int e = get_rho(s, tem, exp(lnp), nel, nh, pel, dens, mw);
+  if (e) err = e;
+  return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
+}
+
+__global__ void __launch_bounds__(64) he_kernel(int ncol, float* __restrict__ atm, int32_t* __restrict__ status) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  const int nz = c_cfg.nz;
+  const long cs = ncol, ps = (long)nz * ncol;
+  auto A = [&](int p, int k) -> float& { return atm[p * ps + k * cs + col]; };
+  Saha s;
+  int err = 0;
+  const int top = nz - 1;
+  double u0 = log((double)A(A_PG, top));
+  double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
+  double nh, pel, dens, mw;
+  saha_init(s, (double)A(A_T, top), false);
+  err = get_rho(s, (double)A(A_T, top), exp(u0), nel, nh, pel, dens, mw);
+  A(A_PG, top) = (float)exp(u0);
+  A(A_RHO, top) = (float)dens;
+  A(A_PEL, top) = (float)pel;
+  A(A_MU, top) = (float)mw;
+  double t0 = c_cfg.zz[top] * 1000.0 * 100.0;
+  for (int k = nz - 2; k >= 0 && !err; --k) {
+    // RK4_LOG (hydrostatic_tools.f90:350-396); z decreasing, dz < 0
+    const double dz = (c_cfg.zz[k] - c_cfg.zz[k + 1]) * 1000.0 * 100.0;
+    const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
+    const double k1 = he_rhs(s, u0, tm0, nel, err) * dz;
+    const double it1 = t0 + dz / 2.0;
+    const double tmid = tm0 / dz * (t0 + dz - it1) + tm1 / dz * (it1 - t0);
+    const double k2 = he_rhs(s, u0 + k1 / 2.0, tmid, nel, err) * dz;
+    const double k3 = he_rhs(s, u0 + k2 / 2.0, tmid, nel, err) * dz;
+    const double k4 = he_rhs(s, u0 + k3, tm1, nel, err) * dz;
+    const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+    saha_init(s, tm1, false);
+    const int e = get_rho(s, tm1, exp(u1), nel, nh, pel, dens, mw);
+    if (e) err = e;
+    A(A_PG, k) = (float)exp(u1);
+    A(A_RHO, k) = (float)dens;
+    A(A_PEL, k) = (float)pel;
+    A(A_MU, k) = (float)mw;
+    u0 = u1;
+    t0 = t0 + dz;
+  }
+  if (status) status[col] = err;
+}
+
+// ---------------------------------------------------------------- layout transposes for FORWARD3D
+// Fortran cube (NZ,NY,NX) -> device SoA [z][col] with col = y + NY*x
+__global__ void cube_to_soa(const float* __restrict__ src, float* __restrict__ dst, int nz, int ncol) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nz * ncol) return;
+  const int col = (int)(i / nz), k = (int)(i - (long)col * nz);
+  dst[(long)k * ncol + col] = src[i];
+}
+__global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ dst, int nz, int ncol) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)nz * ncol) return;
+  const int col = (int)(i / nz), k = (int)(i - (long)col * nz);
+  dst[i] = src[(long)k * ncol + col];
+}
+
+}  // namespace fz
+
+// ==================================================================== C ABI
+using namespace fz;
+
+struct fz_ctx {
+  int device = 0;
+  bool configured = false;
+  bool have_gain = false;
+  Cfg cfg;
+  std::string err;
+  float* d_gain = nullptr;
+  float* d_micro = nullptr;
+  size_t micro_cap = 0;
+  float* d_scratch = nullptr;
+  size_t scratch_cap = 0;
+  int tile = 8192;
+  int block = 256;
+  bool prof = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  int rt_launches = 0;
+};
+
+// the context whose configuration is in a device's constant memory
+static const fz_ctx* g_owner[64];
+
+static int set_err(fz_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, x)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return set_err((c), FZ_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static int upload_cfg(fz_ctx* c) {
+  if (c->device < 64 && g_owner[c->device] == c) return FZ_OK;
+  HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &c->cfg, sizeof(Cfg)));
+  if (c->device < 64) g_owner[c->device] = c;
+  return FZ_OK;
+}
+
+extern "C" {
+
+int fz_create(fz_ctx** out, int device) {
+  if (!out) return FZ_EARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return FZ_EHIP;
+  fz_ctx* c = new fz_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return FZ_EHIP;
+  }
+  hipLaunchKernelGGL(fz_init_constants_kernel, dim3(1), dim3(1), 0, 0);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    delete c;
+    return FZ_EHIP;
+  }
+  *out = c;
+  return FZ_OK;
+}
+
+void fz_destroy(fz_ctx* c) {
+  if (!c) return;
+  if (c->device < 64 && g_owner[c->device] == c) g_owner[c->device] = nullptr;
+  (void)hipSetDevice(c->device);
+  if (c->d_gain) (void)hipFree(c->d_gain);
+  if (c->d_micro) (void)hipFree(c->d_micro);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  for (int i = 0; i < 3; ++i)
+    if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+  delete c;
+}
+
+const char* fz_last_error(const fz_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+// Zeeman pattern of one line (magsplit.f90:38-187), components grouped by type
+static int zeeman_host(const fz_line& L, std::vector<float>& shift, std::vector<float>& str, int ntype[3]) {
+  const int dj = (int)(L.ju - L.jl);
+  if (std::abs(dj) > 1) return -1;
+  double gl = 0.0, gu = 0.0;
+  if (L.jl != 0) gl = 1.5 + (L.sl * (L.sl + 1.0) - L.ll * (L.ll + 1.0)) / (2.0 * L.jl * (L.jl + 1.0));
+  if (L.ju != 0) gu = 1.5 + (L.su * (L.su + 1.0) - L.lu * (L.lu + 1.0)) / (2.0 * L.ju * (L.ju + 1.0));
+  std::vector<double> sh[3], st[3];
+  const int nml = (int)(2 * L.jl + 1), nmu = (int)(2 * L.ju + 1);
+  for (int j = 0; j < nml; ++j) {
+    const double ml = (double)j - L.jl;
+    for (int k = 0; k < nmu; ++k) {
+      const double mu = (double)k - L.ju;
+      const int dm = (int)(mu - ml);
+      if (std::abs(dm) > 1) continue;
+      const double jl = L.jl, ju = L.ju;
+      double s;
+      if (dj == -1) s = dm == -1 ? (jl + mu) * (ju + mu + 2.0) : (dm == 0 ? 2.0 * (jl * jl - mu * mu) : (jl - mu) * (ju - mu + 2.0));
+      else if (dj == 0) s = dm == -1 ? (ju - mu) * (ju + mu + 1.0) : (dm == 0 ? 2.0 * mu * mu : (ju + mu) * (ju - mu + 1.0));
+      else s = dm == -1 ? (ju - mu) * (jl - mu) : (dm == 0 ? 2.0 * (ju * ju - mu * mu) : (ju + mu) * (jl + mu));
+      // shift in mA/G rounded to REAL(4) as in the reference
+      sh[dm + 1].push_back((double)(float)(4.66685e-10 * (ml * gl - mu * gu) * L.lambda0 * L.lambda0));
+      st[dm + 1].push_back(s);
+    }
+  }
+  for (int t = 0; t < 3; ++t) {
+    double sum = 0.0;
+    for (double v : st[t]) sum += v;
+    ntype[t] = (int)st[t].size();
+    for (size_t i = 0; i < st[t].size(); ++i) {
+      shift.push_back((float)sh[t][i]);
+      str.push_back((float)(st[t][i] / sum));
+    }
+  }
+  return 0;
+}
+
+int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* regs, int nreg, int nz,
+                 const double* z_km) {
+  if (!c || !lines || !regs || !z_km) return FZ_EARG;
+  if (nlines < 1 || nlines > MAXL || nreg < 1 || nreg > MAXR || nz < 2 || nz > MAXNZ)
+    return set_err(c, FZ_ECONFIG, "configuration outside limits");
+  Cfg& g = c->cfg;
+  memset(&g, 0, sizeof(g));
+  g.nlines = nlines;
+  g.nreg = nreg;
+  g.nz = nz;
+  std::vector<float> shift, str;
+  double matom[FZ_NELEM];
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyFromSymbol(matom, HIP_SYMBOL(fz_matom), sizeof(matom)));
+  for (int i = 0; i < nlines; ++i) {
+    const fz_line& L = lines[i];
+    if (L.zn < 1 || L.zn > FZ_NELEM || L.ion < 1 || L.ion > 3) return set_err(c, FZ_EION, "bad element/ion");
+    g.line[i] = {L.zn - 1, L.ion, L.lambda0, L.loggf, L.eplow, L.sigma, L.alpha};
+    g.dopfac[i] = (float)(1.0e3 * L.lambda0 / LIGHT * sqrt(2.0 * KBOL / (matom[L.zn - 1] * MAMU)));
+    g.vfac[i] = (float)(-1.0e3 * L.lambda0 / LIGHT);
+    g.comp0[i] = (int)shift.size();
+    if (zeeman_host(L, shift, str, g.ntype[i]) != 0) return set_err(c, FZ_ECONFIG, "|Delta J| > 1");
+  }
+  if ((int)shift.size() > MAXC) return set_err(c, FZ_ECONFIG, "too many Zeeman components");
+  for (size_t i = 0; i < shift.size(); ++i) {
+    g.comp_shift[i] = shift[i];
+    g.comp_str[i] = str[i];
+  }
+  int numw = 0, ns = 0;
+  for (int r = 0; r < nreg; ++r) {
+    const fz_region& R = regs[r];
+    if (R.nblend < 1 || R.nblend > 4 || R.nwave < 1) return set_err(c, FZ_ECONFIG, "bad region");
+    g.reg_nblend[r] = R.nblend;
+    g.reg_slot0[r] = ns;
+    g.reg_pix0[r] = numw;
+    g.reg_nwave[r] = R.nwave;
+    g.reg_wstart[r] = R.wave_start;
+    g.reg_wstep[r] = R.wave_step;
+    g.reg_l0main[r] = lines[R.line[0]].lambda0;
+    for (int b = 0; b < R.nblend; ++b) {
+      if (R.line[b] < 0 || R.line[b] >= nlines || ns >= MAXS) return set_err(c, FZ_ECONFIG, "bad blend");
+      g.slot_line[ns] = R.line[b];
+      g.slot_bldiff[ns] = b == 0 ? 0.f : (float)(1.0e3 * (lines[R.line[b]].lambda0 - lines[R.line[0]].lambda0));
+      ++ns;
+    }
+    numw += R.nwave;
+  }
+  g.nslot = ns;
+  g.numw = numw;
+  g.nfield = F_REG + 4 * nreg + 8 * ns;
+  // layer thickness uses the spacing to the next height, the top one the
+  // spacing below (rtesolver.f90:439-443); z rounded to REAL(4) like MODEL(:,13)
+  for (int k = 0; k < nz; ++k) {
+    const double zk = (double)(float)z_km[k];
+    const double zo = (k == nz - 1) ? (double)(float)z_km[k - 1] : (double)(float)z_km[k + 1];
+    g.dz_cm[k] = (float)(1.0e5 * fabs(zo - zk));
+    g.zz[k] = zk;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &g, sizeof(Cfg)));
+  if (c->device < 64) g_owner[c->device] = c;
+  if (c->d_gain) (void)hipFree(c->d_gain);
+  HIPCHK(c, hipMalloc(&c->d_gain, sizeof(float) * 4 * numw));
+  std::vector<float> ones(4 * numw, 1.0f);
+  for (int l = 0; l < numw; ++l) ones[4 * l + 3] = -1.0f;
+  HIPCHK(c, hipMemcpy(c->d_gain, ones.data(), sizeof(float) * 4 * numw, hipMemcpyHostToDevice));
+  c->configured = true;
+  c->have_gain = false;
+  return FZ_OK;
+}
+
+int fz_numw(const fz_ctx* c) { return (c && c->configured) ? c->cfg.numw : -1; }
+
+int fz_set_gain(fz_ctx* c, const float* gain) {
+  if (!c || !c->configured || !gain) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy(c->d_gain, gain, sizeof(float) * 4 * c->cfg.numw, hipMemcpyHostToDevice));
+  c->have_gain = true;
+  return FZ_OK;
+}
+
+int fz_gain_hsra(fz_ctx* c, float* gain) {
+  if (!c || !c->configured) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int ru = upload_cfg(c);
+  if (ru) return ru;
+  double* kc = nullptr;
+  int32_t* derr = nullptr;
+  HIPCHK(c, hipMalloc(&kc, sizeof(double) * FZ_HSRA_LEN * c->cfg.nreg));
+  HIPCHK(c, hipMalloc(&derr, sizeof(int32_t)));
+  hipLaunchKernelGGL(hsra_kernel, dim3(1), dim3(64), 0, 0, kc, derr);
+  hipLaunchKernelGGL(gain_kernel, dim3((c->cfg.numw + 63) / 64), dim3(64), 0, 0, kc, c->d_gain);
+  int32_t e = 0;
+  HIPCHK(c, hipMemcpy(&e, derr, sizeof(int32_t), hipMemcpyDeviceToHost));
+  (void)hipFree(kc);
+  (void)hipFree(derr);
+  if (e) return set_err(c, e, "HSRA equation of state failed");
+  if (gain) HIPCHK(c, hipMemcpy(gain, c->d_gain, sizeof(float) * 4 * c->cfg.numw, hipMemcpyDeviceToHost));
+  c->have_gain = true;
+  return FZ_OK;
+}
+
+static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
+  if (*cap >= bytes) return FZ_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *cap = 0;
+  if (hipMalloc((void**)p, bytes) != hipSuccess) return set_err(c, FZ_ENOMEM, "device allocation failed");
+  *cap = bytes;
+  return FZ_OK;
+}
+
+int fz_set_tile(fz_ctx* c, int tile, int block) {
+  if (!c || tile < 1 || block < 64 || block > 1024 || block % 64) return FZ_EARG;
+  c->tile = tile;
+  c->block = block;
+  return FZ_OK;
+}
+
+int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac, int32_t* d_status,
+                   void* stream) {
+  if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
+  if (rf_mode != FZ_RF_NONE && !d_jac) return set_err(c, FZ_EARG, "jacobian output missing");
+  if (rf_mode < 0 || rf_mode > 2) return FZ_EARG;
+  const Cfg& g = c->cfg;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = upload_cfg(c);
+  if (r) return r;
+  r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
+  if (r) return r;
+  const int tile = std::min(c->tile, ncol);
+  if (rf_mode != FZ_RF_NONE) {
+    r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * 4 * tile * g.numw);
+    if (r) return r;
+  }
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
+  hipLaunchKernelGGL(micro_kernel, dim3((ncol + 63) / 64), dim3(64), 0, st, ncol, d_atm, c->d_micro, d_status,
+                     rf_mode != FZ_RF_NONE ? 1 : 0);
+  HIPCHK(c, hipGetLastError());
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
+  c->rt_launches = 0;
+  for (int c0 = 0; c0 < ncol; c0 += tile) {
+    RtArgs a;
+    a.ncol = ncol;
+    a.col0 = c0;
+    a.ntile = std::min(tile, ncol - c0);
+    a.atm = d_atm;
+    a.micro = c->d_micro;
+    a.gain = c->d_gain;
+    a.status = d_status;
+    a.stokes = d_stokes;
+    a.jac = d_jac;
+    a.scratch = c->d_scratch;
+    const long nthr = (long)a.ntile * g.numw;
+    const dim3 grid((unsigned)((nthr + c->block - 1) / c->block)), blk(c->block);
+    if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL(rt_kernel<FZ_RF_NONE>, grid, blk, 0, st, a);
+    else if (rf_mode == FZ_RF_SPHERICAL) hipLaunchKernelGGL(rt_kernel<FZ_RF_SPHERICAL>, grid, blk, 0, st, a);
+    else hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, grid, blk, 0, st, a);
+    HIPCHK(c, hipGetLastError());
+    c->rt_launches++;
+  }
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
+  return FZ_OK;
+}
+
+int fz_set_profiling(fz_ctx* c, int on) {
+  if (!c) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (on && !c->ev[0])
+    for (int i = 0; i < 3; ++i) HIPCHK(c, hipEventCreate(&c->ev[i]));
+  c->prof = on != 0;
+  return FZ_OK;
+}
+
+int fz_last_timing(fz_ctx* c, float* micro_ms, float* rt_ms, int* launches) {
+  if (!c || !c->prof) return FZ_EARG;
+  HIPCHK(c, hipEventSynchronize(c->ev[2]));
+  float a = 0.f, b = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+  HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  if (micro_ms) *micro_ms = a;
+  if (rt_ms) *rt_ms = b;
+  if (launches) *launches = c->rt_launches;
+  return FZ_OK;
+}
+
+int fz_hydrostatic_dev(fz_ctx* c, int ncol, float* d_atm, int32_t* d_status, void* stream) {
+  if (!c || !c->configured || ncol < 1 || !d_atm) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  int ru = upload_cfg(c);
+  if (ru) return ru;
+  hipLaunchKernelGGL(he_kernel, dim3((ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, ncol, d_atm, d_status);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
+                 const float* bx3d, const float* by3d, const float* bz3d, const float* vx3d, const float* vy3d,
+                 const float* vz3d, float* tau3d5, const int32_t* am_i_done, int rf_mode, double* syn3d, float* dsyn) {
+  if (!c || !c->configured || nx < 1 || ny < 1 || !syn3d) return FZ_EARG;
+  const Cfg& g = c->cfg;
+  const int nz = g.nz, numw = g.numw;
+  const int np = rf_mode == FZ_RF_SPHERICAL ? 5 : (rf_mode == FZ_RF_CARTESIAN ? 7 : 0);
+  if (np && !dsyn) return FZ_EARG;
+  // columns still to be computed (AM_I_DONE == 0, forward.f90:120-140)
+  std::vector<int> cols;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      if (!am_i_done || am_i_done[y + (size_t)ny * x] == 0) cols.push_back(y + ny * x);
+  const int n = (int)cols.size();
+  if (n == 0) return FZ_OK;
+  const float* cubes[NATM] = {tem3d, pg3d, rho3d, pel3d, mw3d, bx3d, by3d, bz3d, vx3d, vy3d, vz3d, tau3d5};
+  const size_t colbytes = sizeof(float) * nz;
+  std::vector<float> h(colbytes / sizeof(float) * n);
+  float *d_atm = nullptr, *d_tmp = nullptr, *d_st = nullptr, *d_jac = nullptr;
+  int32_t* d_status = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMalloc(&d_atm, colbytes * NATM * n));
+  HIPCHK(c, hipMalloc(&d_tmp, colbytes * n));
+  HIPCHK(c, hipMalloc(&d_st, sizeof(float) * 4 * numw * n));
+  HIPCHK(c, hipMalloc(&d_status, sizeof(int32_t) * n));
+  if (np) HIPCHK(c, hipMalloc(&d_jac, sizeof(float) * (size_t)np * nz * 4 * numw * n));
+  const unsigned tb = 256, tg = (unsigned)(((long)nz * n + tb - 1) / tb);
+  for (int p = 0; p < NATM; ++p) {
+    for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * nz], cubes[p] + (size_t)cols[i] * nz, colbytes);
+    HIPCHK(c, hipMemcpy(d_tmp, h.data(), colbytes * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(cube_to_soa, dim3(tg), dim3(tb), 0, 0, d_tmp, d_atm + (size_t)p * nz * n, nz, n);
+  }
+  int rc = fz_forward_dev(c, n, d_atm, rf_mode, d_st, d_jac, d_status, nullptr);
+  if (rc == FZ_OK) {
+    std::vector<int32_t> stat(n);
+    HIPCHK(c, hipMemcpy(stat.data(), d_status, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n && rc == FZ_OK; ++i)
+      if (stat[i]) rc = set_err(c, stat[i], "column " + std::to_string(cols[i]) + " failed");
+  }
+  if (rc == FZ_OK) {
+    // model updated in place: rho, Pe, mu, log tau5 (SPLIT_MODEL3D, forward.f90:1225-1270)
+    float* outs[NATM] = {nullptr, nullptr, rho3d, pel3d, mw3d, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tau3d5};
+    for (int p = 0; p < NATM; ++p) {
+      if (!outs[p]) continue;
+      hipLaunchKernelGGL(soa_to_cube, dim3(tg), dim3(tb), 0, 0, d_atm + (size_t)p * nz * n, d_tmp, nz, n);
+      HIPCHK(c, hipMemcpy(h.data(), d_tmp, colbytes * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) memcpy(outs[p] + (size_t)cols[i] * nz, &h[(size_t)i * nz], colbytes);
+    }
+    std::vector<float> st((size_t)4 * numw * n);
+    HIPCHK(c, hipMemcpy(st.data(), d_st, sizeof(float) * st.size(), hipMemcpyDeviceToHost));
+    const size_t nf = (size_t)4 * numw;
+    for (int i = 0; i < n; ++i)
+      for (size_t f = 0; f < nf; ++f) syn3d[(size_t)cols[i] * nf + f] = (double)st[(size_t)i * nf + f];
+    if (np) {
+      const size_t per = (size_t)np * nz * 4 * numw;
+      std::vector<float> jh(per);
+      for (int i = 0; i < n; ++i) {
+        HIPCHK(c, hipMemcpy(jh.data(), d_jac + (size_t)i * per, sizeof(float) * per, hipMemcpyDeviceToHost));
+        // device [np][nz][4][numw] -> DSYN(NFREQ=4*numw, IFREEP=np*nz): same order
+        memcpy(dsyn + (size_t)cols[i] * per, jh.data(), sizeof(float) * per);
+      }
+    }
+  }
+  (void)hipFree(d_atm);
+  (void)hipFree(d_tmp);
+  (void)hipFree(d_st);
+  (void)hipFree(d_status);
+  if (d_jac) (void)hipFree(d_jac);
+  return rc;
+}
+
+int fz_malloc(fz_ctx* c, void** p, uint64_t bytes) {
+  if (!c || !p) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (hipMalloc(p, bytes) != hipSuccess) return set_err(c, FZ_ENOMEM, "hipMalloc failed");
+  return FZ_OK;
+}
+int fz_free(fz_ctx* c, void* p) {
+  if (!c) return FZ_EARG;
+  HIPCHK(c, hipFree(p));
+  return FZ_OK;
+}
+int fz_memcpy_h2d(fz_ctx* c, void* d, const void* s, uint64_t b) {
+  HIPCHK(c, hipMemcpy(d, s, b, hipMemcpyHostToDevice));
+  return FZ_OK;
+}
+int fz_memcpy_d2h(fz_ctx* c, void* d, const void* s, uint64_t b) {
+  HIPCHK(c, hipMemcpy(d, s, b, hipMemcpyDeviceToHost));
+  return FZ_OK;
+}
+int fz_synchronize(fz_ctx* c) {
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipDeviceSynchronize());
+  return FZ_OK;
+}
+
+}  // extern "C"

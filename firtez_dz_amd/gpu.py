@@ -1,0 +1,215 @@
+"""Python host binding of the C ABI (include/firtez_gpu.h).
+
+Mirrors the reference's FORWARD3D operator (forward.f90:43-306): a cube of
+columns goes in, Stokes profiles (SYN3D) and response functions (DSYN) come
+out, and the cube's rho, Pe, mu and log tau5 are updated in place.  The HIP
+library must be present: there is no CPU fallback, and every entry point
+raises on a nonzero return code.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import lines as _lines
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfirtez_gpu.so")
+
+RF_NONE, RF_SPHERICAL, RF_CARTESIAN = 0, 1, 2
+NPARAM = {RF_NONE: 0, RF_SPHERICAL: 5, RF_CARTESIAN: 7}
+PARAM_NAMES = {RF_SPHERICAL: ("T", "B", "gamma", "phi", "vlos"),
+               RF_CARTESIAN: ("T", "Pg", "rho", "Bx", "By", "Bz", "vz")}
+# device atmosphere rows (d_atm[12][nz][ncol])
+ATM = ("T", "Pg", "rho", "Pe", "mu", "Bx", "By", "Bz", "vx", "vy", "vz", "tau5")
+
+ERRORS = {1: "equation of state", 2: "non-finite value", 3: "ionisation stage", 4: "wavelength < 1800 A",
+          5: "configuration", 6: "HIP runtime", 7: "device memory", 8: "argument"}
+
+_lib = None
+
+
+class FzError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libfirtez_gpu.so (built by firtez_dz_amd.build); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FzError("libfirtez_gpu.so is not built: run python -m firtez_dz_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp, fp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
+    L.fz_create.argtypes = [ctypes.POINTER(vp), ip]
+    L.fz_destroy.argtypes = [vp]
+    L.fz_last_error.argtypes = [vp]
+    L.fz_last_error.restype = ctypes.c_char_p
+    L.fz_configure.argtypes = [vp, ctypes.POINTER(_lines.CLine), ip, ctypes.POINTER(_lines.CRegion), ip, ip, dp]
+    L.fz_numw.argtypes = [vp]
+    L.fz_gain_hsra.argtypes = [vp, fp]
+    L.fz_set_gain.argtypes = [vp, fp]
+    L.fz_forward_dev.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp]
+    L.fz_hydrostatic_dev.argtypes = [vp, ip, vp, vp, vp]
+    L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
+    L.fz_set_tile.argtypes = [vp, ip, ip]
+    L.fz_synchronize.argtypes = [vp]
+    L.fz_set_profiling.argtypes = [vp, ip]
+    L.fz_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_int)]
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/firtez_gpu.h."""
+    return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
+            "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
+            "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing"]
+
+
+def _f(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Context:
+    """One device context: line setup, height grid, GAIN1D."""
+
+    def __init__(self, device: int = 0):
+        self.lib = library()
+        self.h = ctypes.c_void_p()
+        rc = self.lib.fz_create(ctypes.byref(self.h), device)
+        if rc:
+            raise FzError("fz_create failed (%d): no usable GPU %d" % (rc, device))
+        self.numw = 0
+        self.nz = 0
+        self.lineset = None
+
+    def close(self):
+        if self.h:
+            self.lib.fz_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.lib.fz_last_error(self.h).decode()
+            raise FzError("%s: %s error (%d) %s" % (what, ERRORS.get(rc, "?"), rc, msg))
+
+    def configure(self, ls: _lines.LineSetup, z_km):
+        cl, cr = _lines.c_arrays(ls)
+        z = np.ascontiguousarray(z_km, dtype=np.float64)
+        self._check(self.lib.fz_configure(self.h, cl, len(ls.lines), cr, len(ls.regions), len(z),
+                                          z.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "fz_configure")
+        self.numw = self.lib.fz_numw(self.h)
+        self.nz = len(z)
+        self.lineset = ls
+        return self.numw
+
+    def gain_hsra(self) -> np.ndarray:
+        """GAIN1D(4, numw) computed on the device from the HSRA model."""
+        g = np.zeros((self.numw, 4), np.float32)
+        self._check(self.lib.fz_gain_hsra(self.h, _f(g)), "fz_gain_hsra")
+        return g.T.copy()
+
+    def set_gain(self, gain):
+        g = np.ascontiguousarray(np.asarray(gain, np.float32).T)
+        self._check(self.lib.fz_set_gain(self.h, _f(g)), "fz_set_gain")
+
+    def set_tile(self, columns: int, block: int = 256):
+        self._check(self.lib.fz_set_tile(self.h, columns, block), "fz_set_tile")
+
+    # ---------------------------------------------------------------- device path
+    def forward_dev(self, ncol: int, atm_ptr: int, rf_mode: int, stokes_ptr: int, jac_ptr: Optional[int],
+                    status_ptr: Optional[int] = None, stream: Optional[int] = None):
+        """Asynchronous launch on device buffers (see fz_forward_dev)."""
+        self._check(self.lib.fz_forward_dev(self.h, ncol, atm_ptr, rf_mode, stokes_ptr, jac_ptr or None,
+                                            status_ptr or None, stream or None), "fz_forward_dev")
+
+    def hydrostatic_dev(self, ncol: int, atm_ptr: int, status_ptr: Optional[int] = None,
+                        stream: Optional[int] = None):
+        self._check(self.lib.fz_hydrostatic_dev(self.h, ncol, atm_ptr, status_ptr or None, stream or None),
+                    "fz_hydrostatic_dev")
+
+    def set_profiling(self, on=True):
+        self._check(self.lib.fz_set_profiling(self.h, int(on)), "fz_set_profiling")
+
+    def last_timing(self):
+        """(micro_ms, rt_ms, rt_launches) of the last forward call (HIP events)."""
+        a, b, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        self._check(self.lib.fz_last_timing(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)),
+                    "fz_last_timing")
+        return a.value, b.value, n.value
+
+    def synchronize(self):
+        self._check(self.lib.fz_synchronize(self.h), "fz_synchronize")
+
+    # ---------------------------------------------------------------- torch convenience
+    def forward_torch(self, atm, rf_mode=RF_SPHERICAL, stokes=None, jac=None, status=None):
+        """atm: torch float32 cuda tensor [12, nz, ncol] (updated in place).
+        Returns (stokes [ncol,4,numw], jac [ncol,np,nz,4,numw] or None, status [ncol])."""
+        import torch
+        assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous()
+        assert atm.shape[0] == 12 and atm.shape[1] == self.nz
+        ncol = atm.shape[2]
+        dev = atm.device
+        if stokes is None:
+            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=dev)
+        npar = NPARAM[rf_mode]
+        if npar and jac is None:
+            jac = torch.empty((ncol, npar, self.nz, 4, self.numw), dtype=torch.float32, device=dev)
+        if status is None:
+            status = torch.zeros(ncol, dtype=torch.int32, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        self.forward_dev(ncol, atm.data_ptr(), rf_mode, stokes.data_ptr(), jac.data_ptr() if npar else None,
+                         status.data_ptr(), stream)
+        return stokes, (jac if npar else None), status
+
+    def hydrostatic_torch(self, atm, status=None):
+        import torch
+        ncol = atm.shape[2]
+        if status is None:
+            status = torch.zeros(ncol, dtype=torch.int32, device=atm.device)
+        self.hydrostatic_dev(ncol, atm.data_ptr(), status.data_ptr(), torch.cuda.current_stream(atm.device).cuda_stream)
+        return status
+
+    # ---------------------------------------------------------------- FORWARD3D mirror
+    def forward3d(self, cube: dict, am_i_done=None, rf_mode=RF_NONE):
+        """FORWARD3D on host cubes (forward.f90:43-306).
+
+        cube: dict of Fortran-order arrays shaped (nz, ny, nx) float32 for
+        'T','Pg','rho','Pe','mu','Bx','By','Bz','vx','vy','vz','tau5'; rho, Pe,
+        mu and tau5 are updated in place.  Returns SYN3D (4*numw, ny, nx)
+        float64 and, for rf_mode != RF_NONE, DSYN (4*numw, np*nz, ny, nx) float32
+        (both Fortran order)."""
+        nz, ny, nx = cube["T"].shape
+        if nz != self.nz:
+            raise FzError("cube has nz=%d, context nz=%d" % (nz, self.nz))
+        arrs = []
+        for name in ATM:
+            a = cube[name]
+            if not (a.dtype == np.float32 and a.flags.f_contiguous):
+                raise FzError("cube['%s'] must be float32 Fortran-contiguous" % name)
+            arrs.append(a)
+        nfreq = 4 * self.numw
+        syn = np.zeros((nfreq, ny, nx), dtype=np.float64, order="F")
+        npar = NPARAM[rf_mode]
+        dsyn = np.zeros((nfreq, npar * nz, ny, nx), dtype=np.float32, order="F") if npar else None
+        done = None
+        if am_i_done is not None:
+            done = np.asfortranarray(am_i_done, dtype=np.int32)
+        rc = self.lib.fz_forward3d(self.h, nx, ny, *[_f(a) for a in arrs],
+                                   done.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if done is not None else None,
+                                   rf_mode, syn.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   _f(dsyn) if dsyn is not None else None)
+        self._check(rc, "fz_forward3d")
+        return syn, dsyn

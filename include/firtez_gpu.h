@@ -1,0 +1,125 @@
+/* firtez_gpu.h -- C ABI of the MI355X implementation of the FIRTEZ-dz per-column
+ * polarized radiative-transfer path (forward synthesis + analytic response
+ * functions).  Plain C types only; a Fortran host binds it through
+ * iso_c_binding (firtez_dz_amd/fortran/firtez_gpu_mod.f90).
+ *
+ * All entry points return 0 on success or one of the FZ_E* codes; the text of
+ * the last error of a context is available from fz_last_error().  The
+ * reference STOPs on the same conditions (file:line given per code).
+ */
+#ifndef FIRTEZ_GPU_H
+#define FIRTEZ_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FZ_OK = 0,
+  FZ_EEOS = 1,      /* electron-density iteration failed / N_H<0  (chemical_equilibrium.f90:259-268) */
+  FZ_ENAN = 2,      /* non-finite value  (voigt.f:91-95, absorption_matrix.f90:1882-1889) */
+  FZ_EION = 3,      /* ionisation stage outside 1..3  (line_opacity.f90:86-90) */
+  FZ_ELAMBDA = 4,   /* wavelength below 1800 A  (cont_opacity.f90:83-86) */
+  FZ_ECONFIG = 5,   /* invalid line/region/height configuration */
+  FZ_EHIP = 6,      /* HIP runtime error */
+  FZ_ENOMEM = 7,    /* device allocation failed */
+  FZ_EARG = 8       /* invalid argument */
+};
+
+/* Response-function parameter sets */
+enum {
+  FZ_RF_NONE = 0,
+  FZ_RF_SPHERICAL = 1, /* d/d(T|Pg, |B|, inclination, azimuth, v_los) at every height */
+  FZ_RF_CARTESIAN = 2  /* d/d(T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z): DSYN1D slots 1-7 (rtesolver.f90:209-258) */
+};
+
+/* One row of the line table.  Replaces the FORWARD_PARAM per-line arrays
+ * LINE_ZN, LINE_ION, LINE_L0, LOGGF, EPLOW, SIGMA, ALPHA, SL..JU
+ * (forward_param.f90:16-20) filled by NREAD_LINES_DATABASE. */
+typedef struct {
+  int32_t zn, ion;
+  double lambda0, loggf, eplow, sigma, alpha;
+  double sl, ll, jl, su, lu, ju;
+} fz_line;
+
+/* One spectral region = one LINES: control-file row (input_file.f90:253-367):
+ * main line + blends (indices into the line table), WAVE grid in mA. */
+typedef struct {
+  int32_t nblend;
+  int32_t line[4];
+  int32_t nwave;
+  double wave_start, wave_step;
+} fz_region;
+
+typedef struct fz_ctx fz_ctx;
+
+/* Context on one device.  No counterpart in the reference (module globals). */
+int fz_create(fz_ctx** ctx, int device);
+void fz_destroy(fz_ctx* ctx);
+const char* fz_last_error(const fz_ctx* ctx);
+
+/* Lines, regions and height grid z_km[nz] (bottom first).  Replaces
+ * NREAD_LINES / WAVE_INIT (input_file.f90:253-367, pre_post_duties.f90:592-621)
+ * and the ZZ grid (grid_param.f90). */
+int fz_configure(fz_ctx* ctx, const fz_line* lines, int nlines, const fz_region* regions,
+                 int nregions, int nz, const double* z_km);
+int fz_numw(const fz_ctx* ctx);
+
+/* GAIN1D(4,NUMW) continuum normalisation from the HSRA model, computed on the
+ * device; replaces GET_NORM (normalization.f90:696-754).  gain: host, 4*numw
+ * floats, layout gain[4*l + s]. */
+int fz_gain_hsra(fz_ctx* ctx, float* gain);
+/* Use a given GAIN1D instead (e.g. the reference's). */
+int fz_set_gain(fz_ctx* ctx, const float* gain);
+
+/* FORWARD3D on device-resident data (forward.f90:43-306, per column the
+ * FORWARD1D of forward.f90:857-1029 with HYDROSTATIC=no).
+ *   d_atm    [12][nz][ncol] float, in/out: T, Pg, rho, Pe, mu, Bx, By, Bz, vx,
+ *            vy, vz, log tau5 (rho, Pe, mu, log tau5 are updated like the
+ *            reference updates the cube)
+ *   d_stokes [ncol][4][numw] float: SYN1D*GAIN1D
+ *   d_jac    [ncol][np][nz][4][numw] float (np = 5 spherical, 7 cartesian), the
+ *            propagated response functions times GAIN1D; NULL if rf_mode=NONE
+ *   d_status [ncol] int32 or NULL: per-column FZ_E* code
+ *   stream   hipStream_t (NULL = default stream); the call is asynchronous. */
+int fz_forward_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
+                   int32_t* d_status, void* stream);
+
+/* FORWARD3D on host arrays in the reference's layout (forward.f90:43-306):
+ * cubes(NZ,NY,NX) column-major float (TEM3D..TAU3D5, updated in place),
+ * am_i_done(NY,NX) (columns with nonzero are skipped), syn3d(NFREQ,NY,NX)
+ * double with NFREQ = 4*NUMW in Stokes-major blocks, dsyn(NFREQ,IFREEP,NY,NX)
+ * float with IFREEP = np*NZ (parameter-major, height within) or NULL. */
+int fz_forward3d(fz_ctx* ctx, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d,
+                 float* mw3d, const float* bx3d, const float* by3d, const float* bz3d,
+                 const float* vx3d, const float* vy3d, const float* vz3d, float* tau3d5,
+                 const int32_t* am_i_done, int rf_mode, double* syn3d, float* dsyn);
+
+/* Hydrostatic equilibrium from the top (RK4_INTEGRATION_LOG,
+ * hydrostatic_tools.f90:150-222) for ncol columns on the device: reads T and
+ * the top Pg (and Pe) of d_atm, writes Pg, rho, Pe, mu. */
+int fz_hydrostatic_dev(fz_ctx* ctx, int ncol, float* d_atm, int32_t* d_status, void* stream);
+
+/* Device memory helpers for hosts without their own allocator. */
+int fz_malloc(fz_ctx* ctx, void** ptr, uint64_t bytes);
+int fz_free(fz_ctx* ctx, void* ptr);
+int fz_memcpy_h2d(fz_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int fz_memcpy_d2h(fz_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int fz_synchronize(fz_ctx* ctx);
+
+/* Kernel timing: when enabled, the library records HIP events on the launch
+ * stream around the microphysics launch and around the radiative-transfer
+ * launches of each fz_forward_dev call; fz_last_timing waits for them and
+ * returns the elapsed milliseconds of the last call. */
+int fz_set_profiling(fz_ctx* ctx, int on);
+int fz_last_timing(fz_ctx* ctx, float* micro_ms, float* rt_ms, int* rt_launches);
+
+/* Columns processed per radiative-transfer launch (scratch is sized for this
+ * tile); the autotuner sets it.  Default 8192. */
+int fz_set_tile(fz_ctx* ctx, int columns_per_launch, int block_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

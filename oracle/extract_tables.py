@@ -359,6 +359,7 @@ def main():
                  " * (atom_database.f90:47-119, 121-1036; normalization.f90:146-345).\n"
                  " * Data only: element constants, partition-function programs, HSRA model.\n"
                  " * Do not edit by hand. */\n#pragma once\n\n")
+        fh.write("#ifndef FZ_DATA_QUAL\n#define FZ_DATA_QUAL static\n#endif\n")
         fh.write("#define FZ_NELEM 92\n#define FZ_HSRA_LEN 387\n")
         fh.write("#define FZ_PF_NCODE %d\n#define FZ_PF_NCONST %d\n\n" % (len(code), len(consts)))
         for i, n in enumerate(OPS):
@@ -367,16 +368,16 @@ def main():
             fh.write("#define FZ_PV_%s %d\n" % (n, i))
         fh.write("\n")
         for name, arr in (("fz_xi", xi), ("fz_xii", xii), ("fz_abund", ab), ("fz_matom", ma)):
-            fh.write("static const double %s[FZ_NELEM] = {\n%s};\n" % (name, c_doubles(arr)))
-        fh.write("static const unsigned short fz_pf_start[FZ_NELEM + 1] = {\n  %s};\n"
+            fh.write("FZ_DATA_QUAL const double %s[FZ_NELEM] = {\n%s};\n" % (name, c_doubles(arr)))
+        fh.write("FZ_DATA_QUAL const unsigned short fz_pf_start[FZ_NELEM + 1] = {\n  %s};\n"
                  % ", ".join(str(s) for s in start))
-        fh.write("static const unsigned char fz_pf_op[FZ_PF_NCODE] = {\n  %s};\n"
+        fh.write("FZ_DATA_QUAL const unsigned char fz_pf_op[FZ_PF_NCODE] = {\n  %s};\n"
                  % ", ".join(str(op) for op, _ in code))
-        fh.write("static const unsigned short fz_pf_arg[FZ_PF_NCODE] = {\n  %s};\n"
+        fh.write("FZ_DATA_QUAL const unsigned short fz_pf_arg[FZ_PF_NCODE] = {\n  %s};\n"
                  % ", ".join(str(a) for _, a in code))
-        fh.write("static const double fz_pf_const[FZ_PF_NCONST] = {\n%s};\n" % c_doubles(consts))
+        fh.write("FZ_DATA_QUAL const double fz_pf_const[FZ_PF_NCONST] = {\n%s};\n" % c_doubles(consts))
         for name, arr in (("fz_hsra_t", h_t), ("fz_hsra_pg", h_p), ("fz_hsra_rho", h_r), ("fz_hsra_z", h_z)):
-            fh.write("static const float %s[FZ_HSRA_LEN] = {\n%s};\n"
+            fh.write("FZ_DATA_QUAL const float %s[FZ_HSRA_LEN] = {\n%s};\n"
                      % (name, c_doubles([float(f32(v)) for v in arr], 6)))
     print("wrote", OUT, "ops", len(code), "consts", len(consts))
 

@@ -1,0 +1,157 @@
+"""Write the golden fixtures from the reference's own Fortran (oracle/_ref).
+
+Runs only in a container that has the reference tree (oracle/_ref is built
+from it by oracle/ref/Makefile).  Every number below comes out of the
+reference's compiled modules (EOS, opacities, Voigt, Zeeman, absorption
+matrix, evolution operator, HSRA normalisation, hydrostatic RK4); the column
+loop and the RF propagation are driven by the harness in oracle/ref.
+
+Fixtures (tests/golden/*.npz, data only):
+  units.npz        partition functions, EOS + DERIVVAR, continuum/line opacity,
+                   damping, Voigt/Faraday-Voigt, Zeeman patterns
+  syn1.npz         tests/test_syn1 setup (LINES: 4,5 500 -700.0 5.0) on
+                   default/test_atmos.bin: GAIN1D, Stokes, updated model
+  rf.npz           tests/test_rf setup on default/rf_test_atmos.bin with all
+                   seven Cartesian response functions (every 8th height)
+  seeded_c1.npz    a 4x4 patch of the seeded 16x16x64 cube (2 lines, 112
+                   wavelengths): hydrostatic Pg, Stokes for 16 columns and
+                   the full response functions of 2 columns
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "..", ".."))
+
+import oracle  # noqa: E402
+from firtez_dz_amd import atmos, lines, modelio  # noqa: E402
+
+REF = os.environ.get("FZ_REFERENCE", "/root/reference")
+CART = (1, 1, 1, 1, 1, 1, 1, 0)
+
+
+def model_from_cube(c, col=0):
+    nz = c["T"].shape[0]
+    m = np.zeros((13, nz), np.float32)
+    names = ("T", "Pg", "rho", "Pe", "mu", "Bx", "By", "Bz", None, None, "vz", "tau5")
+    for i, nm in enumerate(names):
+        if nm:
+            m[i] = c[nm][:, 0, col]
+    m[12] = c["z"][:, 0, 0]
+    return m
+
+
+def units(r):
+    out = {}
+    rng = np.random.default_rng(7)
+    elems = np.arange(1, 93)
+    temps = np.array([1800., 3500., 4321.5, 5400., 6250., 7310.5, 8800., 1.2e4, 1.62e4, 2.5e4])
+    U = np.zeros((92, len(temps), 3))
+    DU = np.zeros_like(U)
+    for i, z in enumerate(elems):
+        for j, t in enumerate(temps):
+            U[i, j], DU[i, j] = r.partition(int(z), float(t))
+    out.update(pf_T=temps, pf_U=U, pf_DU=DU)
+    ls = lines.fe6301_6302()
+    r.setup(ls, resp=True, atmpar=CART)
+    n = 24
+    T = np.float32(rng.uniform(3600, 11000, n))
+    P = np.float32(10 ** rng.uniform(1.5, 5.5, n))
+    eos = np.zeros((n, 5))
+    dv = np.zeros((n, 25))
+    kc = np.zeros((n, 2))
+    dkc = np.zeros((n, 2, 4))
+    kl = np.zeros((n, 2))
+    dkl = np.zeros((n, 2, 6))
+    dmp = np.zeros((n, 2))
+    ddm = np.zeros((n, 2, 4))
+    for i in range(n):
+        _, eos[i], dv[i] = r.eos(float(T[i]), float(P[i]), 0.0)
+        nh, ne = eos[i, 0], eos[i, 1]
+        for j, lam in enumerate((5000.0, ls.lines[0].lambda0)):
+            _, kc[i, j], d = r.opac_cont(float(T[i]), lam, ne, nh)
+            dkc[i, j] = d[17:21]
+        for j in range(2):
+            _, kl[i, j], d = r.opac_line(j, float(T[i]), ne, nh)
+            dkl[i, j] = d[[13, 14, 15, 16, 21, 22]]
+            dmp[i, j], d2 = r.damping(j, float(T[i]), nh)
+            ddm[i, j] = d2[4:8]
+    out.update(eos_T=T, eos_P=P, eos_out=eos, eos_dv=dv, kc=kc, dkc=dkc, kl=kl, dkl=dkl, damp=dmp, ddamp=ddm)
+    a = np.array([1e-3, 1e-2, 0.05, 0.3, 1.5])
+    v = np.linspace(-25, 25, 101)
+    H = np.zeros((len(a), len(v)))
+    F = np.zeros_like(H)
+    for i, aa in enumerate(a):
+        H[i], F[i] = r.voigt(float(aa), v)
+    out.update(voigt_a=a, voigt_v=v, voigt_H=H, voigt_F=F)
+    for j in range(2):
+        s, st, ty = r.zeeman(j)
+        out["zeeman%d_shift" % j], out["zeeman%d_strength" % j], out["zeeman%d_type" % j] = s, st, ty
+    np.savez_compressed(os.path.join(HERE, "units.npz"), **out)
+
+
+def syn1(r):
+    ls = lines.setup([((4, 5), 500, -700.0, 5.0)])
+    r.setup(ls, resp=False)
+    g = r.gain()
+    r.set_gain(g)
+    c = modelio.read_model(os.path.join(REF, "default", "test_atmos.bin"))
+    m = model_from_cube(c)
+    mo, syn, _, kc5 = r.forward1d(m)
+    np.savez_compressed(os.path.join(HERE, "syn1.npz"), model_in=m, model_out=mo, gain=g, stokes=syn, kc5=kc5,
+                        lines_rows=np.array([4, 5]), nwave=500, wave_start=-700.0, wave_step=5.0)
+
+
+def rf(r):
+    ls = lines.setup([((4, 5), 500, -700.0, 5.0)])
+    r.setup(ls, resp=True, atmpar=CART)
+    g = r.gain()
+    r.set_gain(g)
+    c = modelio.read_model(os.path.join(REF, "default", "rf_test_atmos.bin"))
+    m = model_from_cube(c)
+    mo, syn, dsyn, kc5 = r.forward1d(m)
+    keep = np.arange(0, m.shape[1], 8)
+    np.savez_compressed(os.path.join(HERE, "rf.npz"), model_in=m, model_out=mo, gain=g, stokes=syn,
+                        rf_heights=keep, rf=dsyn[keep][:, :, :7, :].astype(np.float32))
+
+
+def seeded(r):
+    ls = lines.fe6301_6302()
+    r.setup(ls, resp=True, atmpar=CART)
+    g = r.gain()
+    r.set_gain(g)
+    atm, z = atmos.seeded(16, 16, 64, seed=1)
+    cols = [x * 16 + y for x in range(4) for y in range(4)]
+    models = []
+    stokes = []
+    rfs = []
+    for i, col in enumerate(cols):
+        m = atmos.column_model(atm, z, col)
+        pg, rho, pel, mw, nh = r.he(m[0], float(m[1, -1]), z)
+        m[1], m[2], m[3], m[4] = pg, rho, pel, mw
+        models.append(m.copy())
+        mo, syn, dsyn, _ = r.forward1d(m)
+        stokes.append(syn)
+        if i < 2:
+            rfs.append(dsyn[:, :, :7, :].astype(np.float32))
+    np.savez_compressed(os.path.join(HERE, "seeded_c1.npz"), cols=np.array(cols), z=z, gain=g,
+                        models=np.array(models), stokes=np.array(stokes), rf=np.array(rfs), seed=1)
+
+
+def main():
+    if not oracle.ref_available():
+        oracle.build_ref()
+    r = oracle.Ref()
+    units(r)
+    syn1(r)
+    rf(r)
+    seeded(r)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()
