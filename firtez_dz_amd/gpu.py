@@ -43,6 +43,13 @@ def library():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FzError("libfirtez_gpu.so is not built: run python -m firtez_dz_amd.build")
+    # One HIP runtime per process: when PyTorch is installed its bundled
+    # libamdhip64 (same SONAME) is loaded first and the library binds to it,
+    # so device pointers and streams can be shared with torch tensors.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, ip, dp, fp = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
     L.fz_create.argtypes = [ctypes.POINTER(vp), ip]

@@ -70,6 +70,7 @@ class Cpu:
         L.fzc_forward1d.argtypes = [ctypes.c_int, _fp, _dp, _dp, _dp]
         L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
+        L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
         self.numw = 0
         self.resp = False
 
@@ -126,6 +127,17 @@ class Cpu:
         ty = np.zeros(64, dtype=np.int32)
         n = self.lib.fzc_zeeman(line, _ptr(s, _dp), _ptr(st, _dp), _ptr(ty, _ip))
         return s[:n], st[:n], ty[:n]
+
+    def evol(self, x7, dz_km, dx=None):
+        """Evolution operator (4x4) and its derivatives along dx (8 x 7)."""
+        x7 = np.ascontiguousarray(x7, dtype=np.float64)
+        dx = np.zeros((8, 7)) if dx is None else np.ascontiguousarray(dx, dtype=np.float64)
+        ev = np.zeros(16)
+        dev = np.zeros(8 * 16)
+        e = self.lib.fzc_evol(_ptr(x7, _dp), dz_km, _ptr(dx, _dp), _ptr(ev, _dp), _ptr(dev, _dp))
+        if e:
+            raise RuntimeError("fzc_evol error %d" % e)
+        return ev.reshape(4, 4), dev.reshape(8, 4, 4)
 
     def gain(self):
         g = np.zeros((self.numw, 4), dtype=np.float32)

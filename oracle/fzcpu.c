@@ -1213,6 +1213,23 @@ static int evol_operator(int K, int nz, const double* zz, const double* x7, cons
   return E_OK;
 }
 
+/* unit entry for tests: evolution operator of one layer of thickness dz_km
+ * (a 2-height grid) and its derivatives along 8 directions dx[8][7] */
+int fzc_evol(const double* x7, double dz_km, const double* dx, double* evol, double* dev) {
+  double zz[2] = {0.0, dz_km};
+  evol_t E;
+  int saved = RESP;
+  RESP = 1;
+  int e = evol_operator(0, 2, zz, x7, dx, &E);
+  RESP = saved;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      evol[i * 4 + j] = E.evol[i][j];
+      for (int p = 0; p < NPD; ++p) dev[(p * 4 + i) * 4 + j] = E.dev[p][i][j];
+    }
+  return e;
+}
+
 /* ------------------------------------------------ source function (rtesolver.f90:1008-1066) */
 static double region_lambda_cm(int l) {
   for (int m = 0; m < NRG; ++m)

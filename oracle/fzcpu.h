@@ -49,6 +49,10 @@ int fzc_damping(int line, float T, double nh, double* damp, double* d);
 void fzc_voigt(int n, double a, const double* v, double* h, double* f);
 int fzc_zeeman(int line, double* shift, double* strength, int32_t* type);
 
+/* evolution operator of one layer (rtesolver.f90:269-1004): x7 = eta_I..rho_V,
+ * dx[8][7] directions; evol[16] row-major, dev[8][16] */
+int fzc_evol(const double* x7, double dz_km, const double* dx, double* evol, double* dev);
+
 /* GAIN1D from the HSRA continuum (normalization.f90:696-754) */
 int fzc_gain(float* gain);
 void fzc_set_gain(const float* gain);
