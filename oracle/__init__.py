@@ -49,6 +49,33 @@ def ref_available() -> bool:
     return os.path.exists(REF_LIB)
 
 
+class LmCfg(ctypes.Structure):
+    """fzc_lmcfg (oracle/fzcpu.h): per-pixel inversion configuration."""
+    _fields_ = [("nz", ctypes.c_int32), ("numw", ctypes.c_int32),
+                ("atmpar", ctypes.c_int32 * 8), ("invstk", ctypes.c_int32 * 4),
+                ("nodes", ctypes.c_int32 * 8), ("pen_typ", ctypes.c_int32 * 8),
+                ("pen_alp", ctypes.c_double * 8),
+                ("regul", ctypes.c_int32), ("solver", ctypes.c_int32),
+                ("node_expansion", ctypes.c_int32), ("curic", ctypes.c_int32),
+                ("svdtol", ctypes.c_double), ("inu", ctypes.c_double), ("mpert", ctypes.c_double)]
+
+
+def lm_cfg(nz, numw, atmpar, nodes, invstk=(1, 1, 1, 1), solver=1, node_expansion=1, svdtol=1e-4,
+           regul=False, pen_typ=(0,) * 8, pen_alp=(0.0,) * 8, inu=None, mpert=0.15, curic=1):
+    c = LmCfg()
+    c.nz, c.numw = nz, numw
+    c.atmpar[:] = list(atmpar)
+    c.invstk[:] = list(invstk)
+    c.nodes[:] = [n if a else 0 for n, a in zip(nodes, atmpar)]
+    c.pen_typ[:] = list(pen_typ)
+    c.pen_alp[:] = list(pen_alp)
+    c.regul, c.solver, c.node_expansion, c.curic = int(regul), solver, node_expansion, curic
+    c.svdtol = svdtol
+    c.inu = float(sum(invstk) * numw) if inu is None else float(inu)
+    c.mpert = mpert
+    return c
+
+
 class Cpu:
     """ctypes view of fzcpu.c."""
 
@@ -71,6 +98,12 @@ class Cpu:
         L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
+        L.fzc_int_coefs.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
+        L.fzc_njevals.argtypes = [ctypes.c_int] * 6
+        L.fzc_eqv_dsyn.argtypes = [ctypes.POINTER(LmCfg), _dp, _dp, _fp]
+        L.fzc_get_perturbation.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_double, _dp, _dp, _dp,
+                                           ctypes.c_int, ctypes.c_double]
+        L.fzc_lm_step.argtypes = [ctypes.POINTER(LmCfg), _dp, _dp, _dp, _dp, _dp, _fp, _fp, _dp, _fp, _dp]
         self.numw = 0
         self.resp = False
 
@@ -174,6 +207,75 @@ class Cpu:
         if e:
             raise RuntimeError("fzc_forward_batch error %d" % e)
         return m, syn, dsyn
+
+    # ---------------------------------------------------------------- LM step
+    def int_coefs(self, nz, nj):
+        """COEFS (nz, nj) of GET_INT_COEFS."""
+        c = np.zeros((nj, nz))
+        e = self.lib.fzc_int_coefs(nz, nj, _ptr(c, _dp))
+        if e:
+            raise RuntimeError("fzc_int_coefs error %d" % e)
+        return c.T.copy()
+
+    def njevals(self, apar, curic, maxiter, cycpow=2, toffset=1, nz=64):
+        return self.lib.fzc_njevals(apar, curic, maxiter, cycpow, toffset, nz)
+
+    @staticmethod
+    def coefs_block(cfg, coefs_by_param):
+        """Concatenate per-parameter COEFS (nz, nj) in inverted-parameter order
+        into the (ifreep, nz) block the LM routines take."""
+        cols = []
+        for p in range(8):
+            if cfg.atmpar[p]:
+                c = np.asarray(coefs_by_param[p], dtype=np.float64)
+                assert c.shape == (cfg.nz, cfg.nodes[p])
+                cols.append(c.T)
+        return np.ascontiguousarray(np.concatenate(cols, axis=0))
+
+    def eqv_dsyn(self, cfg, coefs, dsyn):
+        """dsyn (nz, numw, 8, 4) from forward1d -> EQVDSYN (nfreq, ifreep) float32."""
+        co = np.ascontiguousarray(coefs, dtype=np.float64)
+        ifreep = co.shape[0]
+        nfreq = sum(cfg.invstk) * cfg.numw
+        out = np.zeros((ifreep, nfreq), np.float32)
+        d = np.ascontiguousarray(dsyn, dtype=np.float64)
+        self.lib.fzc_eqv_dsyn(ctypes.byref(cfg), _ptr(co, _dp), _ptr(d, _dp), _ptr(out, _fp))
+        return out.T.copy()
+
+    def get_perturbation(self, jac, diff, lam, reg_pen=None, reg_hss=None, dtype=1, svdtol=1e-4):
+        """jac (nf, np) -> delta (np) of GET_PERTURBATION."""
+        nf, np_ = jac.shape
+        jt = np.ascontiguousarray(np.asarray(jac, dtype=np.float64).T)
+        df = np.ascontiguousarray(diff, dtype=np.float64)
+        delta = np.zeros(np_)
+        rp = None if reg_pen is None else _ptr(np.ascontiguousarray(reg_pen, dtype=np.float64), _dp)
+        rh = None if reg_hss is None else _ptr(np.ascontiguousarray(np.asarray(reg_hss, np.float64).T), _dp)
+        e = self.lib.fzc_get_perturbation(np_, nf, _ptr(jt, _dp), _ptr(df, _dp), lam, _ptr(delta, _dp), rp, rh,
+                                          dtype, svdtol)
+        if e:
+            raise RuntimeError("fzc_get_perturbation error %d" % e)
+        return delta
+
+    def lm_step(self, cfg, coefs, obs, syn, best_syn, sla, jac_cur, jac_best, isigmap, model_cur, model_best):
+        """One GET_PERT_1D.  jac_* (nfreq, ifreep) float32, models (13, nz)
+        float32.  Returns (model_cur, model_best, sla, best_syn, jac_best, delta)."""
+        co = np.ascontiguousarray(coefs, dtype=np.float64)
+        ifreep = co.shape[0]
+        m = np.ascontiguousarray(np.stack([model_cur, model_best]), dtype=np.float32).copy()
+        bs = np.ascontiguousarray(best_syn, dtype=np.float64).copy()
+        sl = np.ascontiguousarray(sla, dtype=np.float64).copy()
+        jc = np.ascontiguousarray(np.asarray(jac_cur, np.float32).T)
+        jb = np.ascontiguousarray(np.asarray(jac_best, np.float32).T).copy()
+        ob = np.ascontiguousarray(obs, dtype=np.float64)
+        sy = np.ascontiguousarray(syn, dtype=np.float64)
+        isg = np.ascontiguousarray(isigmap, dtype=np.float64)
+        delta = np.zeros(ifreep)
+        e = self.lib.fzc_lm_step(ctypes.byref(cfg), _ptr(co, _dp), _ptr(ob, _dp), _ptr(sy, _dp), _ptr(bs, _dp),
+                                 _ptr(sl, _dp), _ptr(jc, _fp), _ptr(jb, _fp), _ptr(isg, _dp), _ptr(m, _fp),
+                                 _ptr(delta, _dp))
+        if e:
+            raise RuntimeError("fzc_lm_step error %d" % e)
+        return m[0], m[1], sl, bs, jb.T.copy(), delta
 
     def he(self, tem, pg_top, z_km, pel_top=0.0):
         nz = len(tem)

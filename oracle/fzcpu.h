@@ -72,6 +72,34 @@ int fzc_forward_batch(int ncol, int nz, float* models, double* syn, double* dsyn
 int fzc_he(int nz, float* tem, float* pg, float* rho, float* pel, float* mw,
            double* nh, const double* zz);
 
+
+/* ------------------------------------------------ Levenberg-Marquardt step (fzlm.c) */
+
+/* Per-pixel inversion configuration: INV_ATMPAR, INV_STK, nodes per
+ * parameter for the current cycle (NSLAB_PER_FREEV), regularisation
+ * (VREGULARIZATION, PEN_TYP, PEN_ALP), SOLVER_METHOD, NODE_EXPANSION,
+ * RSVDTOL, INU and MPERT (invert_param.f90, code_modes.f90). */
+typedef struct {
+  int32_t nz, numw;
+  int32_t atmpar[8];
+  int32_t invstk[4];
+  int32_t nodes[8];
+  int32_t pen_typ[8];
+  double pen_alp[8];
+  int32_t regul, solver, node_expansion, curic;
+  double svdtol, inu, mpert;
+} fzc_lmcfg;
+
+int fzc_int_coefs(int nz, int nj, double* coefs);
+int fzc_njevals(int apar, int curic, int maxiter, int cycpow, int toffset, int nz);
+int fzc_eqv_dsyn(const fzc_lmcfg* c, const double* coefs, const double* dsyn, float* eqv);
+int fzc_get_perturbation(int np, int nf, const double* jac, const double* diff, double lambda,
+                         double* delta, const double* reg_pen, const double* reg_hss, int dtype,
+                         double svdtol);
+int fzc_lm_step(const fzc_lmcfg* c, const double* coefs, const double* obs, const double* syn,
+                double* best_syn, double* sla, const float* jac_cur, float* jac_best,
+                const double* isigmap, float* model, double* delta_out);
+
 #ifdef __cplusplus
 }
 #endif
