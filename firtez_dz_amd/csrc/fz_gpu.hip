@@ -133,7 +133,8 @@ struct RtArgs {
   const float* gain; // [numw][4]
   const int32_t* status;
   float* stokes;     // [ncol][4][numw]
-  float* jac;        // [ncol][np][nz][4][numw]
+  float* jac;        // [ncol - jcol0][np][nz][4][numw]
+  int jcol0;         // column of jac[0]
   float* scratch;    // [nz][4][ntile*numw]
 };
 
@@ -384,7 +385,7 @@ __global__ void __launch_bounds__(256) rt_kernel(RtArgs a) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = (i == j) ? 1.f : 0.f;
-    float* jb = a.jac + (long)col * NP * nz * 4 * numw + l;
+    float* jb = a.jac + (long)(col - a.jcol0) * NP * nz * 4 * numw + l;
     for (int k = nz - 1; k >= 0; --k) {
       float outv[NP][4];
       const Layer ly = load_layer(a, col, k);
@@ -603,6 +604,8 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 
 }  // namespace fz
 
+#include "fz_lm.hpp"
+
 // ==================================================================== C ABI
 using namespace fz;
 
@@ -622,6 +625,15 @@ struct fz_ctx {
   bool prof = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   int rt_launches = 0;
+  // inversion
+  bool inv_set = false;
+  LmArgs lm;                  // per-cycle constants; per-call pointers filled at launch
+  int slot_of[LM_MAXNP];
+  double* d_coefs = nullptr;  // [np][nz]
+  double* d_colsum = nullptr; // [np]
+  float* d_jtile = nullptr;   // per-height response functions of one tile
+  size_t jtile_cap = 0;
+  size_t lm_lds = 0;
 };
 
 // the context whose configuration is in a device's constant memory
@@ -673,6 +685,9 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_gain) (void)hipFree(c->d_gain);
   if (c->d_micro) (void)hipFree(c->d_micro);
   if (c->d_scratch) (void)hipFree(c->d_scratch);
+  if (c->d_coefs) (void)hipFree(c->d_coefs);
+  if (c->d_colsum) (void)hipFree(c->d_colsum);
+  if (c->d_jtile) (void)hipFree(c->d_jtile);
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -870,6 +885,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     a.status = d_status;
     a.stokes = d_stokes;
     a.jac = d_jac;
+    a.jcol0 = 0;
     a.scratch = c->d_scratch;
     const long nthr = (long)a.ntile * g.numw;
     const dim3 grid((unsigned)((nthr + c->block - 1) / c->block)), blk(c->block);
@@ -1007,6 +1023,214 @@ int fz_memcpy_d2h(fz_ctx* c, void* d, const void* s, uint64_t b) {
 int fz_synchronize(fz_ctx* c) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipDeviceSynchronize());
+  return FZ_OK;
+}
+
+// ------------------------------------------------------------------ inversion
+
+// GET_POSITIONS + GET_INT_COEFS (get_dmodel.f90:463-477, 563-644)
+int fz_int_coefs(int nz, int nj, double* coefs) {
+  if (!coefs || nz < 1 || nj < 1 || nj > nz) return FZ_EARG;
+  std::fill(coefs, coefs + (size_t)nz * nj, 0.0);
+  if (nj == 1) {
+    for (int k = 0; k < nz; ++k) coefs[k] = 1.0;
+    return FZ_OK;
+  }
+  if (nj == nz) {
+    for (int k = 0; k < nz; ++k) coefs[(size_t)k * nz + k] = 1.0;
+    return FZ_OK;
+  }
+  std::vector<double> pos(nj);
+  const double step = (double)(nz - 1) / (double)(nj - 1);
+  for (int j = 0; j < nj; ++j) pos[j] = ((double)(j + 1) - 1.0) * step + 1.0;
+  for (int j = 0; j < nj; ++j) {
+    const double lo = pos[std::max(j - 1, 0)], hi = pos[std::min(j + 1, nj - 1)], mid = pos[j];
+    for (int i = 1; i <= nz; ++i) {
+      const double x = (double)i;
+      if (x < lo || x > hi) continue;
+      double v;
+      if (x == mid) v = 1.0;
+      else if (x < mid) v = (1.0 - 0.0) / (mid - lo) * (x - lo);
+      else v = (1.0 - 0.0) / (hi - mid) * (hi - x);
+      coefs[(size_t)j * nz + (i - 1)] = v;
+    }
+  }
+  return FZ_OK;
+}
+
+int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq) {
+  if (!c || !inv) return FZ_EARG;
+  if (!c->configured) return set_err(c, FZ_ECONFIG, "fz_configure first");
+  if (inv->atmpar[7]) return set_err(c, FZ_ECONFIG, "P0 inversion is not supported");
+  const int nz = c->cfg.nz, numw = c->cfg.numw;
+  LmArgs& L = c->lm;
+  L = LmArgs();
+  L.nz = nz;
+  L.numw = numw;
+  int nstk = 0;
+  for (int s = 0; s < 4; ++s)
+    if (inv->invstk[s]) L.rowstk[nstk++] = s;
+  if (nstk == 0) return set_err(c, FZ_ECONFIG, "no Stokes parameter inverted");
+  L.nfreq = nstk * numw;
+  static const int mcol[8] = {A_T, A_PG, A_RHO, A_BX, A_BY, A_BZ, A_VZ, A_PG};
+  std::vector<double> coefs;
+  int np = 0, npar = 0;
+  for (int p = 0; p < 8; ++p) {
+    if (!inv->atmpar[p]) continue;
+    const int nj = inv->nodes[p];
+    if (nj < 1 || nj > nz) return set_err(c, FZ_ECONFIG, "nodes must be in 1..nz");
+    if (np + nj > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters");
+    L.par[npar] = p;
+    L.slot[npar] = p;  // Cartesian slots: T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z
+    L.mcol[npar] = mcol[p];
+    L.j0[npar] = np;
+    L.nj[npar] = nj;
+    std::vector<double> cf((size_t)nz * nj);
+    fz_int_coefs(nz, nj, cf.data());
+    coefs.insert(coefs.end(), cf.begin(), cf.end());
+    for (int j = 0; j < nj; ++j) c->slot_of[np + j] = p;
+    np += nj;
+    ++npar;
+  }
+  if (np == 0) return set_err(c, FZ_ECONFIG, "no parameter inverted");
+  L.np = np;
+  L.nparam = npar;
+  std::vector<double> colsum(np);
+  for (int j = 0; j < np; ++j) {
+    double w = 0.0;
+    int kb = nz, ke = -1;
+    for (int k = 0; k < nz; ++k) {
+      const double v = coefs[(size_t)j * nz + k];
+      w += v;
+      if (v != 0.0) {
+        kb = std::min(kb, k);
+        ke = std::max(ke, k);
+      }
+    }
+    colsum[j] = w;
+    L.kbeg[j] = kb;
+    L.kend[j] = ke;
+  }
+  L.regul = inv->regul ? 1 : 0;
+  L.solver = inv->solver;
+  if (L.solver != 1 && L.solver != 2) return set_err(c, FZ_ECONFIG, "solver must be 1 or 2");
+  L.node_expansion = inv->node_expansion;
+  if (L.node_expansion != 1 && L.node_expansion != 2) return set_err(c, FZ_ECONFIG, "node_expansion must be 1 or 2");
+  for (int p = 0; p < 8; ++p) {
+    L.pen_typ[p] = inv->pen_typ[p];
+    L.pen_alp[p] = inv->pen_alp[p];
+  }
+  L.svdtol = inv->svdtol == 0.0 ? 1.0e-4 : inv->svdtol;
+  L.inu = inv->inu == 0.0 ? (double)L.nfreq : inv->inu;
+  L.mpert = inv->mpert == 0.0 ? 0.15 : inv->mpert;
+  L.maxsteps = (double)inv->maxsteps;
+  c->lm_lds = lm_lds_bytes(np, nz, L.regul != 0);
+  if (c->lm_lds > 160 * 1024 - 512) return set_err(c, FZ_ECONFIG, "inversion set-up exceeds the LDS of one workgroup");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->d_coefs) (void)hipFree(c->d_coefs);
+  if (c->d_colsum) (void)hipFree(c->d_colsum);
+  c->d_coefs = nullptr;
+  c->d_colsum = nullptr;
+  HIPCHK(c, hipMalloc((void**)&c->d_coefs, sizeof(double) * coefs.size()));
+  HIPCHK(c, hipMalloc((void**)&c->d_colsum, sizeof(double) * np));
+  HIPCHK(c, hipMemcpy(c->d_coefs, coefs.data(), sizeof(double) * coefs.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_colsum, colsum.data(), sizeof(double) * np, hipMemcpyHostToDevice));
+  HIPCHK(c, hipFuncSetAttribute((const void*)lm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lm_lds));
+  L.coefs = c->d_coefs;
+  L.colsum = c->d_colsum;
+  c->inv_set = true;
+  if (ifreep) *ifreep = np;
+  if (nfreq) *nfreq = L.nfreq;
+  return FZ_OK;
+}
+
+int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, float* d_eqv, int32_t* d_status,
+                         void* stream) {
+  if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes || !d_eqv) return FZ_EARG;
+  if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  const Cfg& g = c->cfg;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = upload_cfg(c);
+  if (r) return r;
+  r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
+  if (r) return r;
+  const int tile = std::min(c->tile, ncol);
+  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * 4 * tile * g.numw);
+  if (r) return r;
+  r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * 7 * g.nz * 4 * g.numw);
+  if (r) return r;
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
+  hipLaunchKernelGGL(micro_kernel, dim3((ncol + 63) / 64), dim3(64), 0, st, ncol, d_atm, c->d_micro, d_status, 1);
+  HIPCHK(c, hipGetLastError());
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
+  c->rt_launches = 0;
+  EqvArgs e;
+  e.nz = g.nz;
+  e.numw = g.numw;
+  e.nfreq = c->lm.nfreq;
+  e.np = c->lm.np;
+  e.nstk = c->lm.nfreq / g.numw;
+  for (int s = 0; s < 4; ++s) e.rowstk[s] = c->lm.rowstk[s];
+  for (int j = 0; j < e.np; ++j) {
+    e.slot_of[j] = c->slot_of[j];
+    e.kbeg[j] = c->lm.kbeg[j];
+    e.kend[j] = c->lm.kend[j];
+  }
+  e.coefs = c->d_coefs;
+  e.eqv = d_eqv;
+  for (int c0 = 0; c0 < ncol; c0 += tile) {
+    RtArgs a;
+    a.ncol = ncol;
+    a.col0 = c0;
+    a.ntile = std::min(tile, ncol - c0);
+    a.atm = d_atm;
+    a.micro = c->d_micro;
+    a.gain = c->d_gain;
+    a.status = d_status;
+    a.stokes = d_stokes;
+    a.jac = c->d_jtile;
+    a.jcol0 = c0;
+    a.scratch = c->d_scratch;
+    const long nthr = (long)a.ntile * g.numw;
+    hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, dim3((unsigned)((nthr + c->block - 1) / c->block)), dim3(c->block),
+                       0, st, a);
+    HIPCHK(c, hipGetLastError());
+    e.ncol_tile = a.ntile;
+    e.col0 = c0;
+    e.jac = c->d_jtile;
+    const long ne = (long)a.ntile * e.nfreq;
+    hipLaunchKernelGGL(eqv_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, e);
+    HIPCHK(c, hipGetLastError());
+    c->rt_launches++;
+  }
+  if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
+  return FZ_OK;
+}
+
+int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stokes, double* d_best_syn, double* d_sla,
+                   const float* d_eqv, float* d_eqv_best, const double* d_isigmap, float* d_atm, float* d_atm_best,
+                   int32_t* d_done, double* d_delta, void* stream) {
+  if (!c || ncol < 1 || !d_obs || !d_stokes || !d_best_syn || !d_sla || !d_eqv || !d_eqv_best || !d_isigmap ||
+      !d_atm || !d_atm_best)
+    return FZ_EARG;
+  if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  HIPCHK(c, hipSetDevice(c->device));
+  LmArgs a = c->lm;
+  a.ncol = ncol;
+  a.obs = d_obs;
+  a.stokes = d_stokes;
+  a.best_syn = d_best_syn;
+  a.sla = d_sla;
+  a.eqv = d_eqv;
+  a.eqv_best = d_eqv_best;
+  a.isigmap = d_isigmap;
+  a.atm = d_atm;
+  a.atm_best = d_atm_best;
+  a.done = d_done;
+  a.delta = d_delta;
+  hipLaunchKernelGGL(lm_kernel, dim3(ncol), dim3(LM_THREADS), c->lm_lds, (hipStream_t)stream, a);
+  HIPCHK(c, hipGetLastError());
   return FZ_OK;
 }
 

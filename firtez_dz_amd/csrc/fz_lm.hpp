@@ -1,0 +1,605 @@
+// Per-pixel Levenberg-Marquardt step of the FIRTEZ-dz inversion on the GPU.
+//
+//   eqv_kernel : node contraction of the per-height response functions
+//                (CALCULATE_EQUIVALENT_DSYN, forward.f90:552-681): one thread
+//                per (pixel, spectral row), FP64 accumulation over the support
+//                of each node's hat function, REAL(4) store.
+//   lm_kernel  : one 256-thread workgroup per pixel, FP64 throughout except
+//                the REAL(4) model (GET_PERT_1D, get_dmodel.f90:1201-1244):
+//                chi^2 and accept/reject (:960-1032), lambda update
+//                (:917-956), Jacobian normalisation (:725-770), Gauss-Newton
+//                normal matrix and gradient streamed through LDS in chunks of
+//                64 spectral rows, damped pseudo-inverse by a parallel cyclic
+//                Jacobi eigen-decomposition in LDS (the symmetric form of the
+//                truncated SVD of lm.f90:61-133), and the limited
+//                perturbation applied through the node basis (:1715-2016).
+#pragma once
+
+namespace fz {
+
+constexpr int LM_MAXNP = 64;     // free parameters per pixel (nodes summed over parameters)
+constexpr int LM_THREADS = 256;
+constexpr int LM_CH = 64;        // spectral rows per LDS chunk
+constexpr int LM_SLOTS = (LM_MAXNP * (LM_MAXNP + 1) / 2 + LM_MAXNP + LM_THREADS - 1) / LM_THREADS;
+
+// get_dmodel.f90:31 (PARAM_NORMS) and :1903-1919 (hard bounds)
+__constant__ double c_lm_norm[8] = {6000.0, 1.0e5, 1.0e-5, 1.0e3, 1.0e3, 1.0e3, 1.0e6, 6.0e2};
+__constant__ double c_lm_low[8] = {2500.0, 1.0e-2, 1.0e-10, -5.0e4, -5.0e4, -5.0e4, -1.0e7, 1.0e-8};
+__constant__ double c_lm_upp[8] = {1.8e4, 1.0e12, 1.0e10, 5.0e4, 5.0e4, 5.0e4, 1.0e7, 1.0e12};
+
+struct LmArgs {
+  int ncol, nz, numw, nfreq, np;
+  int nparam;                  // inverted physical parameters
+  int par[8];                  // physical index 0..7 (T, Pg, rho, Bx, By, Bz, vz, P0)
+  int slot[8];                 // response-function slot in the Cartesian per-height output
+  int mcol[8];                 // model column
+  int j0[8], nj[8];            // first free parameter, nodes
+  int kbeg[LM_MAXNP], kend[LM_MAXNP];  // support of each node's hat function
+  int rowstk[4];               // Stokes index of each inverted Stokes block
+  int regul, solver, node_expansion, pen_typ[8];
+  double pen_alp[8];
+  double svdtol, inu, mpert, maxsteps;
+  const double* coefs;         // [np][nz]
+  const double* colsum;        // [np] column sums of the node basis
+  // per pixel
+  const double* obs;           // [ncol][nfreq]
+  const float* stokes;         // [ncol][4][numw] current synthesis (times GAIN1D)
+  double* best_syn;            // [ncol][nfreq]
+  double* sla;                 // [ncol][9]
+  const float* eqv;            // [ncol][np][nfreq] current equivalent response functions
+  float* eqv_best;             // [ncol][np][nfreq]
+  const double* isigmap;       // [nfreq]
+  float* atm;                  // [12][nz][ncol] current model (in: model of eqv; out: new trial)
+  float* atm_best;             // [12][nz][ncol]
+  int32_t* done;               // [ncol] AM_I_DONE
+  double* delta;               // [ncol][np] applied node perturbation, or null
+};
+
+// ---------------------------------------------------------------- node contraction
+struct EqvArgs {
+  int ncol_tile, col0, nz, numw, nfreq, np, nstk;
+  int rowstk[4];
+  int slot_of[LM_MAXNP];       // response-function slot of each free parameter
+  int kbeg[LM_MAXNP], kend[LM_MAXNP];
+  const double* coefs;         // [np][nz]
+  const float* jac;            // tile: [ntile][7][nz][4][numw]
+  float* eqv;                  // [ncol][np][nfreq]
+};
+
+__global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)a.ncol_tile * a.nfreq;
+  if (t >= n) return;
+  const int ct = (int)(t / a.nfreq);
+  const int f = (int)(t - (long)ct * a.nfreq);
+  const int b = f / a.numw, l = f - b * a.numw;
+  const int s = a.rowstk[b];
+  const long cs = (long)4 * a.numw;
+  const float* jc = a.jac + (long)ct * 7 * a.nz * cs + s * a.numw + l;
+  float* out = a.eqv + ((long)(a.col0 + ct) * a.np) * a.nfreq + f;
+  for (int j = 0; j < a.np; ++j) {
+    const float* jp = jc + (long)a.slot_of[j] * a.nz * cs;
+    const double* cf = a.coefs + (long)j * a.nz;
+    double acc = 0.0;
+    for (int k = a.kbeg[j]; k <= a.kend[j]; ++k) acc += (double)jp[k * cs] * cf[k];
+    out[(long)j * a.nfreq] = (float)acc;
+  }
+}
+
+// ---------------------------------------------------------------- LM step
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int t = threadIdx.x;
+  red[t] = v;
+  __syncthreads();
+  for (int s = LM_THREADS / 2; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// GET_IT_PENALTIES (get_dmodel.f90:1041-1197) by one thread: pen_res[np],
+// pen_hss[np*np] (row-major, block diagonal) from the model columns in LDS;
+// sm and pen are LDS work vectors of np entries.
+__device__ void lm_penalties(const LmArgs& a, const float* mod, double* pen_res, double* pen_hss, double* sm,
+                             double* pen) {
+  const int np = a.np, nz = a.nz;
+  for (int i = 0; i < np; ++i) pen_res[i] = 0.0;
+  for (int i = 0; i < np * np; ++i) pen_hss[i] = 0.0;
+  for (int ip = 0; ip < a.nparam; ++ip) {
+    const int p = a.par[ip], ji = a.j0[ip], jd = a.nj[ip];
+    const float* col = mod + a.mcol[ip] * nz;
+    const double alp = sqrt(a.pen_alp[p] / (double)np);
+    for (int j = 0; j < jd; ++j) {
+      const double* cf = a.coefs + (long)(ji + j) * nz;
+      double s = 0.0;
+      for (int k = 0; k < nz; ++k) s = s + (double)col[k] * cf[k];
+      sm[j] = s / a.colsum[ji + j] / c_lm_norm[p];
+    }
+    // L: type 0 -> diag(1/jd), type 1 -> first differences (first row empty)
+    for (int k = 0; k < jd; ++k) {
+      double v = 0.0;
+      if (a.pen_typ[p] == 0) v = (1.0 * (1.0 / (double)jd)) * alp * sm[k];
+      else if (a.pen_typ[p] == 1 && k > 0) v = (1.0 * 1.0) * alp * sm[k] + (-1.0 * 1.0) * alp * sm[k - 1];
+      pen[k] = v;
+    }
+    for (int j = 0; j < jd; ++j) {
+      double s = 0.0;
+      if (a.pen_typ[p] == 0) s = (1.0 / (double)jd) * alp * pen[j];
+      else if (a.pen_typ[p] == 1) {
+        if (j > 0) s += alp * pen[j];
+        if (j + 1 < jd) s += -alp * pen[j + 1];
+      }
+      pen_res[ji + j] = s;
+    }
+    for (int i = 0; i < jd; ++i)
+      for (int j = 0; j < jd; ++j) {
+        double s = 0.0;
+        if (a.pen_typ[p] == 0) {
+          if (i == j) s = ((1.0 / (double)jd) * alp) * ((1.0 / (double)jd) * alp);
+        } else if (a.pen_typ[p] == 1) {
+          // (L^T L)_ij = sum_k L_ki L_kj, L_kk = alp (k>0), L_k,k-1 = -alp
+          for (int k = 1; k < jd; ++k) {
+            const double lki = (k == i) ? alp : ((k - 1 == i) ? -alp : 0.0);
+            const double lkj = (k == j) ? alp : ((k - 1 == j) ? -alp : 0.0);
+            s += lki * lkj;
+          }
+        }
+        pen_hss[(ji + i) * np + (ji + j)] = s;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
+  extern __shared__ double smem[];
+  const int t = threadIdx.x;
+  const int pix = blockIdx.x;
+  if (a.done && a.done[pix] != 0) return;
+  const int np = a.np, nz = a.nz, nf = a.nfreq, numw = a.numw;
+  const int ld = np + 1;                       // LDS row stride of A and V
+  const int npe = np + (np & 1);               // even size for the tournament ordering
+  // LDS carve-up
+  double* A = smem;                            // [npe][ld]  (ld >= npe when np odd: ld = np+1 = npe)
+  double* V = A + npe * ld;
+  double* Jn = V + npe * ld;                   // [LM_CH][np]
+  double* rch = Jn + LM_CH * np;               // [LM_CH]
+  double* g = rch + LM_CH;                     // [np]
+  double* dv = g + np;                         // [np] eigenvalues / projections
+  double* del = dv + np;                       // [np]
+  double* red = del + np;                      // [LM_THREADS]
+  double* rot = red + LM_THREADS;              // [npe/2][2]
+  double* nzp = rot + npe;                     // [nz]
+  double* nod = nzp + nz;                      // [4][np] node work vectors
+  double* pen_res = nod + 4 * np;              // [np]
+  double* pen_hss = pen_res + np;              // [np*np] (only when regularising)
+  float* mod = (float*)(pen_hss + (a.regul ? np * np : 0));   // [12][nz] current model
+  float* bum = mod + 12 * nz;                  // [nz]
+  float* ftmp = bum + nz;                      // [nz]
+  int* izero = (int*)(ftmp + nz);              // [np] zeroed response columns
+  __shared__ double s_sla[9];
+  __shared__ int s_accept, s_conv;
+
+  const long ncol = a.ncol;
+  // ---- load the current model and the pixel's scalars
+  for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm[(long)i * ncol + pix];
+  if (t < 9) s_sla[t] = a.sla[(long)pix * 9 + t];
+  // columns whose absolute sum overflows REAL(4) are zeroed (forward.f90:672-678)
+  const float* ecur = a.eqv + (long)pix * np * nf;
+  for (int j = t; j < np; j += LM_THREADS) {
+    float s = 0.f;
+    for (int f = 0; f < nf; ++f) s += fabsf(ecur[(long)j * nf + f]);
+    izero[j] = (s > 3.40282347e38f) ? 1 : 0;
+  }
+  __syncthreads();
+
+  // ---- NEW_CALCULATE_CHI2 (get_dmodel.f90:960-1032)
+  const double* obs = a.obs + (long)pix * nf;
+  const float* stk = a.stokes + (long)pix * 4 * numw;
+  double part = 0.0;
+  for (int f = t; f < nf; f += LM_THREADS) {
+    const int b = f / numw, l = f - b * numw;
+    const double d = obs[f] - (double)stk[a.rowstk[b] * numw + l];
+    part += d * d * a.isigmap[f];
+  }
+  double chi = block_sum(part, red) / a.inu;
+  if (a.regul) {
+    if (t == 0) lm_penalties(a, mod, pen_res, pen_hss, nod, nod + np);
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += pen_res[i] * 1.0 * 1.0 * pen_res[i];
+    chi = chi + s;
+  }
+  if (t == 0) s_accept = chi < s_sla[2] ? 1 : 0;
+  __syncthreads();
+  const bool accept = s_accept != 0;
+  double* bsyn = a.best_syn + (long)pix * nf;
+  float* ebest = a.eqv_best + (long)pix * np * nf;
+  if (accept) {
+    // best := current (model, equivalent response functions, synthesis)
+    for (int i = t; i < 12 * nz; i += LM_THREADS) a.atm_best[(long)i * ncol + pix] = mod[i];
+    for (long i = t; i < (long)np * nf; i += LM_THREADS) ebest[i] = izero[i / nf] ? 0.f : ecur[i];
+    for (int f = t; f < nf; f += LM_THREADS) {
+      const int b = f / numw, l = f - b * numw;
+      bsyn[f] = (double)stk[a.rowstk[b] * numw + l];
+    }
+  } else {
+    // current := best
+    for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm_best[(long)i * ncol + pix];
+  }
+  __syncthreads();
+  if (!accept && a.regul) {
+    if (t == 0) lm_penalties(a, mod, pen_res, pen_hss, nod, nod + np);
+  }
+  if (t == 0) {
+    // UPDATE_LAMBDA (get_dmodel.f90:917-956) with INV_SLA(2,3,9,1,4), factor 0.01
+    double chicur = chi, chipre = s_sla[2], lam = s_sla[8], steps = s_sla[0], precyc = s_sla[3];
+    if ((chipre - chicur) > 0) {
+      if ((chipre - chicur) > 0.01 * chipre) steps = fmax(0.0, steps - 1.0);
+      else steps = steps + 1.0;
+      chipre = chicur;
+      lam = lam / sqrt(10.0);
+    } else {
+      if (fabs(precyc - 1.0) < 1.0e-2) {
+        precyc = 0.0;
+        lam = lam * 3.0;
+      } else {
+        lam = lam * 10.0;
+      }
+      steps = steps + 1.0;
+    }
+    lam = fmax(1.0e-2, lam);
+    s_sla[0] = steps;
+    s_sla[1] = chicur;
+    s_sla[2] = chipre;
+    s_sla[3] = precyc;
+    s_sla[8] = lam;
+  }
+  __syncthreads();
+
+  // ---- normal matrix and gradient of the normalised problem
+  //      J_n = J sqrt(w) / sqrt(nu) * PARAM_NORM,  r = (obs - best) sqrt(w) / sqrt(nu)
+  const float* jsrc = accept ? ecur : ebest;
+  const double snu = sqrt(a.inu);
+  const int ntri = np * (np + 1) / 2, nq = ntri + np;
+  double acc[LM_SLOTS];
+  int qi[LM_SLOTS], qj[LM_SLOTS];
+#pragma unroll
+  for (int u = 0; u < LM_SLOTS; ++u) {
+    acc[u] = 0.0;
+    const int q = t + u * LM_THREADS;
+    int i = -1, j = -1;
+    if (q < ntri) {
+      // row-major upper triangle: find i with q in [i*np - i(i-1)/2, ...)
+      int r = 0, base = 0;
+      while (q >= base + (np - r)) {
+        base += np - r;
+        ++r;
+      }
+      i = r;
+      j = r + (q - base);
+    } else if (q < nq) {
+      i = q - ntri;
+      j = -2;  // gradient entry
+    }
+    qi[u] = i;
+    qj[u] = j;
+  }
+  for (int f0 = 0; f0 < nf; f0 += LM_CH) {
+    const int nc = min(LM_CH, nf - f0);
+    for (int e = t; e < nc * np; e += LM_THREADS) {
+      const int c = e / np, j = e - c * np;
+      const int f = f0 + c;
+      double v = (accept && izero[j]) ? 0.0 : (double)jsrc[(long)j * nf + f];
+      int ip = 0;
+      while (j >= a.j0[ip] + a.nj[ip]) ++ip;
+      Jn[c * np + j] = v * sqrt(a.isigmap[f]) / snu * (1.0 * c_lm_norm[a.par[ip]]);
+    }
+    for (int c = t; c < nc; c += LM_THREADS) {
+      const int f = f0 + c;
+      rch[c] = (obs[f] - bsyn[f]) * sqrt(a.isigmap[f]) / snu;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < LM_SLOTS; ++u) {
+      const int i = qi[u], j = qj[u];
+      if (i < 0) continue;
+      double s = acc[u];
+      if (j >= 0) {
+        for (int c = 0; c < nc; ++c) s += Jn[c * np + i] * Jn[c * np + j];
+      } else {
+        for (int c = 0; c < nc; ++c) s += Jn[c * np + i] * rch[c];
+      }
+      acc[u] = s;
+    }
+    __syncthreads();
+  }
+  const double lam = s_sla[8];
+  // A = H + REG_HSS, diag += diag * lambda; g = J^T r - REG_PEN
+  for (int e = t; e < npe * ld; e += LM_THREADS) A[e] = 0.0;
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < LM_SLOTS; ++u) {
+    const int i = qi[u], j = qj[u];
+    if (i < 0) continue;
+    if (j >= 0) {
+      A[i * ld + j] = acc[u];
+      A[j * ld + i] = acc[u];
+    } else {
+      g[i] = acc[u] - (a.regul ? pen_res[i] : 0.0);
+    }
+  }
+  __syncthreads();
+  if (a.regul)
+    for (int e = t; e < np * np; e += LM_THREADS) {
+      const int i = e / np, j = e - i * np;
+      A[i * ld + j] += pen_hss[e];
+    }
+  __syncthreads();
+  for (int i = t; i < np; i += LM_THREADS) A[i * ld + i] = A[i * ld + i] + A[i * ld + i] * lam;
+  for (int e = t; e < npe * ld; e += LM_THREADS) {
+    const int i = e / ld, j = e - i * ld;
+    V[e] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+
+  // ---- symmetric eigen-decomposition, parallel cyclic Jacobi (tournament order)
+  if (a.solver == 1) {
+    const int half = npe / 2;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+      double o = 0.0, tt = 0.0;
+      for (int e = t; e < np * np; e += LM_THREADS) {
+        const int i = e / np, j = e - i * np;
+        const double x = A[i * ld + j];
+        tt += x * x;
+        if (i != j) o += x * x;
+      }
+      const double off = block_sum(o, red), tot = block_sum(tt, red);
+      if (off <= 1e-30 * tot || off == 0.0) break;
+      for (int rnd = 0; rnd < npe - 1; ++rnd) {
+        // pair i: players L[i], L[npe-1-i] with L = [0, (rnd+k)%(npe-1)+1 ...]
+        if (t < half) {
+          const int i = t;
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          double cs = 1.0, sn = 0.0;
+          if (q < np) {
+            const double apq = A[p * ld + q];
+            if (apq != 0.0) {
+              const double th = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
+              const double tq = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+              cs = 1.0 / sqrt(1.0 + tq * tq);
+              sn = tq * cs;
+            }
+          }
+          rot[2 * i] = cs;
+          rot[2 * i + 1] = sn;
+        }
+        __syncthreads();
+        // columns of A and V: A <- A J, V <- V J
+        for (int e = t; e < half * npe; e += LM_THREADS) {
+          const int i = e / npe, k = e - i * npe;
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          const double cs = rot[2 * i], sn = rot[2 * i + 1];
+          if (sn == 0.0 && cs == 1.0) continue;
+          const double akp = A[k * ld + p], akq = A[k * ld + q];
+          A[k * ld + p] = cs * akp - sn * akq;
+          A[k * ld + q] = sn * akp + cs * akq;
+          const double vkp = V[k * ld + p], vkq = V[k * ld + q];
+          V[k * ld + p] = cs * vkp - sn * vkq;
+          V[k * ld + q] = sn * vkp + cs * vkq;
+        }
+        __syncthreads();
+        // rows of A: A <- J^T A
+        for (int e = t; e < half * npe; e += LM_THREADS) {
+          const int i = e / npe, k = e - i * npe;
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          const double cs = rot[2 * i], sn = rot[2 * i + 1];
+          if (sn == 0.0 && cs == 1.0) continue;
+          const double apk = A[p * ld + k], aqk = A[q * ld + k];
+          A[p * ld + k] = cs * apk - sn * aqk;
+          A[q * ld + k] = sn * apk + cs * aqk;
+        }
+        __syncthreads();
+      }
+    }
+    // truncated pseudo-inverse: keep |d| >= svdtol * max|d|
+    double mx = 0.0;
+    for (int i = 0; i < np; ++i) mx = fmax(mx, fabs(A[i * ld + i]));
+    const double thr = mx * (a.svdtol > 0 ? a.svdtol : 10.0e-4);
+    for (int e = t; e < np; e += LM_THREADS) {
+      const double d = A[e * ld + e];
+      double pr = 0.0;
+      if (!(fabs(d) < thr)) {
+        for (int i = 0; i < np; ++i) pr += V[i * ld + e] * g[i];
+        pr /= d;
+      }
+      dv[e] = pr;
+    }
+    __syncthreads();
+    for (int i = t; i < np; i += LM_THREADS) {
+      double s = 0.0;
+      for (int e = 0; e < np; ++e) s += V[i * ld + e] * dv[e];
+      del[i] = s;
+    }
+  } else {
+    // NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453), by one thread
+    if (t == 0) {
+      double* xv = nod;
+      double* pv = nod + np;
+      double* rv = nod + 2 * np;
+      double* qv = nod + 3 * np;
+      double* ck = dv;
+      for (int i = 0; i < np; ++i) {
+        xv[i] = 0.0;
+        pv[i] = g[i];
+        rv[i] = g[i];   // bet - alp x with x = 0
+      }
+      double crho = 0.0;
+      for (int i = 0; i < np; ++i) crho += rv[i] * rv[i];
+      for (int i = 0; i < np; ++i) {
+        double s = 0.0;
+        for (int j = 0; j < np; ++j) s += A[i * ld + j] * pv[j];
+        qv[i] = s;
+      }
+      double calpha = 0.0;
+      for (int i = 0; i < np; ++i) calpha += pv[i] * qv[i];
+      calpha = crho / calpha;
+      for (int i = 0; i < np; ++i) xv[i] = xv[i] + calpha * pv[i];
+      for (int i = 0; i < np; ++i) rv[i] = rv[i] - calpha * qv[i];
+      double pcond = 1.0e29, dcond = -1.0e29;
+      bool conv = false;
+      for (int it = 1; it <= 100000; ++it) {
+        const double crho0 = crho;
+        crho = 0.0;
+        for (int i = 0; i < np; ++i) crho += rv[i] * rv[i];
+        const double cbeta = crho / crho0;
+        for (int i = 0; i < np; ++i) pv[i] = rv[i] + cbeta * pv[i];
+        for (int i = 0; i < np; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < np; ++j) s += A[i * ld + j] * pv[j];
+          qv[i] = s;
+        }
+        calpha = 0.0;
+        for (int i = 0; i < np; ++i) calpha += pv[i] * qv[i];
+        calpha = crho / calpha;
+        for (int i = 0; i < np; ++i) xv[i] = xv[i] + calpha * pv[i];
+        for (int i = 0; i < np; ++i) rv[i] = rv[i] - calpha * qv[i];
+        double cond = 0.0;
+        for (int i = 0; i < np; ++i) {
+          double s = 0.0;
+          for (int j = 0; j < np; ++j) s += A[i * ld + j] * xv[j];
+          ck[i] = s;
+          const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
+          cond += e * e;
+        }
+        cond = cond / (double)np;
+        if (cond < 1.0e-3 || dcond > 1.0e30) {
+          for (int i = 0; i < np; ++i) del[i] = xv[i];
+          conv = true;
+          break;
+        }
+        dcond = cond - pcond;
+        pcond = cond;
+      }
+      if (!conv)
+        for (int i = 0; i < np; ++i) del[i] = xv[i] / 10.0;
+    }
+  }
+  __syncthreads();
+  // NaN filter (get_dmodel.f90:806-822)
+  if (t == 0) {
+    double s = 0.0;
+    for (int i = 0; i < np; ++i) s += del[i];
+    if (s != s)
+      for (int i = 0; i < np; ++i)
+        if (del[i] != del[i]) del[i] = 0.0;
+  }
+  __syncthreads();
+
+  // ---- LIMIT_PERTURBATION -> REARANGE_DA_MODEL -> SORT_DELTA_TO_ATMOSPHERE
+  double* itd = nod;            // [np] scaled / limited node perturbation
+  double* avg = nod + np;       // [np] node means of the model
+  double* sumv = nod + 2 * np;  // [np]
+  for (int ip = 0; ip < a.nparam; ++ip) {
+    const int p = a.par[ip], j0 = a.j0[ip], nj = a.nj[ip];
+    float* col = mod + a.mcol[ip] * nz;
+    const double factor = c_lm_norm[p], inf = c_lm_low[p], sup = c_lm_upp[p];
+    const double absolute = c_lm_norm[p] * a.mpert;
+    for (int k = t; k < nz; k += LM_THREADS) bum[k] = col[k];
+    for (int j = t; j < nj; j += LM_THREADS) {
+      const double* cf = a.coefs + (long)(j0 + j) * nz;
+      double s = 0.0;
+      for (int k = 0; k < nz; ++k) s = s + (double)col[k] * cf[k];
+      const double av = s / a.colsum[j0 + j];
+      double lim;
+      if (absolute > 0) lim = fmax(absolute, fabs(a.mpert * av));
+      else lim = fabs(a.mpert * av);
+      if (av == 0) lim = absolute;
+      double x = del[j0 + j] * factor;
+      if (fabs(x) > lim) x = x / fabs(x) * lim;
+      itd[j] = x;
+      avg[j] = av;
+      sumv[j] = av + x;
+      del[j0 + j] = x;
+    }
+    __syncthreads();
+    for (int k = t; k < nz; k += LM_THREADS) {
+      double s = 0.0;
+      for (int j = 0; j < nj; ++j) s = s + sumv[j] * a.coefs[(long)(j0 + j) * nz + k];
+      nzp[k] = s;
+    }
+    __syncthreads();
+    if (a.node_expansion == 2) {
+      for (int k = t; k < nz; k += LM_THREADS) {
+        float o = (float)nzp[k];
+        if ((double)o < inf) o = (float)inf;
+        if ((double)o > sup) o = (float)sup;
+        col[k] = o - col[k];
+      }
+    } else if (a.node_expansion == 1) {
+      for (int k = t; k < nz; k += LM_THREADS) {
+        nzp[k] = nzp[k] - (double)col[k];
+        ftmp[k] = (float)nzp[k];
+      }
+      __syncthreads();
+      for (int j = t; j < nj; j += LM_THREADS) {
+        const double* cf = a.coefs + (long)(j0 + j) * nz;
+        double s = 0.0;
+        for (int k = 0; k < nz; ++k) s = s + (double)ftmp[k] * cf[k];
+        itd[j] = s / a.colsum[j0 + j];
+        del[j0 + j] = itd[j];
+      }
+      __syncthreads();
+      for (int k = t; k < nz; k += LM_THREADS) {
+        double s = 0.0;
+        for (int j = 0; j < nj; ++j) s = s + itd[j] * a.coefs[(long)(j0 + j) * nz + k];
+        const float o = col[k] + (float)s;
+        if ((double)o < inf) s = inf - (double)col[k];
+        if ((double)o > sup) s = sup - (double)col[k];
+        col[k] = (float)s;
+      }
+    }
+    __syncthreads();
+    if (p == 0 && t == 0) {
+      // temperature tail rule in log tau5 (get_dmodel.f90:1970-1984)
+      const float* tau = mod + A_TAU * nz;
+      const float tllim = -11.5f, tulim = 0.5f;
+      for (int k = nz - 1; k >= 0; --k) {
+        if (tau[k] < tllim) {
+          const float x = fabsf(tau[k] - tllim) / 0.5f;
+          col[k] = expf(-(x * x)) * col[k];
+        } else if (tau[k] > tulim) {
+          if (k + 1 < nz) col[k] = col[k + 1];
+        }
+      }
+    }
+    __syncthreads();
+    for (int k = t; k < nz; k += LM_THREADS) col[k] = col[k] + bum[k];
+    __syncthreads();
+  }
+
+  // ---- write back
+  for (int i = t; i < 12 * nz; i += LM_THREADS) a.atm[(long)i * ncol + pix] = mod[i];
+  if (t < 9) a.sla[(long)pix * 9 + t] = s_sla[t];
+  if (a.delta)
+    for (int i = t; i < np; i += LM_THREADS) a.delta[(long)pix * np + i] = del[i];
+  if (t == 0 && a.done && s_sla[0] > a.maxsteps) a.done[pix] = 1;
+}
+
+// LDS bytes of lm_kernel for np free parameters and nz heights
+inline size_t lm_lds_bytes(int np, int nz, bool regul) {
+  const int npe = np + (np & 1), ld = np + 1;
+  size_t d = (size_t)2 * npe * ld + (size_t)LM_CH * np + LM_CH + 3 * np + LM_THREADS + npe + nz + 4 * np + np +
+             (regul ? (size_t)np * np : 0);
+  size_t f = (size_t)12 * nz + 2 * nz + np;   // floats + ints
+  return d * 8 + f * 4;
+}
+
+}  // namespace fz

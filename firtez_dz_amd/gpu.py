@@ -36,6 +36,19 @@ class FzError(RuntimeError):
     pass
 
 
+class CInversion(ctypes.Structure):
+    """fz_inversion (include/firtez_gpu.h)."""
+    _fields_ = [("atmpar", ctypes.c_int32 * 8), ("invstk", ctypes.c_int32 * 4), ("nodes", ctypes.c_int32 * 8),
+                ("pen_typ", ctypes.c_int32 * 8), ("pen_alp", ctypes.c_double * 8),
+                ("regul", ctypes.c_int32), ("solver", ctypes.c_int32), ("node_expansion", ctypes.c_int32),
+                ("maxsteps", ctypes.c_int32), ("svdtol", ctypes.c_double), ("inu", ctypes.c_double),
+                ("mpert", ctypes.c_double)]
+
+
+# inverted physical parameters, in the reference's INV_ATMPAR order
+INV_PARAMS = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "P0")
+
+
 def library():
     """Load libfirtez_gpu.so (built by firtez_dz_amd.build); raise if absent."""
     global _lib
@@ -68,6 +81,11 @@ def library():
     L.fz_set_profiling.argtypes = [vp, ip]
     L.fz_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                  ctypes.POINTER(ctypes.c_int)]
+    L.fz_int_coefs.argtypes = [ip, ip, dp]
+    L.fz_set_inversion.argtypes = [vp, ctypes.POINTER(CInversion), ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int)]
+    L.fz_forward_nodes_dev.argtypes = [vp, ip, vp, vp, vp, vp, vp]
+    L.fz_lm_step_dev.argtypes = [vp, ip] + [vp] * 12
     _lib = L
     return L
 
@@ -76,7 +94,35 @@ def exported_symbols():
     """Names declared in include/firtez_gpu.h."""
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
-            "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing"]
+            "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
+            "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev"]
+
+
+def int_coefs(nz: int, nj: int) -> np.ndarray:
+    """Node basis COEFS (nz, nj) of GET_INT_COEFS (get_dmodel.f90:563-644)."""
+    c = np.zeros((nj, nz))
+    rc = library().fz_int_coefs(nz, nj, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if rc:
+        raise FzError("fz_int_coefs: bad sizes nz=%d nj=%d" % (nz, nj))
+    return c.T.copy()
+
+
+def inversion(nodes: dict, invstk=(1, 1, 1, 1), solver=1, node_expansion=1, maxsteps=5, svdtol=0.0, inu=0.0,
+              mpert=0.0, regul=False, pen_typ: Optional[dict] = None, pen_alp: Optional[dict] = None) -> CInversion:
+    """fz_inversion from {parameter name: nodes}, e.g. {"T": 2, "Bz": 1, "vz": 1}."""
+    c = CInversion()
+    for name, n in nodes.items():
+        i = INV_PARAMS.index(name)
+        c.atmpar[i] = 1
+        c.nodes[i] = int(n)
+    c.invstk[:] = [int(x) for x in invstk]
+    for name, v in (pen_typ or {}).items():
+        c.pen_typ[INV_PARAMS.index(name)] = int(v)
+    for name, v in (pen_alp or {}).items():
+        c.pen_alp[INV_PARAMS.index(name)] = float(v)
+    c.regul, c.solver, c.node_expansion, c.maxsteps = int(regul), solver, node_expansion, maxsteps
+    c.svdtol, c.inu, c.mpert = svdtol, inu, mpert
+    return c
 
 
 def _f(a):
@@ -188,6 +234,54 @@ class Context:
             status = torch.zeros(ncol, dtype=torch.int32, device=atm.device)
         self.hydrostatic_dev(ncol, atm.data_ptr(), status.data_ptr(), torch.cuda.current_stream(atm.device).cuda_stream)
         return status
+
+    # ---------------------------------------------------------------- inversion
+    def set_inversion(self, inv: CInversion):
+        """Configure the inversion cycle; returns (IFREEP, NFREQ)."""
+        npr, nf = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.fz_set_inversion(self.h, ctypes.byref(inv), ctypes.byref(npr), ctypes.byref(nf)),
+                    "fz_set_inversion")
+        self.ifreep, self.nfreq = npr.value, nf.value
+        return self.ifreep, self.nfreq
+
+    def forward_nodes_torch(self, atm, stokes=None, eqv=None, status=None):
+        """FORWARD3D in inversion mode: atm [12, nz, ncol] float32 cuda (updated
+        in place) -> (stokes [ncol,4,numw], eqv [ncol,IFREEP,NFREQ], status)."""
+        import torch
+        assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous()
+        ncol = atm.shape[2]
+        dev = atm.device
+        if stokes is None:
+            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=dev)
+        if eqv is None:
+            eqv = torch.empty((ncol, self.ifreep, self.nfreq), dtype=torch.float32, device=dev)
+        if status is None:
+            status = torch.zeros(ncol, dtype=torch.int32, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        self._check(self.lib.fz_forward_nodes_dev(self.h, ncol, atm.data_ptr(), stokes.data_ptr(), eqv.data_ptr(),
+                                                  status.data_ptr(), stream), "fz_forward_nodes_dev")
+        return stokes, eqv, status
+
+    def lm_step_torch(self, obs, stokes, best_syn, sla, eqv, eqv_best, isigmap, atm, atm_best, done=None,
+                      delta=None):
+        """One GET_PERT_1D per pixel (see fz_lm_step_dev); all arguments torch
+        cuda tensors, updated in place."""
+        import torch
+        ncol = atm.shape[2]
+        stream = torch.cuda.current_stream(atm.device).cuda_stream
+        for name, t, dt in (("obs", obs, torch.float64), ("best_syn", best_syn, torch.float64),
+                            ("sla", sla, torch.float64), ("isigmap", isigmap, torch.float64),
+                            ("stokes", stokes, torch.float32), ("eqv", eqv, torch.float32),
+                            ("eqv_best", eqv_best, torch.float32), ("atm", atm, torch.float32),
+                            ("atm_best", atm_best, torch.float32)):
+            if not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+                raise FzError("%s must be a contiguous %s cuda tensor" % (name, dt))
+        self._check(self.lib.fz_lm_step_dev(self.h, ncol, obs.data_ptr(), stokes.data_ptr(), best_syn.data_ptr(),
+                                            sla.data_ptr(), eqv.data_ptr(), eqv_best.data_ptr(), isigmap.data_ptr(),
+                                            atm.data_ptr(), atm_best.data_ptr(),
+                                            done.data_ptr() if done is not None else None,
+                                            delta.data_ptr() if delta is not None else None, stream),
+                    "fz_lm_step_dev")
 
     # ---------------------------------------------------------------- FORWARD3D mirror
     def forward3d(self, cube: dict, am_i_done=None, rf_mode=RF_NONE):

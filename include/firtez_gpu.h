@@ -119,6 +119,61 @@ int fz_last_timing(fz_ctx* ctx, float* micro_ms, float* rt_ms, int* rt_launches)
  * tile); the autotuner sets it.  Default 8192. */
 int fz_set_tile(fz_ctx* ctx, int columns_per_launch, int block_threads);
 
+
+/* ------------------------------------------------------------------ inversion */
+
+/* Per-cycle inversion set-up: which parameters and Stokes parameters are
+ * inverted, the nodes of each parameter in this cycle, regularisation and
+ * solver.  Mirrors INV_ATMPAR, INV_STK, NSLAB_PER_FREEV, VREGULARIZATION,
+ * PEN_TYP, PEN_ALP, SOLVER_METHOD, NODE_EXPANSION, RSVDTOL, INU, MPERT and
+ * MAXSTEPS (invert_param.f90, code_modes.f90, input_file.f90:421-453,
+ * get_dmodel.f90:415-421).  P0 (atmpar[7]) is not supported. */
+typedef struct {
+  int32_t atmpar[8];     /* T, Pg, rho, Bx, By, Bz, v_z, P0 */
+  int32_t invstk[4];     /* I, Q, U, V */
+  int32_t nodes[8];      /* nodes of each inverted parameter (1..nz) */
+  int32_t pen_typ[8];    /* 0: amplitude, 1: first differences */
+  double pen_alp[8];
+  int32_t regul;         /* VREGULARIZATION */
+  int32_t solver;        /* 1: truncated SVD pseudo-inverse, 2: CG iteration */
+  int32_t node_expansion;/* 1: SIR/NICOLE re-projection, 2: direct */
+  int32_t maxsteps;      /* pixels with more steps without improvement are done */
+  double svdtol;         /* RSVDTOL (0 -> 1e-4) */
+  double inu;            /* INU (0 -> NFREQ) */
+  double mpert;          /* MPERT (0 -> 0.15) */
+} fz_inversion;
+
+/* Node basis COEFS(nz, nj) of GET_INT_COEFS (get_dmodel.f90:563-644):
+ * hat functions on nj equidistant height indices; coefs[j*nz + k]. Host only. */
+int fz_int_coefs(int nz, int nj, double* coefs);
+
+/* Configure the inversion of this context (after fz_configure); returns the
+ * number of free parameters IFREEP through *ifreep and NFREQ through *nfreq. */
+int fz_set_inversion(fz_ctx* ctx, const fz_inversion* inv, int* ifreep, int* nfreq);
+
+/* FORWARD3D in inversion mode on device data: Stokes as fz_forward_dev and
+ * the node-contracted response functions EQVDSYN (forward.f90:552-681) in
+ * d_eqv[ncol][IFREEP][NFREQ] float (times GAIN1D). */
+int fz_forward_nodes_dev(fz_ctx* ctx, int ncol, float* d_atm, float* d_stokes, float* d_eqv,
+                         int32_t* d_status, void* stream);
+
+/* One Levenberg-Marquardt step for every pixel with d_done[pix] == 0
+ * (GET_DMODEL3DS / GET_PERT_1D, get_dmodel.f90:1327-1566, 1201-1244):
+ *   d_obs      [ncol][NFREQ] double   observed profiles (OBS3D)
+ *   d_stokes   [ncol][4][NUMW] float  current synthesis (fz_forward_nodes_dev)
+ *   d_best_syn [ncol][NFREQ] double   in/out (BEST_SYN)
+ *   d_sla      [ncol][9] double       in/out (INV_MAS(1:9): steps, chi2,
+ *                                     previous chi2, PRECYC, ..., lambda)
+ *   d_eqv      [ncol][IFREEP][NFREQ]  current EQVDSYN; d_eqv_best in/out
+ *   d_isigmap  [NFREQ] double         weights WSTK^2/ISIGMA^2
+ *   d_atm      [12][nz][ncol]         in: current model; out: next trial
+ *   d_atm_best [12][nz][ncol]         in/out: best model
+ *   d_done     [ncol] int32           in/out: AM_I_DONE (set when steps > maxsteps)
+ *   d_delta    [ncol][IFREEP] double  applied node perturbation, or NULL */
+int fz_lm_step_dev(fz_ctx* ctx, int ncol, const double* d_obs, const float* d_stokes, double* d_best_syn,
+                   double* d_sla, const float* d_eqv, float* d_eqv_best, const double* d_isigmap,
+                   float* d_atm, float* d_atm_best, int32_t* d_done, double* d_delta, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

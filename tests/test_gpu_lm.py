@@ -1,0 +1,221 @@
+"""GPU parity of the inversion step (node-contracted response functions and
+the per-pixel Levenberg-Marquardt step) against the oracle.
+
+  * EQVDSYN: the device contraction of its own per-height response functions
+    against the oracle's FP64 forward + contraction (RF tolerance, relative to
+    the largest entry of each column);
+  * one LM step, accept and reject paths: device and oracle fed the same
+    inputs (the device synthesis and EQVDSYN) must agree on chi^2, lambda and
+    the step count, the applied node perturbation and the new models;
+  * an LM loop on the device lowers chi^2 like the oracle's loop does.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from firtez_dz_amd import atmos, gpu, lines
+
+pytestmark = pytest.mark.gpu
+STOKES_TOL = 1e-4
+RF_TOL = 2e-3
+CART = (1, 1, 1, 1, 1, 1, 1, 0)
+NOISE = 1e-3
+
+
+def report(name, err, tol):
+    print("[parity] %-48s max err %.3e (tol %.1e)" % (name, err, tol), flush=True)
+    assert err <= tol, name
+
+
+def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7):
+    import torch
+    assert torch.cuda.is_available()
+    ls = lines.fe6301_6302()
+    atm, z = atmos.seeded(nx, ny, nz, seed=seed)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t).cpu().numpy() == 0)
+    true = t.cpu().numpy()
+    ncol = nx * ny
+    cpu.setup(ls, resp=True, atmpar=CART)
+    cpu.set_gain(gain)
+    models = np.stack([atmos.column_model(true, z, c) for c in range(ncol)])
+    _, syn, _ = cpu.forward_batch(models)
+    numw = gpu_ctx.numw
+    rng = np.random.default_rng(seed + 100)
+    obs = np.transpose(syn, (0, 2, 1)).reshape(ncol, 4 * numw) + rng.normal(0, NOISE, (ncol, 4 * numw))
+    guess = true.copy()
+    guess[0] += 250.0 * np.linspace(1.0, 0.2, nz, dtype=np.float32)[:, None]
+    guess[5:8] *= np.float32(0.7)
+    guess[10] += np.float32(1.0e5)
+    inv = gpu.inversion(nodes)
+    ifreep, nfreq = gpu_ctx.set_inversion(inv)
+    atmpar = [0] * 8
+    nd = [0] * 8
+    for k, v in nodes.items():
+        i = gpu.INV_PARAMS.index(k)
+        atmpar[i], nd[i] = 1, v
+    cfg = oracle.lm_cfg(nz, numw, atmpar, nd)
+    coefs = cpu.coefs_block(cfg, {i: cpu.int_coefs(nz, nd[i]) for i in range(8) if atmpar[i]})
+    return dict(z=z, gain=gain, true=true, obs=obs, guess=guess, cfg=cfg, coefs=coefs, ifreep=ifreep,
+                nfreq=nfreq, numw=numw, ncol=ncol, nz=nz, ls=ls)
+
+
+NODES_C1 = {"T": 2, "Bx": 1, "By": 1, "Bz": 1, "vz": 1}
+NODES_C2 = {"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}
+
+
+@pytest.mark.parametrize("nodes", [NODES_C1, NODES_C2], ids=["cycle1", "cycle2"])
+def test_eqvdsyn_vs_oracle(gpu_ctx, cpu, nodes):
+    import torch
+    P = problem(gpu_ctx, cpu, nodes)
+    t = torch.from_numpy(P["guess"].copy()).cuda()
+    st, eqv, status = gpu_ctx.forward_nodes_torch(t)
+    torch.cuda.synchronize()
+    st, eqv, status, atm = st.cpu().numpy(), eqv.cpu().numpy(), status.cpu().numpy(), t.cpu().numpy()
+    assert np.all(status == 0)
+    assert eqv.shape == (P["ncol"], P["ifreep"], P["nfreq"])
+    worst_s = worst_j = 0.0
+    for c in range(P["ncol"]):
+        m = atmos.column_model(atm, P["z"], c)
+        _, syn, dsyn, _ = cpu.forward1d(m)
+        worst_s = max(worst_s, np.max(np.abs(st[c] - syn)))
+        ref = cpu.eqv_dsyn(P["cfg"], P["coefs"], dsyn)          # (nfreq, ifreep)
+        for j in range(P["ifreep"]):
+            sc = np.abs(ref[:, j]).max()
+            if sc > 0:
+                worst_j = max(worst_j, np.max(np.abs(eqv[c, j] - ref[:, j])) / sc)
+    report("inversion forward Stokes vs oracle", worst_s, STOKES_TOL)
+    report("EQVDSYN (%d free parameters) vs oracle" % P["ifreep"], worst_j, RF_TOL)
+
+
+def _oracle_step(cpu, P, obs, syn, bsyn, sla, eqv, ebest, isig, cur, best):
+    out = []
+    for c in range(P["ncol"]):
+        mc = atmos.column_model(cur, P["z"], c)
+        mb = atmos.column_model(best, P["z"], c)
+        r = cpu.lm_step(P["cfg"], P["coefs"], obs[c], syn[c], bsyn[c], sla[c], eqv[c].T, ebest[c].T, isig, mc, mb)
+        out.append(r)
+    return out
+
+
+@pytest.mark.parametrize("nodes", [NODES_C1, NODES_C2], ids=["cycle1", "cycle2"])
+def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes):
+    import torch
+    P = problem(gpu_ctx, cpu, nodes)
+    ncol, nf, npr, numw = P["ncol"], P["nfreq"], P["ifreep"], P["numw"]
+    d = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    atm = d(P["guess"], torch.float32)
+    st, eqv, status = gpu_ctx.forward_nodes_torch(atm)
+    obs = d(P["obs"])
+    isig = d(np.full(nf, 1.0 / NOISE ** 2))
+    sla0 = np.tile(np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]), (ncol, 1))
+    sla = d(sla0)
+    bsyn = d(np.zeros((ncol, nf)))
+    ebest = torch.zeros_like(eqv)
+    best = atm.clone()
+    done = torch.zeros(ncol, dtype=torch.int32, device=atm.device)
+    delta = torch.zeros((ncol, npr), dtype=torch.float64, device=atm.device)
+    torch.cuda.synchronize()
+    # host copies of the inputs, fed identically to the oracle
+    h_st = st.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
+    h_eqv = eqv.cpu().numpy()
+    h_cur, h_best = atm.cpu().numpy(), best.cpu().numpy()
+    ref = _oracle_step(cpu, P, P["obs"], h_st, np.zeros((ncol, nf)), sla0, h_eqv, np.zeros_like(h_eqv),
+                       np.full(nf, 1.0 / NOISE ** 2), h_cur, h_best)
+    gpu_ctx.lm_step_torch(obs, st, bsyn, sla, eqv, ebest, isig, atm, best, done, delta)
+    torch.cuda.synchronize()
+    g_sla, g_cur, g_best, g_del = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy(), delta.cpu().numpy()
+    g_eb, g_bs = ebest.cpu().numpy(), bsyn.cpu().numpy()
+    werr = dict(chi=0.0, delta=0.0, model=0.0)
+    for c in range(ncol):
+        mc, mb, sl, bs, jb, dl = ref[c]
+        assert g_sla[c, 0] == sl[0] and g_sla[c, 8] == pytest.approx(sl[8], rel=1e-14)
+        werr["chi"] = max(werr["chi"], abs(g_sla[c, 1] - sl[1]) / sl[1])
+        assert np.array_equal(g_best[:, :, c], h_cur[:, :, c])         # accepted: best := current
+        assert np.array_equal(g_eb[c], h_eqv[c]) and np.array_equal(g_bs[c], h_st[c])
+        werr["delta"] = max(werr["delta"], np.max(np.abs(g_del[c] - dl) / (np.abs(dl) + 1e-6 * np.abs(dl).max())))
+        for row in (0, 5, 6, 7, 10):
+            werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
+                                                      (np.abs(mc[row]) + 1e-3)))
+    report("LM step (accept) chi2 vs oracle (relative)", werr["chi"], 1e-11)
+    report("LM step (accept) node perturbation vs oracle", werr["delta"], 1e-6)
+    report("LM step (accept) new model vs oracle (relative)", werr["model"], 2e-6)
+
+    # second step with a worse synthesis: every pixel rejects
+    worse = (st + 0.05).contiguous()
+    h_worse = worse.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
+    h_sla, h_cur2, h_best2 = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy()
+    h_bs2, h_eb2, h_eqv2 = bsyn.cpu().numpy(), ebest.cpu().numpy(), eqv.cpu().numpy()
+    ref2 = _oracle_step(cpu, P, P["obs"], h_worse, h_bs2, h_sla, h_eqv2, h_eb2, np.full(nf, 1.0 / NOISE ** 2),
+                        h_cur2, h_best2)
+    gpu_ctx.lm_step_torch(obs, worse, bsyn, sla, eqv, ebest, isig, atm, best, done, delta)
+    torch.cuda.synchronize()
+    g_sla, g_cur, g_best, g_del = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy(), delta.cpu().numpy()
+    werr = dict(delta=0.0, model=0.0)
+    for c in range(ncol):
+        mc, mb, sl, bs, jb, dl = ref2[c]
+        assert g_sla[c, 0] == sl[0] == h_sla[c, 0] + 1
+        assert g_sla[c, 8] == pytest.approx(sl[8], rel=1e-14) and sl[8] == pytest.approx(10 * h_sla[c, 8])
+        assert np.array_equal(g_best[:, :, c], h_best2[:, :, c])      # rejected: best kept
+        werr["delta"] = max(werr["delta"], np.max(np.abs(g_del[c] - dl) / (np.abs(dl) + 1e-6 * np.abs(dl).max())))
+        for row in (0, 5, 6, 7, 10):
+            werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
+                                                      (np.abs(mc[row]) + 1e-3)))
+    report("LM step (reject) node perturbation vs oracle", werr["delta"], 1e-6)
+    report("LM step (reject) new model vs oracle (relative)", werr["model"], 2e-6)
+
+
+def test_lm_loop_lowers_chi2_like_oracle(gpu_ctx, cpu):
+    """Ten iterations (forward with response functions + LM step) of one
+    cycle on 16 pixels, device against oracle."""
+    import torch
+    P = problem(gpu_ctx, cpu, NODES_C1)
+    ncol, nf, numw = P["ncol"], P["nfreq"], P["numw"]
+    d = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+    isig_h = np.full(nf, 1.0 / NOISE ** 2)
+    sla0 = np.tile(np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]), (ncol, 1))
+    atm = d(P["guess"], torch.float32)
+    best = atm.clone()
+    obs, isig, sla = d(P["obs"]), d(isig_h), d(sla0)
+    bsyn = d(np.zeros((ncol, nf)))
+    ebest = torch.zeros((ncol, P["ifreep"], nf), dtype=torch.float32, device=atm.device)
+    g_first = None
+    for it in range(10):
+        st, eqv, status = gpu_ctx.forward_nodes_torch(atm)
+        gpu_ctx.lm_step_torch(obs, st, bsyn, sla, eqv, ebest, isig, atm, best)
+        if it == 0:
+            g_first = sla.cpu().numpy()[:, 2].copy()
+    torch.cuda.synchronize()
+    g_chi = sla.cpu().numpy()[:, 2]
+    # oracle loop
+    cur = P["guess"].copy()
+    o_best = cur.copy()
+    o_sla = sla0.copy()
+    o_bsyn = np.zeros((ncol, nf))
+    o_eb = np.zeros((ncol, P["ifreep"], nf), np.float32)
+    o_first = None
+    for it in range(10):
+        for c in range(ncol):
+            m = atmos.column_model(cur, P["z"], c)
+            mo, syn, dsyn, _ = cpu.forward1d(m)
+            cur[:, :, c] = mo[:12]
+            eq = cpu.eqv_dsyn(P["cfg"], P["coefs"], dsyn)
+            mb = atmos.column_model(o_best, P["z"], c)
+            mc, mb, sl, bs, jb, _ = cpu.lm_step(P["cfg"], P["coefs"], P["obs"][c], syn.reshape(-1), o_bsyn[c],
+                                                o_sla[c], eq, o_eb[c].T, isig_h, mo, mb)
+            cur[:, :, c] = mc[:12]
+            o_best[:, :, c] = mb[:12]
+            o_sla[c], o_bsyn[c], o_eb[c] = sl, bs, jb.T
+        if it == 0:
+            o_first = o_sla[:, 2].copy()
+    o_chi = o_sla[:, 2]
+    print("[lm] median chi2 first/last: device %.4g -> %.4g, oracle %.4g -> %.4g"
+          % (np.median(g_first), np.median(g_chi), np.median(o_first), np.median(o_chi)), flush=True)
+    report("LM first chi2 device vs oracle (relative)",
+           np.max(np.abs(g_first - o_first) / o_first), 1e-3)
+    assert np.median(g_chi) < 0.5 * np.median(g_first)
+    assert np.median(o_chi) < 0.5 * np.median(o_first)
+    report("LM loop median chi2 device vs oracle (relative)",
+           abs(np.median(g_chi) - np.median(o_chi)) / np.median(o_chi), 0.1)
