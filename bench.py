@@ -116,6 +116,8 @@ def main():
     nlaunch = (ncol + args.tile - 1) // args.tile
     ncomp = 16
     fl = flops.per_column_lambda(args.nz, 2, ncomp, npar)
+    # per launch: the tile's columns x wavelengths x flop, over the average launch time
+    launch_ms = rt_avg_ms / nlaunch
     achieved = fl * ncol * numw / (rt_avg_ms * 1e-3) / 1e12
     print("[bench] step %.1f ms; micro %.1f ms; rt %.1f ms (%d launches); %.0f kflop/column-lambda"
           % (ms_step, np.mean(micro_ms), rt_avg_ms, nlaunch, fl / 1e3), file=sys.stderr, flush=True)
@@ -127,9 +129,12 @@ def main():
                                "d/d(T,|B|,incl,azim,v_los) per height" % (args.nx, args.ny, args.nz, numw),
                    "columns": ncol, "nz": args.nz, "wavelengths": numw, "parameters": npar,
                    "kernel_ms": {"micro": float(np.mean(micro_ms)), "rt": rt_avg_ms}},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "rt_kernel<spherical>", "flop_per_column_lambda": fl},
+        # compute-bound: FP32 arithmetic on the vector ALUs (no matrix work in
+        # this kernel); priced against MI355X's 157.3 TFLOP/s FP32 peak
+        "roofline": {"bound": "mfma", "pipe": "fp32 valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "kernel": "rt_kernel<1>", "launch_ms": launch_ms, "launches_per_step": nlaunch,
+                     "flop_per_column_lambda": fl},
     }
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)

@@ -605,6 +605,7 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 }  // namespace fz
 
 #include "fz_lm.hpp"
+#include "fz_invert.hpp"
 
 // ==================================================================== C ABI
 using namespace fz;
@@ -1231,6 +1232,170 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
   a.delta = d_delta;
   hipLaunchKernelGGL(lm_kernel, dim3(ncol), dim3(LM_THREADS), c->lm_lds, (hipStream_t)stream, a);
   HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+// GET_NJEVALS (get_dmodel.f90:375-407) with TOFFSET = 1 (pre_post_duties.f90:645)
+static int njevals(int apar, int curic, int maxiter, int cycpow, int nz) {
+  const int toffset = 1;
+  if (apar == 1) {
+    if (curic == 0 || curic == maxiter) return nz;
+    return std::min(nz, (int)std::pow((double)cycpow, (double)(curic + toffset - 1)));
+  }
+  if (curic == 0) return nz;
+  if (curic == 1) return 1;
+  if (curic == maxiter) return nz;
+  return std::min(nz, (int)std::pow((double)cycpow, (double)(curic - 1)));
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
+                    float* d_stokes, double* d_chi2, int* iterations, void* stream) {
+  if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
+  if (ctl->atmpar[7]) return set_err(c, FZ_ECONFIG, "P0 inversion is not supported");
+  const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  // MAXITER (pre_post_duties.f90:635-646) from the parameters' maximum nodes
+  int nfreev = 0, nfreep = 0, infreepmax = 0;
+  for (int p = 0; p < 8; ++p) {
+    if (!ctl->atmpar[p]) continue;
+    const int vmax = ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+    ++nfreev;
+    nfreep += vmax;
+    infreepmax = std::max(infreepmax, vmax);
+  }
+  if (nfreev == 0) return set_err(c, FZ_ECONFIG, "no parameter inverted");
+  const int cycpow = ctl->cycpow > 0 ? ctl->cycpow : 2;
+  const int maxiter = (nfreep == nfreev) ? 1 : (int)std::floor(std::log((double)infreepmax) / std::log((double)cycpow));
+  if (maxiter < 1) return set_err(c, FZ_ECONFIG, "no inversion cycle");
+  const int maxsteps = ctl->maxsteps > 0 ? ctl->maxsteps : 5;
+  const int itcap = ctl->max_iterations > 0 ? ctl->max_iterations : 1000;
+  // Stokes weights WSTK / MINW in REAL(4) (input_file.f90:1298-1303; weights of
+  // Stokes parameters not inverted are zero) and ISIGMAP = WSTK^2 / ISIGMA^2
+  // with ISIGMA a REAL(8) taken from a REAL(4) value (input_file.f90:181,433)
+  int nstk = 0;
+  float wsum = 0.f, wst[4];
+  for (int s = 0; s < 4; ++s) {
+    wst[s] = ctl->invstk[s] ? ctl->wstk[s] : 0.f;
+    if (ctl->invstk[s]) ++nstk;
+    wsum += wst[s] * wst[s];
+  }
+  if (nstk == 0) return set_err(c, FZ_ECONFIG, "no Stokes parameter inverted");
+  const float minw = sqrtf(wsum) / sqrtf((float)nstk);
+  const double isigma = (double)(float)(ctl->noise > 0 ? ctl->noise : 4.0e-3);
+  const int nfreq = nstk * numw;
+  std::vector<double> isig(nfreq);
+  {
+    int b = 0;
+    for (int s = 0; s < 4; ++s) {
+      if (!ctl->invstk[s]) continue;
+      const float w = wst[s] / minw;
+      for (int l = 0; l < numw; ++l) isig[b * numw + l] = (double)(w * w) / (isigma * isigma);
+      ++b;
+    }
+  }
+  int npmax = 0;
+  for (int p = 0; p < 8; ++p)
+    if (ctl->atmpar[p]) npmax += ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+  if (npmax > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters in the last cycle");
+  DevBuf b_isig, b_best, b_eqv, b_eqvb, b_bsyn, b_sla, b_done, b_stat, b_t0, b_t1;
+  const size_t natm = (size_t)12 * nz * ncol;
+  HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));
+  HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * natm));
+  HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * (size_t)ncol * npmax * nfreq));
+  HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * (size_t)ncol * npmax * nfreq));
+  HIPCHK(c, hipMalloc(&b_bsyn.p, sizeof(double) * (size_t)ncol * nfreq));
+  HIPCHK(c, hipMalloc(&b_sla.p, sizeof(double) * (size_t)ncol * 9));
+  HIPCHK(c, hipMalloc(&b_done.p, sizeof(int32_t) * ncol));
+  HIPCHK(c, hipMalloc(&b_stat.p, sizeof(double) * 2));
+  HIPCHK(c, hipMalloc(&b_t0.p, sizeof(double) * (size_t)nz * ncol));
+  HIPCHK(c, hipMalloc(&b_t1.p, sizeof(double) * (size_t)nz * ncol));
+  HIPCHK(c, hipMemcpyAsync(b_isig.p, isig.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, st));
+  float* best = (float*)b_best.p;
+  double* sla = (double*)b_sla.p;
+  int32_t* done = (int32_t*)b_done.p;
+  const unsigned gatm = (unsigned)((natm + 255) / 256), grow = (unsigned)(((size_t)nz * ncol + 255) / 256);
+  static const int rowof[8] = {A_T, A_PG, A_RHO, A_BX, A_BY, A_BZ, A_VZ, A_PG};
+  int total_it = 0;
+  for (int cyc = 1; cyc <= maxiter; ++cyc) {
+    // INITIALIZE_CYCLE (invert3d.f90:228-325)
+    fz_inversion inv;
+    std::memset(&inv, 0, sizeof(inv));
+    for (int p = 0; p < 8; ++p) {
+      if (!ctl->atmpar[p]) continue;
+      const int vmax = ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+      inv.atmpar[p] = 1;
+      inv.nodes[p] = std::min(njevals(p + 1, cyc, maxiter, cycpow, nz), vmax);
+      inv.pen_typ[p] = ctl->pen_typ[p];
+      inv.pen_alp[p] = ctl->pen_alp[p];
+    }
+    for (int s = 0; s < 4; ++s) inv.invstk[s] = ctl->invstk[s];
+    inv.regul = ctl->regul;
+    inv.solver = ctl->solver > 0 ? ctl->solver : 1;
+    inv.node_expansion = ctl->node_expansion > 0 ? ctl->node_expansion : 1;
+    inv.maxsteps = maxsteps;
+    inv.svdtol = ctl->svdtol;
+    int r = fz_set_inversion(c, &inv, nullptr, nullptr);
+    if (r) return r;
+    hipLaunchKernelGGL(inv_init_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, sla, done);
+    // SET_TAU (:492-727): a synthesis updates log tau5 when Pg and P0 are not inverted
+    if (!ctl->atmpar[1] && !ctl->atmpar[7]) {
+      r = fz_forward_dev(c, ncol, d_atm, FZ_RF_NONE, d_stokes, nullptr, nullptr, stream);
+      if (r) return r;
+    }
+    // UPDATE_CURRENT_TO_BEST, then SMOOTHING_B on the best model into both
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, d_atm, best, (long)natm);
+    if (ctl->smoothing) {
+      for (int p = 0; p < 8; ++p) {
+        if (!ctl->atmpar[p]) continue;
+        float* rb = best + (size_t)rowof[p] * nz * ncol;
+        float* rc = d_atm + (size_t)rowof[p] * nz * ncol;
+        double *t0 = (double*)b_t0.p, *t1 = (double*)b_t1.p;
+        hipLaunchKernelGGL(to_double_kernel, dim3(grow), dim3(256), 0, st, rb, t0, (long)nz * ncol);
+        const int ext[3] = {nx, ny, nz};
+        for (int ax = 0; ax < 3; ++ax) {
+          if (ext[ax] <= 4) continue;
+          hipLaunchKernelGGL(smooth_axis_kernel, dim3(grow), dim3(256), 0, st, t0, t1, nz, ny, nx, ax);
+          std::swap(t0, t1);
+        }
+        hipLaunchKernelGGL(from_double_kernel, dim3(grow), dim3(256), 0, st, t0, rc, rb, (long)nz * ncol);
+      }
+    }
+    HIPCHK(c, hipGetLastError());
+    const double need = (double)std::max(1, maxsteps - (maxiter - cyc));
+    for (int it = 0; it < itcap; ++it) {
+      // MODELLING -> SOLVE_DMODEL -> MAKE_DECISION
+      r = fz_forward_nodes_dev(c, ncol, d_atm, d_stokes, (float*)b_eqv.p, nullptr, stream);
+      if (r) return r;
+      r = fz_lm_step_dev(c, ncol, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p, (float*)b_eqvb.p,
+                         (const double*)b_isig.p, d_atm, best, done, nullptr, stream);
+      if (r) return r;
+      hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, st, ncol, sla, done, (double*)b_stat.p);
+      double hs[2];
+      HIPCHK(c, hipMemcpyAsync(hs, b_stat.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      ++total_it;
+      if (ncol - (int)hs[1] <= 0) break;
+      if (hs[0] / (double)ncol >= need) break;
+    }
+    // END_CYCLE (:328-403): current := best, synthesis without response functions
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, best, d_atm, (long)natm);
+    r = fz_forward_dev(c, ncol, d_atm, FZ_RF_NONE, d_stokes, nullptr, nullptr, stream);
+    if (r) return r;
+  }
+  if (d_chi2) {
+    HIPCHK(c, hipMemcpy2DAsync(d_chi2, sizeof(double), sla + 2, sizeof(double) * 9, sizeof(double), ncol,
+                               hipMemcpyDeviceToDevice, st));
+  }
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (iterations) *iterations = total_it;
   return FZ_OK;
 }
 

@@ -45,6 +45,15 @@ class CInversion(ctypes.Structure):
                 ("mpert", ctypes.c_double)]
 
 
+class CInvertControl(ctypes.Structure):
+    """fz_invert_control (include/firtez_gpu.h)."""
+    _fields_ = [("atmpar", ctypes.c_int32 * 8), ("nslb_max", ctypes.c_int32 * 8), ("invstk", ctypes.c_int32 * 4),
+                ("wstk", ctypes.c_float * 4), ("noise", ctypes.c_double), ("cycpow", ctypes.c_int32),
+                ("maxsteps", ctypes.c_int32), ("solver", ctypes.c_int32), ("node_expansion", ctypes.c_int32),
+                ("regul", ctypes.c_int32), ("pen_typ", ctypes.c_int32 * 8), ("pen_alp", ctypes.c_double * 8),
+                ("svdtol", ctypes.c_double), ("max_iterations", ctypes.c_int32), ("smoothing", ctypes.c_int32)]
+
+
 # inverted physical parameters, in the reference's INV_ATMPAR order
 INV_PARAMS = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "P0")
 
@@ -86,6 +95,8 @@ def library():
                                    ctypes.POINTER(ctypes.c_int)]
     L.fz_forward_nodes_dev.argtypes = [vp, ip, vp, vp, vp, vp, vp]
     L.fz_lm_step_dev.argtypes = [vp, ip] + [vp] * 12
+    L.fz_invert3d_dev.argtypes = [vp, ip, ip, vp, vp, ctypes.POINTER(CInvertControl), vp, vp,
+                                  ctypes.POINTER(ctypes.c_int), vp]
     _lib = L
     return L
 
@@ -95,7 +106,27 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
-            "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev"]
+            "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev"]
+
+
+def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
+                   maxsteps=5, solver=1, node_expansion=1, svdtol=0.0, max_iterations=0, smoothing=True,
+                   regul=False, pen_typ: Optional[dict] = None, pen_alp: Optional[dict] = None) -> CInvertControl:
+    """fz_invert_control from {parameter name: maximum nodes}, e.g. {"T": 4, "vz": 4}."""
+    c = CInvertControl()
+    for name, n in max_nodes.items():
+        i = INV_PARAMS.index(name)
+        c.atmpar[i] = 1
+        c.nslb_max[i] = int(n)
+    c.invstk[:] = [int(x) for x in invstk]
+    c.wstk[:] = [float(x) for x in wstk]
+    c.noise, c.cycpow, c.maxsteps, c.solver, c.node_expansion = noise, cycpow, maxsteps, solver, node_expansion
+    c.regul, c.svdtol, c.max_iterations, c.smoothing = int(regul), svdtol, max_iterations, int(smoothing)
+    for name, v in (pen_typ or {}).items():
+        c.pen_typ[INV_PARAMS.index(name)] = int(v)
+    for name, v in (pen_alp or {}).items():
+        c.pen_alp[INV_PARAMS.index(name)] = float(v)
+    return c
 
 
 def int_coefs(nz: int, nj: int) -> np.ndarray:
@@ -282,6 +313,25 @@ class Context:
                                             done.data_ptr() if done is not None else None,
                                             delta.data_ptr() if delta is not None else None, stream),
                     "fz_lm_step_dev")
+
+    def invert3d_torch(self, atm, obs, control: CInvertControl, nx: int, ny: int, stokes=None, chi2=None):
+        """INVERT3D on device tensors: atm [12, nz, nx*ny] float32 (initial
+        model in, inverted model out), obs [nx*ny, NFREQ] float64.  Returns
+        (stokes [ncol,4,numw], chi2 [ncol], LM iterations)."""
+        import torch
+        ncol = nx * ny
+        assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous() and atm.shape[2] == ncol
+        assert obs.is_cuda and obs.dtype == torch.float64 and obs.is_contiguous() and obs.shape[0] == ncol
+        if stokes is None:
+            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=atm.device)
+        if chi2 is None:
+            chi2 = torch.empty(ncol, dtype=torch.float64, device=atm.device)
+        its = ctypes.c_int()
+        stream = torch.cuda.current_stream(atm.device).cuda_stream
+        self._check(self.lib.fz_invert3d_dev(self.h, nx, ny, atm.data_ptr(), obs.data_ptr(), ctypes.byref(control),
+                                             stokes.data_ptr(), chi2.data_ptr(), ctypes.byref(its), stream),
+                    "fz_invert3d_dev")
+        return stokes, chi2, its.value
 
     # ---------------------------------------------------------------- FORWARD3D mirror
     def forward3d(self, cube: dict, am_i_done=None, rf_mode=RF_NONE):

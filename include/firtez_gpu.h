@@ -174,6 +174,42 @@ int fz_lm_step_dev(fz_ctx* ctx, int ncol, const double* d_obs, const float* d_st
                    double* d_sla, const float* d_eqv, float* d_eqv_best, const double* d_isigmap,
                    float* d_atm, float* d_atm_best, int32_t* d_done, double* d_delta, void* stream);
 
+/* Control of a whole inversion (the INVERSION SETUP, STOKES SETUP and MISC
+ * SETUP sections of the control file, input_file.f90:1064-1135, 1280-1303,
+ * 170-190). */
+typedef struct {
+  int32_t atmpar[8];     /* inverted parameters (T, Pg, rho, Bx, By, Bz, v_z; P0 unsupported) */
+  int32_t nslb_max[8];   /* maximum nodes of each parameter (NSLB_MAX, "TEM 4") */
+  int32_t invstk[4];     /* inverted Stokes parameters */
+  float wstk[4];         /* Stokes weights before normalisation (STKI 1 ...) */
+  double noise;          /* ISIGMA (NOISE), 0 -> 4e-3 */
+  int32_t cycpow;        /* POW, 0 -> 2 */
+  int32_t maxsteps;      /* MAXITER keyword (MAXSTEPS), 0 -> 5 */
+  int32_t solver;        /* LINEAR_SOLVER, 0 -> 1 */
+  int32_t node_expansion;/* PERTURBATION_TYPE, 0 -> 1 */
+  int32_t regul;
+  int32_t pen_typ[8];
+  double pen_alp[8];
+  double svdtol;         /* 0 -> 1e-4 */
+  int32_t max_iterations;/* cap on LM iterations per cycle, 0 -> 1000 */
+  int32_t smoothing;     /* SMOOTHING_B at every cycle start (the reference always applies it) */
+} fz_invert_control;
+
+/* INVERT3D (invert3d.f90:57-178) on device data: the cycle schedule (nodes
+ * per cycle from GET_NJEVALS, get_dmodel.f90:375-407), SET_TAU, the 1-2-1
+ * smoothing of the inverted parameters, LM iterations (forward with
+ * node-contracted response functions + fz_lm_step_dev) until the mean step
+ * count reaches max(1, MAXSTEPS - (MAXITER - cycle)) or every pixel is done,
+ * and END_CYCLE's best -> current + forward synthesis.
+ *   d_atm    [12][nz][nx*ny] in: initial model, out: inverted model
+ *   d_obs    [nx*ny][NFREQ] double observed profiles (inverted Stokes blocks)
+ *   d_stokes [nx*ny][4][NUMW] float out: synthesis of the inverted model
+ *   d_chi2   [nx*ny] double out (or NULL): best chi^2 of the last cycle
+ *   iterations (or NULL): LM iterations done, summed over cycles
+ * Synchronises the stream once per iteration (the stop test). */
+int fz_invert3d_dev(fz_ctx* ctx, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
+                    float* d_stokes, double* d_chi2, int* iterations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

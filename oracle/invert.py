@@ -1,0 +1,151 @@
+"""Restatement of the reference's inversion cycle driver (INVERT3D,
+invert3d.f90:57-403) over the oracle's per-column forward and LM step.
+
+TEST INFRASTRUCTURE (the oracle): only tests/ use it, as the checker.  The
+arithmetic lives in fzcpu.c / fzlm.c; this module keeps the schedule:
+  * MAXITER from the parameters' maximum nodes (pre_post_duties.f90:635-646),
+    nodes per cycle min(GET_NJEVALS, NSLB_MAX) (get_dmodel.f90:375-452);
+  * INITIALIZE_CYCLE (invert3d.f90:228-325): INV_MAS reset, SET_TAU
+    synthesis, best := current, SMOOTHING_B (model_spatial_extension.f90:
+    351-491: 1-2-1 filter along x, y, z of every inverted parameter, FP64);
+  * iterations: forward with response functions, node contraction, LM step
+    per pixel, MAKE_DECISION (invert3d.f90:820-859);
+  * END_CYCLE (:328-403): current := best and a synthesis.
+Columns are numbered col = y + ny * x; atm is [12][nz][ncol] float32.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import lm_cfg
+
+ROWOF = (0, 1, 2, 5, 6, 7, 10, 1)
+
+
+def njevals(apar, curic, maxiter, cycpow, nz, toffset=1):
+    if apar == 1:
+        if curic == 0 or curic == maxiter:
+            return nz
+        return min(nz, int(cycpow ** (curic + toffset - 1)))
+    if curic in (0,):
+        return nz
+    if curic == 1:
+        return 1
+    if curic == maxiter:
+        return nz
+    return min(nz, int(cycpow ** (curic - 1)))
+
+
+def weights(invstk, wstk, noise, numw):
+    """ISIGMAP: REAL(4) normalised weights over REAL(8) ISIGMA**2."""
+    w = np.array([wstk[s] if invstk[s] else 0.0 for s in range(4)], np.float32)
+    nstk = int(sum(1 for s in range(4) if invstk[s]))
+    minw = np.sqrt(np.float32(np.sum(w * w, dtype=np.float32))) / np.sqrt(np.float32(nstk))
+    isigma = float(np.float32(noise))
+    out = []
+    for s in range(4):
+        if invstk[s]:
+            ws = np.float32(w[s] / minw)
+            out.append(np.full(numw, float(np.float32(ws * ws)) / (isigma * isigma)))
+    return np.concatenate(out)
+
+
+def smooth(a, nx, ny):
+    """SMOOTHING_B on one parameter: a [nz][ncol] float32 -> float32."""
+    nz = a.shape[0]
+    v = a.astype(np.float64).reshape(nz, nx, ny)       # col = y + ny*x
+    if nx > 4:
+        w = v.copy()
+        w[:, 1:-1, :] = (v[:, 2:, :] + v[:, :-2, :] + v[:, 1:-1, :] * 2.0) / 4.0
+        v = w
+    if ny > 4:
+        w = v.copy()
+        w[:, :, 1:-1] = (v[:, :, 2:] + v[:, :, :-2] + v[:, :, 1:-1] * 2.0) / 4.0
+        v = w
+    if nz > 4:
+        w = v.copy()
+        w[1:-1] = (v[2:] + v[:-2] + v[1:-1] * 2.0) / 4.0
+        v = w
+    return v.reshape(nz, nx * ny).astype(np.float32)
+
+
+def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 1, 1, 1), noise=4e-3, cycpow=2,
+             maxsteps=5, solver=1, node_expansion=1, svdtol=1e-4, smoothing=True, max_iterations=1000):
+    """cpu: oracle.Cpu set up with resp=True and all Cartesian slots.
+    max_nodes: {physical index 0..6: NSLB_MAX}.  Returns (model, stokes
+    [ncol,4,numw], chi2 [ncol], iterations)."""
+    cur = np.array(atm, np.float32, copy=True)
+    nz, ncol = cur.shape[1], cur.shape[2]
+    numw = cpu.numw
+    atmpar = [1 if p in max_nodes else 0 for p in range(8)]
+    vmax = {p: min(max_nodes[p], nz) for p in max_nodes}
+    nfreev, nfreep = len(vmax), sum(vmax.values())
+    maxiter = 1 if nfreep == nfreev else int(math.floor(math.log(max(vmax.values())) / math.log(cycpow)))
+    isig = weights(invstk, wstk, noise, numw)
+    nfreq = len(isig)
+    rows = [s for s in range(4) if invstk[s]]
+    total = 0
+
+    def forward_all(model, resp):
+        syn_all = np.zeros((ncol, 4, numw))
+        dsyn_all = []
+        for c in range(ncol):
+            m = np.zeros((13, nz), np.float32)
+            m[:12] = model[:, :, c]
+            m[12] = np.asarray(z, np.float32)
+            mo, syn, dsyn, _ = cpu.forward1d(m)
+            model[:, :, c] = mo[:12]
+            syn_all[c] = syn
+            dsyn_all.append(dsyn if resp else None)
+        return syn_all, dsyn_all
+
+    sla = None
+    syn = None
+    for cyc in range(1, maxiter + 1):
+        nodes = [0] * 8
+        for p in vmax:
+            nodes[p] = min(njevals(p + 1, cyc, maxiter, cycpow, nz), vmax[p])
+        cfg = lm_cfg(nz, numw, atmpar, nodes, invstk=invstk, solver=solver, node_expansion=node_expansion,
+                     svdtol=svdtol, inu=nfreq)
+        coefs = cpu.coefs_block(cfg, {p: cpu.int_coefs(nz, nodes[p]) for p in vmax})
+        np_ = coefs.shape[0]
+        sla = np.tile(np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]), (ncol, 1))
+        done = np.zeros(ncol, bool)
+        if not atmpar[1] and not atmpar[7]:
+            forward_all(cur, False)                       # SET_TAU
+        best = cur.copy()
+        if smoothing:
+            for p in vmax:
+                r = ROWOF[p]
+                best[r] = smooth(best[r], nx, ny)
+                cur[r] = best[r]
+        bsyn = np.zeros((ncol, nfreq))
+        ebest = np.zeros((ncol, nfreq, np_), np.float32)
+        need = max(1, maxsteps - (maxiter - cyc))
+        for it in range(max_iterations):
+            syn, dsyn = forward_all(cur, True)
+            for c in range(ncol):
+                if done[c]:
+                    continue
+                eq = cpu.eqv_dsyn(cfg, coefs, dsyn[c])
+                mc = np.zeros((13, nz), np.float32)
+                mc[:12] = cur[:, :, c]
+                mc[12] = z
+                mb = mc.copy()
+                mb[:12] = best[:, :, c]
+                s_rows = syn[c][rows].reshape(-1)
+                mc2, mb2, sl, bs, jb, _ = cpu.lm_step(cfg, coefs, obs[c], s_rows, bsyn[c], sla[c], eq, ebest[c],
+                                                      isig, mc, mb)
+                cur[:, :, c] = mc2[:12]
+                best[:, :, c] = mb2[:12]
+                sla[c], bsyn[c], ebest[c] = sl, bs, jb
+                if sl[0] > maxsteps:
+                    done[c] = True
+            total += 1
+            if ncol - done.sum() <= 0 or sla[:, 0].sum() / ncol >= need:
+                break
+        cur = best.copy()
+        syn, _ = forward_all(cur, False)
+    return cur, syn, sla[:, 2].copy(), total

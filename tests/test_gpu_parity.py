@@ -3,7 +3,12 @@ outputs and the FP64 oracle on the same seeded inputs.
 
 Tolerances (FP32 bulk arithmetic on the device, FP64 in the oracle):
   Stokes          max |dS| <= 1e-4 I_c   (I_c: HSRA continuum = 1 after GAIN1D)
-  Jacobians       max |dJ| <= RF_TOL * max|J| per (parameter, Stokes)
+  Jacobians       max |dJ| <= RF_TOL * max|J| per parameter: the error of
+                  d(I,Q,U,V)/dx over all heights and wavelengths, relative to
+                  the largest response of the continuum-normalised spectrum
+                  to that parameter (so a Stokes parameter with almost no
+                  signal, e.g. Q for a 3 G transverse field, is judged on the
+                  scale of the parameter's effect, not on its own tiny scale)
   EOS / HE (FP64) relative 1e-9
 """
 import numpy as np
@@ -92,7 +97,7 @@ def test_rf_cartesian_vs_reference(gpu_ctx, golden):
         ours = jac[0, p][g["rf_heights"]]                  # (nzs, 4, numw)
         ref = np.transpose(g["rf"][:, :, p, :], (0, 2, 1))  # (nzs, 4, numw)
         for s in range(4):
-            sc = np.abs(ref[:, s]).max()
+            sc = np.abs(ref).max()
             if sc == 0:
                 continue
             report("test_rf RF d/d%s Stokes %d" % (names[p], s), np.max(np.abs(ours[:, s] - ref[:, s])) / sc, RF_TOL)
@@ -110,7 +115,7 @@ def test_seeded_patch_vs_reference(gpu_ctx, golden):
         for p in range(7):
             ref = np.transpose(g["rf"][i][:, :, p, :], (0, 2, 1))
             for s in range(4):
-                sc = np.abs(ref[:, s]).max()
+                sc = np.abs(ref).max()
                 if sc > 0:
                     report("seeded col %d RF p%d s%d" % (i, p, s), np.max(np.abs(jac[i, p][:, s] - ref[:, s])) / sc,
                            RF_TOL)
@@ -161,7 +166,7 @@ def test_seeded_cube_spherical_vs_oracle(gpu_ctx, cpu):
         dB, dg, df = cart_to_sph(jc, bx, by, bz)
         for p, ref in ((0, dT), (1, dB), (2, dg), (3, df), (4, dv)):
             for s in range(4):
-                sc = np.abs(ref[:, s]).max()
+                sc = np.abs(ref).max()
                 if sc > 0:
                     report("cube col %d spherical p%d s%d" % (c, p, s),
                            np.max(np.abs(jac[c, p][:, s] - ref[:, s])) / sc, RF_TOL)
