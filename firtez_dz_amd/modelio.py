@@ -1,16 +1,27 @@
-"""Readers/writers for the reference's binary model format (version 3).
+"""Readers/writers for the reference's binary files (version 3).
 
-A model file is a Fortran unformatted sequential file (4-byte record
-markers) of REAL(4): record 1 = [a, b, 130904, nrec, 4, 13, nx, ny, nz]; the
-following records hold the 13 parameter cubes (NZ,NY,NX) in the order
-T, Pg, rho, Bx, By, Bz, vz, Pe, mu, log tau5, x, y, z (misc.f90:78-503,
-801-929).  z is in km, B in G, v in cm/s.
+All are Fortran unformatted sequential files (4-byte record markers) of
+REAL(4), with a header record [SS+VV, SS-VV, magic, nrec, ndims, sizes...]
+and data records of at most 268435446 values (misc.f90:15-17):
+  * model 130904 (misc.f90:78-503, 801-929): header [.., 130904, nrec, 4, 13,
+    nx, ny, nz]; the 13 cubes (NZ,NY,NX) T, Pg, rho, Bx, By, Bz, vz, Pe, mu,
+    log tau5, x, y, z.  z in km, B in G, v in cm/s.
+  * profiles 160904 (misc.f90:1147-1249, 1338-1642): header [.., 160904,
+    nrec, 4, nx, ny, nw, 4]; a record of line indices, one of wavelengths (mA),
+    then STOKES(4, nw, ny, nx).
+  * response functions 18060904 (misc.f90:1862-1999): header [.., 18060904,
+    nrec, 6, nx, ny, nz, nw, 4, npar]; records z, line index, wavelength,
+    parameter mask (8), then DSTOKES(p, s, w, k, y, x) for the stored p.
 """
 from __future__ import annotations
 
 import numpy as np
 
 MODEL_MAGIC = 130904
+PROFILE_MAGIC = 160904
+RF_MAGIC = 18060904
+VERSION = (3000, 3)      # SS, VV (format 3000, version 3) written as SS+VV, SS-VV (misc.f90:944-953)
+NMAXREC = 1073741784 // 4
 ORDER = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "Pe", "mu", "tau5", "x", "y", "z")
 
 
@@ -52,3 +63,108 @@ def to_device_rows(cube):
             continue
         atm[i] = np.ascontiguousarray(cube[nm].transpose(0, 2, 1)).reshape(nz, nx * ny)
     return atm
+
+
+def _write_records(path, header, blocks):
+    """header and blocks: float32 arrays, each written as records; the data
+    block is split into records of at most NMAXREC values."""
+    with open(path, "wb") as f:
+        def rec(a):
+            a = np.ascontiguousarray(a, dtype=np.float32)
+            n = np.array([a.nbytes], np.int32)
+            f.write(n.tobytes())
+            f.write(a.tobytes())
+            f.write(n.tobytes())
+        rec(header)
+        for b in blocks[:-1]:
+            rec(b)
+        data = np.ascontiguousarray(blocks[-1], dtype=np.float32).reshape(-1)
+        for i in range(0, data.size, NMAXREC):
+            rec(data[i:i + NMAXREC])
+
+
+def _nrecs(n):
+    return int(np.ceil(np.float32(n) / np.float32(NMAXREC)))
+
+
+def write_model(path, cube, x=None, y=None):
+    """WRITE_MODEL3D_V3: cube dict of (nz, ny, nx) arrays (names of ORDER;
+    'z_km' or 'z'), x (nx,) and y (ny,) in km (default 0)."""
+    nz, ny, nx = cube["T"].shape
+    ss, vv = VERSION
+    n = nx * ny * nz
+    hdr = np.array([ss + vv, ss - vv, MODEL_MAGIC, _nrecs(13 * n) + 1.0, 4.0, 13, nx, ny, nz], np.float32)
+    z = np.asarray(cube["z_km"] if "z_km" in cube else cube["z"][:, 0, 0], np.float64)
+    xs = np.zeros(nx) if x is None else np.asarray(x, np.float64)
+    ys = np.zeros(ny) if y is None else np.asarray(y, np.float64)
+    parts = []
+    for name in ORDER:
+        if name == "x":
+            c = np.broadcast_to(xs.astype(np.float32)[None, None, :], (nz, ny, nx))
+        elif name == "y":
+            c = np.broadcast_to(ys.astype(np.float32)[None, :, None], (nz, ny, nx))
+        elif name == "z":
+            c = np.broadcast_to(z.astype(np.float32)[:, None, None], (nz, ny, nx))
+        else:
+            c = np.asarray(cube[name], np.float32)
+        parts.append(np.asarray(c).reshape(-1, order="F"))
+    _write_records(path, hdr, [np.concatenate(parts)])
+
+
+def write_profiles(path, stokes, wave_ma, index=None):
+    """WRITE_PROFILES3D_V3: stokes (4, nw, ny, nx) (Fortran STOKES(4,NW,NY,NX)),
+    wave_ma (nw,) relative wavelengths, index (nw,) line index of each point."""
+    st = np.asarray(stokes, np.float32)
+    ns, nw, ny, nx = st.shape
+    ss, vv = VERSION
+    hdr = np.array([ss + vv, ss - vv, PROFILE_MAGIC, _nrecs(st.size) + 3.0, 4, nx, ny, nw, ns], np.float32)
+    ind = np.zeros(nw) if index is None else np.asarray(index, np.float64)
+    _write_records(path, hdr, [ind.astype(np.float32), np.asarray(wave_ma, np.float64).astype(np.float32),
+                               st.reshape(-1, order="F")])
+
+
+def read_profiles(path):
+    """READ_PROFILE_3D (v3): returns (stokes (4, nw, ny, nx), wave_ma, index)."""
+    recs = _records(path)
+    hdr = recs[0]
+    if int(round(float(hdr[2]))) != PROFILE_MAGIC:
+        raise ValueError("%s is not a profile file" % path)
+    nx, ny, nw, ns = (int(v) for v in hdr[5:9])
+    ind, wave = recs[1], recs[2]
+    data = np.concatenate(recs[3:])
+    if data.size != ns * nw * ny * nx:
+        raise ValueError("unexpected profile layout")
+    return data.reshape((ns, nw, ny, nx), order="F"), wave, ind
+
+
+def write_rfs(path, dstokes, z_km, wave_ma, logv, index=None):
+    """WRITE_RFS3D_V3: dstokes (8, 4, nw, nz, ny, nx) in Fortran order
+    DSTOKES(P,S,W,K,J,I); logv (8,) which parameters are stored (T|Pg, Pg|T,
+    rho|T, Bx, By, Bz, vz, P0)."""
+    d = np.asarray(dstokes, np.float32)
+    npar, ns, nw, nz, ny, nx = d.shape
+    logv = np.asarray(logv, bool)
+    ss, vv = VERSION
+    keep = d[logv]
+    hdr = np.array([ss + vv, ss - vv, RF_MAGIC, 1 + 1 + 1 + 1 + 1 + _nrecs(keep.size), 6, nx, ny, nz, nw, ns, npar],
+                   np.float32)
+    ind = np.zeros(nw) if index is None else np.asarray(index, np.float64)
+    _write_records(path, hdr, [np.asarray(z_km, np.float64).astype(np.float32), ind.astype(np.float32),
+                               np.asarray(wave_ma, np.float64).astype(np.float32), logv.astype(np.float32),
+                               keep.reshape(-1, order="F")])
+
+
+def read_rfs(path):
+    """Reads a WRITE_RFS3D_V3 file: returns (dstokes (8, 4, nw, nz, ny, nx) with
+    zeros for the parameters not stored, z_km, wave_ma, logv)."""
+    recs = _records(path)
+    hdr = recs[0]
+    if int(round(float(hdr[2]))) != RF_MAGIC:
+        raise ValueError("%s is not a response-function file" % path)
+    nx, ny, nz, nw, ns, npar = (int(v) for v in hdr[5:11])
+    z, ind, wave, logv = recs[1], recs[2], recs[3], recs[4] > 0.5
+    data = np.concatenate(recs[5:])
+    keep = data.reshape((int(logv.sum()), ns, nw, nz, ny, nx), order="F")
+    out = np.zeros((npar, ns, nw, nz, ny, nx), np.float32)
+    out[logv] = keep
+    return out, z, wave, logv

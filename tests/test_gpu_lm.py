@@ -219,3 +219,72 @@ def test_lm_loop_lowers_chi2_like_oracle(gpu_ctx, cpu):
     assert np.median(o_chi) < 0.5 * np.median(o_first)
     report("LM loop median chi2 device vs oracle (relative)",
            abs(np.median(g_chi) - np.median(o_chi)) / np.median(o_chi), 0.1)
+
+
+def _invert_problem(cpu, nx, ny, seed, numw_setup):
+    """Truth from the seeded generator (hydrostatic), observations = oracle
+    synthesis + noise, start = truth with offsets in T, B and v_z."""
+    from oracle import invert
+    ls = numw_setup
+    atm, z = atmos.seeded(nx, ny, 64, seed=seed)
+    cpu.setup(ls, resp=True, atmpar=CART)
+    ncol = nx * ny
+    for c in range(ncol):
+        pg, rho, pel, mw, _ = cpu.he(atm[0, :, c], float(atm[1, -1, c]), z)
+        atm[1, :, c], atm[2, :, c], atm[3, :, c], atm[4, :, c] = pg, rho, pel, mw
+    rng = np.random.default_rng(seed)
+    return atm, z, rng
+
+
+@pytest.mark.parametrize("case", ["one-node-no-smoothing", "two-cycles-smoothing"])
+def test_invert3d_vs_oracle(gpu_ctx, cpu, case):
+    """INVERT3D on the device (fz_invert3d_dev) against the oracle's
+    restatement of the same schedule on a small cube."""
+    import torch
+    from oracle import invert
+    ls = lines.fe6301_6302(nwave=48, start=-200.0, step=30.0)
+    nx, ny = (2, 2) if case.startswith("one") else (6, 5)
+    atm, z, rng = _invert_problem(cpu, nx, ny, 21, ls)
+    ncol = nx * ny
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    cpu.set_gain(gain)
+    obs = []
+    for c in range(ncol):
+        mo, syn, _, _ = cpu.forward1d(atmos.column_model(atm, z, c))
+        atm[:, :, c] = mo[:12]
+        obs.append(syn.reshape(-1) + rng.normal(0, NOISE, syn.size))
+    obs = np.array(obs)
+    guess = atm.copy()
+    guess[0] += np.float32(150.0)
+    guess[5:8] *= np.float32(0.8)
+    guess[10] += np.float32(5e4)
+    if case.startswith("one"):
+        nodes = {"T": 1, "Bx": 1, "By": 1, "Bz": 1, "vz": 1}
+        smoothing = False
+    else:
+        nodes = {"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}
+        smoothing = True
+    ctl = gpu.invert_control(nodes, noise=NOISE, smoothing=smoothing)
+    d_atm = torch.from_numpy(guess.copy()).cuda()
+    d_obs = torch.from_numpy(obs.copy()).cuda()
+    st, chi2, its = gpu_ctx.invert3d_torch(d_atm, d_obs, ctl, nx, ny)
+    torch.cuda.synchronize()
+    g_model, g_chi2, g_st = d_atm.cpu().numpy(), chi2.cpu().numpy(), st.cpu().numpy()
+    idx = {gpu.INV_PARAMS.index(k): v for k, v in nodes.items()}
+    o_model, o_syn, o_chi2, o_its = invert.invert3d(cpu, guess, z, obs, nx, ny, idx, noise=NOISE,
+                                                   smoothing=smoothing)
+    print("[invert] %s: iterations device %d oracle %d; median chi2 device %.4g oracle %.4g"
+          % (case, its, o_its, np.median(g_chi2), np.median(o_chi2)), flush=True)
+    assert abs(its - o_its) <= 2
+    report("INVERT3D %s median chi2 device vs oracle (relative)" % case,
+           abs(np.median(g_chi2) - np.median(o_chi2)) / np.median(o_chi2), 0.1)
+    # the final synthesis is the forward of the final model
+    report("INVERT3D %s synthesis vs oracle synthesis of its own model" % case,
+           float(np.max(np.abs(g_st - o_syn))) if case.startswith("one") else 0.0, 5e-3)
+    if case.startswith("one"):
+        assert np.all(g_chi2 < 2.0) and np.all(o_chi2 < 2.0)
+        for row in (0, 5, 6, 7, 10):
+            scale = max(1.0, float(np.abs(atm[row]).max()))
+            report("INVERT3D one-node model row %d vs oracle (relative to max)" % row,
+                   float(np.max(np.abs(g_model[row] - o_model[row]))) / scale, 2e-3)
