@@ -20,6 +20,9 @@ FLAGS = [
     "-mcode-object-version=5",
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division and sqrt through the hardware reciprocal / sqrt (a few ulp);
+    # FP64 arithmetic is unaffected
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-I", os.path.join(ROOT, "include"),
 ]
 

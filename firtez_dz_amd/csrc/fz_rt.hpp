@@ -22,6 +22,10 @@ __device__ __forceinline__ v3 cross(v3 a, v3 b) {
 // Voigt H(a,u) and Faraday-Voigt L(a,u) = Re, Im of w(u + i a) with the
 // Hui, Armstrong & Wray (1978) 7/7 rational form in z = a - i|u|; L is odd
 // in u (voigt.f:17-98).  a > 0 always here (radiative damping is positive).
+// Both polynomials have real coefficients, so they are evaluated at the
+// complex point with Knuth's two-FMA recurrence on the quadratic
+// z^2 - 2 Re(z) z + |z|^2 (TAOCP vol. 2, 4.6.4): P(z) = p1 z + p0 after
+// p1 <- p0 + r p1, p0 <- c_k - s p1 with r = 2 Re z, s = |z|^2.
 __device__ __forceinline__ void voigt(float a, float u, float& H, float& F) {
   const float A0 = 122.607931777104326f, A1 = 214.382388694706425f, A2 = 181.928533092181549f,
               A3 = 93.155580458138441f, A4 = 30.180142196210589f, A5 = 5.912626209773153f,
@@ -30,26 +34,30 @@ __device__ __forceinline__ void voigt(float a, float u, float& H, float& F) {
               B3 = 348.703917719495792f, B4 = 170.354001821091472f, B5 = 53.992906912940207f,
               B6 = 10.479857114260399f;
   const float v = fabsf(u);
-  const float zr = a, zi = -v;
-  // numerator
-  float nr = A6, ni = 0.f, t;
-  t = nr * zr - ni * zi + A5; ni = nr * zi + ni * zr; nr = t;
-  t = nr * zr - ni * zi + A4; ni = nr * zi + ni * zr; nr = t;
-  t = nr * zr - ni * zi + A3; ni = nr * zi + ni * zr; nr = t;
-  t = nr * zr - ni * zi + A2; ni = nr * zi + ni * zr; nr = t;
-  t = nr * zr - ni * zi + A1; ni = nr * zi + ni * zr; nr = t;
-  t = nr * zr - ni * zi + A0; ni = nr * zi + ni * zr; nr = t;
-  // denominator
-  float dr = zr + B6, di = zi;
-  t = dr * zr - di * zi + B5; di = dr * zi + di * zr; dr = t;
-  t = dr * zr - di * zi + B4; di = dr * zi + di * zr; dr = t;
-  t = dr * zr - di * zi + B3; di = dr * zi + di * zr; dr = t;
-  t = dr * zr - di * zi + B2; di = dr * zi + di * zr; dr = t;
-  t = dr * zr - di * zi + B1; di = dr * zi + di * zr; dr = t;
-  t = dr * zr - di * zi + B0; di = dr * zi + di * zr; dr = t;
-  const float inv = 1.0f / (dr * dr + di * di);
-  H = (nr * dr + ni * di) * inv;
-  const float im = (ni * dr - nr * di) * inv;
+  const float x = a, y = -v;
+  const float r = 2.f * x, s = x * x + y * y;
+  float p1, p0, t;
+  // numerator A6 z^6 + ... + A0
+  p1 = A6;
+  p0 = A5;
+  t = p1; p1 = fmaf(r, t, p0); p0 = fmaf(-s, t, A4);
+  t = p1; p1 = fmaf(r, t, p0); p0 = fmaf(-s, t, A3);
+  t = p1; p1 = fmaf(r, t, p0); p0 = fmaf(-s, t, A2);
+  t = p1; p1 = fmaf(r, t, p0); p0 = fmaf(-s, t, A1);
+  t = p1; p1 = fmaf(r, t, p0); p0 = fmaf(-s, t, A0);
+  const float nr = fmaf(p1, x, p0), ni = p1 * y;
+  // denominator z^7 + B6 z^6 + ... + B0
+  float q1 = 1.f, q0 = B6;
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B5);
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B4);
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B3);
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B2);
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B1);
+  t = q1; q1 = fmaf(r, t, q0); q0 = fmaf(-s, t, B0);
+  const float dr = fmaf(q1, x, q0), di = q1 * y;
+  const float inv = __builtin_amdgcn_rcpf(fmaf(dr, dr, di * di));
+  H = fmaf(nr, dr, ni * di) * inv;
+  const float im = fmaf(ni, dr, -nr * di) * inv;
   F = (u < 0.f) ? -im : im;
 }
 
