@@ -97,6 +97,7 @@ class Cpu:
         L.fzc_forward1d.argtypes = [ctypes.c_int, _fp, _dp, _dp, _dp]
         L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
+        L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
         L.fzc_int_coefs.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.fzc_njevals.argtypes = [ctypes.c_int] * 6
@@ -182,6 +183,11 @@ class Cpu:
     def set_gain(self, g):
         g = np.ascontiguousarray(np.asarray(g, dtype=np.float32).T)
         self.lib.fzc_set_gain(_ptr(g, _fp))
+
+    def set_hydrostatic(self, on=True):
+        """HYDROSTATIC: yes -- Pg re-integrated from the top inside forward1d
+        and the hydrostatic term in dI/dT."""
+        self.lib.fzc_set_hydrostatic(int(on))
 
     def forward1d(self, model):
         """model: (13, nz) float32 [T,Pg,rho,Pe,mu,Bx,By,Bz,vx,vy,vz,tau5,z];
@@ -412,6 +418,11 @@ class Ref:
     def set_gain(self, g):
         g = np.ascontiguousarray(np.asarray(g, dtype=np.float32).T)
         self.lib.fzref_set_gain(_ptr(g, _fp))
+
+    def set_hydrostatic(self, on=True):
+        f = self.lib.fzref_set_hydro
+        f.argtypes = [ctypes.c_int]
+        f(int(on))
 
     def forward1d(self, model):
         m = np.asfortranarray(np.asarray(model, dtype=np.float32).T.copy())   # (nz,13) Fortran

@@ -38,6 +38,7 @@ static fzc_line LN[FZC_MAXLINES];
 static int NLN;
 static fzc_region RG[FZC_MAXREG];
 static int NRG, NUMW, RESP;
+static int HYDRO;   /* HYDROSTATIC (and HYDROSTATICDER) */
 static int ATMPAR[8];
 static int PIX0[FZC_MAXREG], PIX1[FZC_MAXREG];
 static double *WAVE;
@@ -359,6 +360,19 @@ static void dne_dt_pg(const saha_t* s, double T, double P, double NE, derivvar_t
     iter++;
   }
   dv->dnelecdtemp_pg = dne_pg_sum(s, T, P, NE, NH, dnew);
+}
+
+/* GET_DNHYDDTEMP (:2008-2052) alone, at the N_H, n_e of a hydrostatic
+ * integration (forward.f90:914-922, HYDROSTATIC=yes). */
+static void eos_derivs_only(float T, float P, float D, double ne, derivvar_t* dv, saha_t* s) {
+  double T8 = (double)T, P8 = (double)P, d8 = (double)D;
+  saha_init(s, T8);
+  dne_dpg_t(s, T8, P8, ne, dv);
+  dne_dt_rho(s, T8, d8, ne, dv);
+  dne_drho_t(s, T8, d8, ne, dv);
+  dne_dt_pg(s, T8, P8, ne, dv);
+  dv->dnhyddtemp_pg = (-1.0 / DENO_SA) * (P8 / (KBOL * pow(T8, 2.0)) + dv->dnelecdtemp_pg);
+  dv->dnhyddtemp_rho = (-MELE / DENO_SAM) * dv->dnelecdtemp_rho;
 }
 
 /* GET_NHNEMU_G (:2391-2441, INV_ATMPAR(3)=.FALSE. branch) followed by
@@ -797,6 +811,7 @@ int fzc_setup(const fzc_line* lines, int nlines, const fzc_region* regs, int nre
 int fzc_numw(void) { return NUMW; }
 void fzc_wave(double* w) { memcpy(w, WAVE, sizeof(double) * NUMW); }
 void fzc_set_gain(const float* g) { memcpy(GAIN, g, sizeof(float) * 4 * NUMW); }
+void fzc_set_hydrostatic(int on) { HYDRO = on ? 1 : 0; }
 
 /* ------------------------------------------------ unit entry points */
 int fzc_eos(float T, float P, double* ne, double* out, double* d) {
@@ -1323,11 +1338,25 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
   saha_t* s = (saha_t*)malloc(sizeof(saha_t));
   const double THR = 2.0e10;
   double ne = 0.0;
+  /* HYDROSTATIC=yes: RK4 from the top first (forward.f90:891-906) */
+  double* vnh = HYDRO ? (double*)malloc(sizeof(double) * nz) : NULL;
+  double* vne = HYDRO ? (double*)malloc(sizeof(double) * nz) : NULL;
+  double* hek = (HYDRO && RESP) ? (double*)calloc(nz, sizeof(double)) : NULL;
+  if (HYDRO) {
+    err = fzc_he(nz, TEM, PG, RHO, PEL, MW, vnh, zz);
+    for (int k = 0; k < nz; ++k) vne[k] = (double)PEL[k] / KBOL / (double)TEM[k];
+  }
   for (int k = 0; k < nz && !err; ++k) {
     double nh;
     derivvar_t dv;
     memset(&dv, 0, sizeof(dv));
-    err = eos_height(&TEM[k], &PG[k], &RHO[k], &MW[k], &PEL[k], &nh, &ne, &dv, s);
+    if (!HYDRO) {
+      err = eos_height(&TEM[k], &PG[k], &RHO[k], &MW[k], &PEL[k], &nh, &ne, &dv, s);
+    } else {
+      nh = vnh[k];
+      ne = vne[k];
+      if (RESP) eos_derivs_only(TEM[k], PG[k], RHO[k], ne, &dv, s);
+    }
     if (err) break;
     err = opac_cont((double)TEM[k], 5000.0, ne, nh, &kc5[k], &dv);
     kc5[k] = kc5[k] / nh * (1.0 / MAMU);
@@ -1386,11 +1415,15 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
       for (int p = 0; p < 7; ++p) {
         m44* D = &E.dev[slot_src[p]];
         double* o = &dsyn[(((size_t)k * NUMW + l) * 8 + p) * 4];
+        /* With HYDROSTATIC(DER) the reference computes rho|T, Bx, By, Bz and
+         * v after its hydrostatic block, which leaves STKIN = SYN1D(:,L), the
+         * layer's outgoing vector (rtesolver.f90:218-258): reproduced. */
+        const double* iv = (HYDRO && p >= 2) ? I : in;
         for (int i = 0; i < 4; ++i) {
           double a = 0.0, b = 0.0, c = 0.0;
           for (int j = 0; j < 4; ++j) {
             a += -(*D)[i][j] * (j == 0 ? sf : 0.0);
-            b += (*D)[i][j] * in[j];
+            b += (*D)[i][j] * iv[j];
           }
           if (p == 0) c = ((i == 0) - E.evol[i][0]) * dsf;
           o[i] = (p == 0) ? (a + c) + b : a + b;
@@ -1398,6 +1431,36 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
       }
       double* o8 = &dsyn[(((size_t)k * NUMW + l) * 8 + 7) * 4];
       for (int i = 0; i < 4; ++i) o8[i] = 1.0;
+      if (HYDRO) {
+        /* GET_HYD_DERIVATIVES (forward.f90:1033-1061): dPg_i/dT_k for i < k,
+         * with the REAL(4) ratio PG(I)/PG(K); then the global term of
+         * rtesolver.f90:218-231: the local Pg|T responses of the layers
+         * below, carried up through their evolution operators (a layer with
+         * O_11 <= 1e-5 is skipped altogether). */
+        const double cte = MAMU / KBOL * GRAV * (zz[k] - zz[k - 1]) * 1.0e3 * 1.0e2;
+        for (int i = 0; i < k; ++i) {
+          float ratio = PG[i] / PG[k];
+          float pm = -PG[k] * MW[k];
+          hek[i] = (double)ratio * ((double)pm * cte / (double)TEM[k] / (double)TEM[k]);
+        }
+        hek[k] = 0.0;
+        double st[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int kb = 0; kb <= k; ++kb) {
+          const double* ev = (kb == k) ? &E.evol[0][0] : &evg[(((size_t)kb * NUMW + l) * 4) * 4];
+          if (ev[0] > 1.0e-5) {
+            const double* d2 = &dsyn[(((size_t)kb * NUMW + l) * 8 + 1) * 4];
+            double r[4];
+            for (int i = 0; i < 4; ++i) {
+              double a = 0.0;
+              for (int q = 0; q < 4; ++q) a += ev[i * 4 + q] * st[q];
+              r[i] = d2[i] * hek[kb] + a;
+            }
+            memcpy(st, r, sizeof(st));
+          }
+        }
+        double* o1 = &dsyn[(((size_t)k * NUMW + l) * 8 + 0) * 4];
+        for (int i = 0; i < 4; ++i) o1[i] = o1[i] + st[i];
+      }
     }
   }
   if (!err) {
@@ -1429,6 +1492,11 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
             }
             memcpy(d, r, sizeof(r));
           }
+          if (ATMPAR[7]) { /* P0 (forward.f90:543-545) */
+            double* d8 = &dsyn[(((size_t)k * NUMW + l) * 8 + 7) * 4];
+            const double* d2 = &dsyn[(((size_t)k * NUMW + l) * 8 + 1) * 4];
+            for (int i = 0; i < 4; ++i) d8[i] = d8[i] * (double)PG[k] / (double)PG[nz - 1] * d2[i];
+          }
         }
       }
       for (int k = 0; k < nz; ++k)
@@ -1449,6 +1517,7 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
     }
     for (int k = 0; k < nz; ++k) TAU5[k] = log10f(TAU5[k]);
   }
+  free(vnh); free(vne); free(hek);
   free(zz); free(kc5); free(syn); free(dsyn); free(evg);
   free(A.eta); free(A.deta); free(A.teta); free(A.tdeta); free(s);
   return err;

@@ -56,6 +56,9 @@ int fzc_evol(const double* x7, double dz_km, const double* dx, double* evol, dou
 /* GAIN1D from the HSRA continuum (normalization.f90:696-754) */
 int fzc_gain(float* gain);
 void fzc_set_gain(const float* gain);
+/* HYDROSTATIC=yes (and HYDROSTATICDER): the forward re-integrates Pg from
+ * the top (RK4_INTEGRATION_LOG) and adds the hydrostatic term to dI/dT. */
+void fzc_set_hydrostatic(int on);
 
 /* One column (forward.f90:857-1029, HYDROSTATIC=no):
  * model[13*nz] column-major MODEL(NZ,13), updated in place;

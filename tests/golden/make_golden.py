@@ -16,6 +16,10 @@ Fixtures (tests/golden/*.npz, data only):
   seeded_c1.npz    a 4x4 patch of the seeded 16x16x64 cube (2 lines, 112
                    wavelengths): hydrostatic Pg, Stokes for 16 columns and
                    the full response functions of 2 columns
+  he.npz           HYDROSTATIC: yes (tests/test_he, test_syn2) on
+                   default/test_atmos.bin, lines 4,5 at 125 wavelengths: the
+                   re-integrated model, Stokes and all eight response
+                   functions including the hydrostatic T term and P0
 """
 import os
 import sys
@@ -140,6 +144,28 @@ def seeded(r):
                         models=np.array(models), stokes=np.array(stokes), rf=np.array(rfs), seed=1)
 
 
+def he(r):
+    ls = lines.setup([((4, 5), 125, -700.0, 20.0)])
+    r.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 1))
+    g = r.gain()
+    r.set_gain(g)
+    c = modelio.read_model(os.path.join(REF, "default", "test_atmos.bin"))
+    m = model_from_cube(c)
+    m[5] += 150.0      # a field and a flow so every response function is non-zero
+    m[6] -= 80.0
+    m[7] += 300.0
+    m[10] += 1.5e5
+    r.set_hydrostatic(True)
+    try:
+        mo, syn, dsyn, kc5 = r.forward1d(m)
+    finally:
+        r.set_hydrostatic(False)
+    keep = np.arange(0, m.shape[1], 8)
+    np.savez_compressed(os.path.join(HERE, "he.npz"), model_in=m, model_out=mo, gain=g, stokes=syn,
+                        rf_heights=keep, rf=dsyn[keep].astype(np.float32), nwave=125, wave_start=-700.0,
+                        wave_step=20.0)
+
+
 def main():
     if not oracle.ref_available():
         oracle.build_ref()
@@ -148,6 +174,7 @@ def main():
     syn1(r)
     rf(r)
     seeded(r)
+    he(r)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -143,3 +143,26 @@ def test_oracle_vs_reference_build(cpu):
         assert np.max(np.abs(a[1] - b[1])) < 1e-12
         for p in range(7):
             assert np.max(np.abs(a[2][:, :, p] - b[2][:, :, p])) <= 1e-9 * np.abs(b[2][:, :, p]).max()
+
+
+def test_hydrostatic_mode_vs_fixture(cpu, golden):
+    """HYDROSTATIC: yes: the RK4 re-integration inside the forward, the
+    hydrostatic term of dI/dT, P0, and the reference's use of the outgoing
+    Stokes vector for the rho, B and v responses in that mode."""
+    g = golden["he"]
+    ls = lines.setup([((4, 5), int(g["nwave"]), float(g["wave_start"]), float(g["wave_step"]))])
+    cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 1))
+    cpu.set_gain(g["gain"])
+    cpu.set_hydrostatic(True)
+    try:
+        mo, syn, dsyn, _ = cpu.forward1d(g["model_in"])
+    finally:
+        cpu.set_hydrostatic(False)
+    for row in (1, 2, 3, 4, 11):
+        assert np.array_equal(mo[row], g["model_out"][row]), row
+    assert np.max(np.abs(syn - g["stokes"])) < 1e-12
+    ours = dsyn[g["rf_heights"]]
+    for p in range(8):
+        sc = np.abs(g["rf"][:, :, p, :]).max()
+        assert sc > 0, p
+        assert np.max(np.abs(ours[:, :, p, :] - g["rf"][:, :, p, :])) <= 1e-6 * sc, p
