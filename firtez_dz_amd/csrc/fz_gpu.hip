@@ -2,9 +2,10 @@
 // and analytic response functions, behind the C ABI of include/firtez_gpu.h.
 //
 // Kernels
-//   micro_kernel : one thread per column, heights bottom->top (the electron
-//                  density warm-starts from the height below, as in FORWARD1D,
-//                  forward.f90:911-960).  FP64: EOS + implicit derivatives,
+//   saha/rho/opac/tau kernels : FP64 microphysics per (column, height): Saha
+//                  factors, the electron-density fixed point (sequential in
+//                  height per column, warm start from the height below as in
+//                  FORWARD1D, forward.f90:911-960), implicit EOS derivatives,
 //                  continuum/line opacity, damping.  Writes a per-height
 //                  record [field][z][column] (columns fastest).
 //   rt_kernel<RF> : one thread per (column, wavelength).  Pass 1 bottom->top:
@@ -57,70 +58,134 @@ __device__ __forceinline__ int fslot(int s, int q) { return F_REG + 4 * c_cfg.nr
 __device__ __forceinline__ int freg(int r, int q) { return F_REG + 4 * r + q; }
 
 // ---------------------------------------------------------------- microphysics
-__global__ void __launch_bounds__(64) micro_kernel(int ncol, float* __restrict__ atm, float* __restrict__ micro,
-                                                   int32_t* __restrict__ status, int der) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
+// FORWARD1D's per-height physics (forward.f90:911-1006) over a tile of
+// columns.  A column loop with one thread per column would give a few ten
+// thousand threads (~1 wave per SIMD); only the electron-density fixed point
+// is sequential in height (warm start from the height below), so the work is
+// split into:
+//   saha_kernel  one thread per (height, column): partition functions and the
+//                Saha factors of all 92 elements -> sbuf[k][e][4][column]
+//   rho_kernel   one thread per column, heights bottom->top: only the warm-
+//                started electron-density fixed point (GET_RHO), reading the
+//                factors from sbuf -> rho, mu, Pe and (n_e, N_H) per height
+//   opac_kernel  one thread per (height, column): implicit EOS derivatives,
+//                continuum and line opacity, damping -> the per-height record
+//   tau_kernel   one thread per column: the REAL(4) log tau5 trapezoid
+// FP64 throughout; the fixed point keeps the reference's iteration path
+// (stopping rule 1e-4, warm start), so the EOS outputs are those of the
+// sequential column loop.
+struct MicroTile {
+  int ncol, col0, ntile, der;
+  float* atm;
+  float* micro;
+  double* sbuf;    // [nz][92][4][ntile]
+  double* eos;     // [2][nz][ntile]: n_e, N_H
+  int32_t* status; // [ncol]
+};
+
+__global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
   const int nz = c_cfg.nz;
-  const long cs = ncol;                       // stride between heights
-  const long ps = (long)nz * ncol;            // stride between parameters / fields
-  auto A = [&](int p, int k) -> float& { return atm[p * ps + k * cs + col]; };
-  auto M = [&](int f, int k) -> float& { return micro[f * ps + k * cs + col]; };
-  Saha s;
-  Deriv d = {};
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nz * a.ntile) return;
+  const int k = (int)(t / a.ntile), ct = (int)(t - (long)k * a.ntile);
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  const double T = (double)a.atm[A_T * ps + k * cs + a.col0 + ct];
+  const double lam = g_c.csaha1 * pow(T, -1.5);
+  const double dlam = -3.0 / (2.0 * T);
+  double* o = a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct;
+  for (int i = 0; i < FZ_NELEM; ++i) {
+    double a1, a2, da1, da2;
+    saha_element(i, T, lam, dlam, a.der != 0, a1, a2, da1, da2);
+    o[(long)(i * 4 + 0) * a.ntile] = a1;
+    o[(long)(i * 4 + 1) * a.ntile] = a2;
+    o[(long)(i * 4 + 2) * a.ntile] = da1;
+    o[(long)(i * 4 + 3) * a.ntile] = da2;
+  }
+}
+
+__global__ void __launch_bounds__(64) rho_kernel(MicroTile a) {
+  const int ct = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ct >= a.ntile) return;
+  const int nz = c_cfg.nz, col = a.col0 + ct;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
   double ne = 0.0;
   int err = ERR_OK;
   for (int k = 0; k < nz; ++k) {
-    const double T = (double)A(A_T, k), P = (double)A(A_PG, k);
-    saha_init(s, T, der != 0);
+    const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
+    const SahaView s(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile);
     double nh, pel, dens, mu;
     err = get_rho(s, T, P, ne, nh, pel, dens, mu);
     if (err) break;
-    const float rhof = (float)dens;
-    A(A_RHO, k) = rhof;
-    A(A_MU, k) = (float)mu;
-    A(A_PEL, k) = (float)pel;
-    if (der) eos_derivatives(s, T, P, (double)rhof, ne, d);
-    double kc[4];
-    err = opac_cont(T, 5000.0, ne, nh, d, false, kc);
-    if (err) break;
+    a.atm[A_RHO * ps + k * cs + col] = (float)dens;
+    a.atm[A_MU * ps + k * cs + col] = (float)mu;
+    a.atm[A_PEL * ps + k * cs + col] = (float)pel;
+    a.eos[(long)k * a.ntile + ct] = ne;
+    a.eos[((long)nz + k) * a.ntile + ct] = nh;
+  }
+  a.status[col] = err;
+}
+
+__global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
+  const int nz = c_cfg.nz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nz * a.ntile) return;
+  const int k = (int)(t / a.ntile), ct = (int)(t - (long)k * a.ntile), col = a.col0 + ct;
+  if (a.status[col] != 0) return;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  auto M = [&](int f) -> float& { return a.micro[f * ps + k * cs + col]; };
+  const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
+  const float rhof = a.atm[A_RHO * ps + k * cs + col];
+  const double ne = a.eos[(long)k * a.ntile + ct], nh = a.eos[((long)nz + k) * a.ntile + ct];
+  const SahaView s(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile);
+  Deriv d = {};
+  if (a.der) eos_derivatives(s, T, P, (double)rhof, ne, d);
+  double kc[4];
+  int err = opac_cont(T, 5000.0, ne, nh, d, false, kc);
+  if (!err) {
     const double kc5 = kc[0] / nh * (1.0 / MAMU);
     const bool bnd = (k == 0) || (kc5 >= 2.0e10);     // forward.f90:884,931,971
-    M(F_FLAG, k) = bnd ? 1.0f : 0.0f;
-    M(F_KC5, k) = (float)kc5;
-    if (bnd) continue;
+    M(F_FLAG) = bnd ? 1.0f : 0.0f;
+    M(F_KC5) = (float)kc5;
+    if (bnd) return;
     for (int r = 0; r < c_cfg.nreg && !err; ++r) {
-      err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, der != 0, kc);
-      for (int q = 0; q < 4; ++q) M(freg(r, q), k) = (float)kc[q];
+      err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, a.der != 0, kc);
+      for (int q = 0; q < 4; ++q) M(freg(r, q)) = (float)kc[q];
       for (int b = 0; b < c_cfg.reg_nblend[r] && !err; ++b) {
         const int sl = c_cfg.reg_slot0[r] + b;
         const LineConst& L = c_cfg.line[c_cfg.slot_line[sl]];
         double kl[4] = {0, 0, 0, 0}, dm[4] = {0, 0, 0, 0};
-        err = opac_line(L, T, ne, nh, s, d, der != 0, kl);
-        damping(L, T, nh, d, der != 0, dm);
-        M(fslot(sl, 0), k) = (float)kl[0];
-        M(fslot(sl, 1), k) = (float)dm[0];
-        M(fslot(sl, 2), k) = (float)kl[1];
-        M(fslot(sl, 3), k) = (float)kl[2];
-        M(fslot(sl, 4), k) = (float)kl[3];
-        M(fslot(sl, 5), k) = (float)dm[1];
-        M(fslot(sl, 6), k) = (float)dm[2];
-        M(fslot(sl, 7), k) = (float)dm[3];
+        err = opac_line(L, T, ne, nh, s, d, a.der != 0, kl);
+        damping(L, T, nh, d, a.der != 0, dm);
+        M(fslot(sl, 0)) = (float)kl[0];
+        M(fslot(sl, 1)) = (float)dm[0];
+        M(fslot(sl, 2)) = (float)kl[1];
+        M(fslot(sl, 3)) = (float)kl[2];
+        M(fslot(sl, 4)) = (float)kl[3];
+        M(fslot(sl, 5)) = (float)dm[1];
+        M(fslot(sl, 6)) = (float)dm[2];
+        M(fslot(sl, 7)) = (float)dm[3];
       }
     }
-    if (err) break;
   }
-  if (!err) {
-    // log tau_5000 by the REAL(4) trapezoid of forward.f90:1000-1006 (z in km)
-    float tau = 1e-9f;
-    A(A_TAU, nz - 1) = log10f(tau);
-    for (int k = nz - 2; k >= 0; --k) {
-      const float a = (M(F_KC5, k) * A(A_RHO, k) + M(F_KC5, k + 1) * A(A_RHO, k + 1)) / 2.0f;
-      tau = (float)((double)tau + (double)a * ((c_cfg.zz[k + 1] - c_cfg.zz[k]) * 1000.0 * 100.0));
-      A(A_TAU, k) = log10f(tau);
-    }
+  if (err) atomicMax(&a.status[col], err);
+}
+
+__global__ void __launch_bounds__(64) tau_kernel(MicroTile a) {
+  const int ct = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ct >= a.ntile) return;
+  const int nz = c_cfg.nz, col = a.col0 + ct;
+  if (a.status[col] != 0) return;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  // log tau_5000 by the REAL(4) trapezoid of forward.f90:1000-1006 (z in km)
+  float tau = 1e-9f;
+  a.atm[A_TAU * ps + (nz - 1) * cs + col] = log10f(tau);
+  for (int k = nz - 2; k >= 0; --k) {
+    const float x = (a.micro[F_KC5 * ps + k * cs + col] * a.atm[A_RHO * ps + k * cs + col] +
+                     a.micro[F_KC5 * ps + (k + 1) * cs + col] * a.atm[A_RHO * ps + (k + 1) * cs + col]) /
+                    2.0f;
+    tau = (float)((double)tau + (double)x * ((c_cfg.zz[k + 1] - c_cfg.zz[k]) * 1000.0 * 100.0));
+    a.atm[A_TAU * ps + k * cs + col] = log10f(tau);
   }
-  if (status) status[col] = err;
 }
 
 // ---------------------------------------------------------------- radiative transfer
@@ -634,6 +699,14 @@ struct fz_ctx {
   double* d_colsum = nullptr; // [np]
   float* d_jtile = nullptr;   // per-height response functions of one tile
   size_t jtile_cap = 0;
+  // microphysics tile buffers
+  int mtile = 16384;
+  float* d_sbuf = nullptr;    // Saha factors [nz][92][4][mtile] (double)
+  size_t sbuf_cap = 0;
+  float* d_eosb = nullptr;    // n_e, N_H [2][nz][mtile] (double)
+  size_t eosb_cap = 0;
+  float* d_stat = nullptr;    // per-column status when the caller passes none
+  size_t stat_cap = 0;
   size_t lm_lds = 0;
 };
 
@@ -689,6 +762,9 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_coefs) (void)hipFree(c->d_coefs);
   if (c->d_colsum) (void)hipFree(c->d_colsum);
   if (c->d_jtile) (void)hipFree(c->d_jtile);
+  if (c->d_sbuf) (void)hipFree(c->d_sbuf);
+  if (c->d_eosb) (void)hipFree(c->d_eosb);
+  if (c->d_stat) (void)hipFree(c->d_stat);
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -845,6 +921,43 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   return FZ_OK;
 }
 
+// Microphysics over all columns, tile by tile; status[ncol] receives each
+// column's FZ_E* code (the caller's array, or the context's own).
+static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool der, hipStream_t st) {
+  const int nz = c->cfg.nz;
+  int32_t* stv = *status;
+  if (!stv) {
+    int r = ensure(c, &c->d_stat, &c->stat_cap, sizeof(int32_t) * (size_t)ncol);
+    if (r) return r;
+    stv = (int32_t*)c->d_stat;
+    *status = stv;
+  }
+  const int mt = std::min(c->mtile, ncol);
+  int r = ensure(c, &c->d_sbuf, &c->sbuf_cap, sizeof(double) * (size_t)nz * FZ_NELEM * 4 * mt);
+  if (r) return r;
+  r = ensure(c, &c->d_eosb, &c->eosb_cap, sizeof(double) * (size_t)2 * nz * mt);
+  if (r) return r;
+  for (int c0 = 0; c0 < ncol; c0 += mt) {
+    MicroTile a;
+    a.ncol = ncol;
+    a.col0 = c0;
+    a.ntile = std::min(mt, ncol - c0);
+    a.der = der ? 1 : 0;
+    a.atm = d_atm;
+    a.micro = c->d_micro;
+    a.sbuf = (double*)c->d_sbuf;
+    a.eos = (double*)c->d_eosb;
+    a.status = stv;
+    const long nkt = (long)nz * a.ntile;
+    hipLaunchKernelGGL(saha_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(rho_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(opac_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(tau_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+    HIPCHK(c, hipGetLastError());
+  }
+  return FZ_OK;
+}
+
 int fz_set_tile(fz_ctx* c, int tile, int block) {
   if (!c || tile < 1 || block < 64 || block > 1024 || block % 64) return FZ_EARG;
   c->tile = tile;
@@ -870,9 +983,8 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
-  hipLaunchKernelGGL(micro_kernel, dim3((ncol + 63) / 64), dim3(64), 0, st, ncol, d_atm, c->d_micro, d_status,
-                     rf_mode != FZ_RF_NONE ? 1 : 0);
-  HIPCHK(c, hipGetLastError());
+  r = run_micro(c, ncol, d_atm, &d_status, rf_mode != FZ_RF_NONE, st);
+  if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
   c->rt_launches = 0;
   for (int c0 = 0; c0 < ncol; c0 += tile) {
@@ -1162,8 +1274,8 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * 7 * g.nz * 4 * g.numw);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
-  hipLaunchKernelGGL(micro_kernel, dim3((ncol + 63) / 64), dim3(64), 0, st, ncol, d_atm, c->d_micro, d_status, 1);
-  HIPCHK(c, hipGetLastError());
+  r = run_micro(c, ncol, d_atm, &d_status, true, st);
+  if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
   c->rt_launches = 0;
   EqvArgs e;

@@ -108,23 +108,50 @@ struct Saha {
   double a1[FZ_NELEM], a2[FZ_NELEM], da1[FZ_NELEM], da2[FZ_NELEM];
 };
 
+// The same four per-element rows read from a device buffer laid out
+// [height][element][4][column] (columns fastest), as the parallel Saha
+// kernel writes them: row q of element j at p[(j*4 + q) * stride].
+struct SahaRow {
+  const double* p;
+  long st;
+  __device__ __forceinline__ double operator[](int j) const { return p[(long)j * st]; }
+};
+struct SahaView {
+  SahaRow a1, a2, da1, da2;
+  __device__ __forceinline__ SahaView(const double* base, long stride)
+      : a1{base, 4 * stride}, a2{base + stride, 4 * stride}, da1{base + 2 * stride, 4 * stride},
+        da2{base + 3 * stride, 4 * stride} {}
+};
+
+// Saha factors of one element at temperature T (the loop body of saha_init)
+__device__ __forceinline__ void saha_element(int i, double T, double lam, double dlam, bool der, double& a1,
+                                             double& a2, double& da1, double& da2) {
+  double u[3], du[3];
+  partition(i, T, u, du);
+  double e1 = g_c.csaha2 * fz_xi[i] / T, e2 = g_c.csaha2 * fz_xii[i] / T;
+  e1 = fmax(fmin(e1, 650.0), -650.0);
+  e2 = fmax(fmin(e2, 650.0), -650.0);
+  a1 = (u[0] / (2.0 * u[1])) * lam * exp(e1);
+  a2 = (u[1] / (2.0 * u[2])) * lam * exp(e2);
+  da1 = da2 = 0.0;
+  if (der) {
+    const double T2 = T * T;
+    da1 = a1 * (du[0] - du[1] + dlam + (-g_c.csaha2 * fz_xi[i] / T2));
+    da2 = a2 * (du[1] - du[2] + dlam + (-g_c.csaha2 * fz_xii[i] / T2));
+  }
+}
+
 __device__ inline void saha_init(Saha& s, double T, bool der) {
   const double lam = g_c.csaha1 * pow(T, -1.5);
   const double dlam = -3.0 / (2.0 * T);
   for (int i = 0; i < FZ_NELEM; ++i) {
-    double u[3], du[3];
-    partition(i, T, u, du);
-    double e1 = g_c.csaha2 * fz_xi[i] / T, e2 = g_c.csaha2 * fz_xii[i] / T;
-    e1 = fmax(fmin(e1, 650.0), -650.0);
-    e2 = fmax(fmin(e2, 650.0), -650.0);
-    const double a1 = (u[0] / (2.0 * u[1])) * lam * exp(e1);
-    const double a2 = (u[1] / (2.0 * u[2])) * lam * exp(e2);
+    double a1, a2, da1, da2;
+    saha_element(i, T, lam, dlam, der, a1, a2, da1, da2);
     s.a1[i] = a1;
     s.a2[i] = a2;
     if (der) {
-      const double T2 = T * T;
-      s.da1[i] = a1 * (du[0] - du[1] + dlam + (-g_c.csaha2 * fz_xi[i] / T2));
-      s.da2[i] = a2 * (du[1] - du[2] + dlam + (-g_c.csaha2 * fz_xii[i] / T2));
+      s.da1[i] = da1;
+      s.da2[i] = da2;
     }
   }
 }
@@ -146,7 +173,8 @@ enum : int { ERR_OK = 0, ERR_EOS = 1, ERR_NAN = 2, ERR_ION = 3, ERR_LAMBDA = 4 }
 
 // GET_RHO (chemical_equilibrium.f90:165-278).  ne: warm start in, result out.
 // pelec is the starting guess's electron pressure, as the reference returns.
-__device__ inline int get_rho(const Saha& s, double T, double PG, double& ne, double& nh,
+template <class S>
+__device__ inline int get_rho(const S& s, double T, double PG, double& ne, double& nh,
                               double& pelec, double& dens, double& mu) {
   const double kT = KBOL * T;
   double n = ne;
@@ -186,7 +214,8 @@ __device__ inline int get_rho(const Saha& s, double T, double PG, double& ne, do
 
 // The four implicit-derivative solves of GET_DNHYDDTEMP
 // (chemical_equilibrium.f90:2008-2052).
-__device__ inline void eos_derivatives(const Saha& s, double T, double P, double RHO, double NE,
+template <class S>
+__device__ inline void eos_derivatives(const S& s, double T, double P, double RHO, double NE,
                                        Deriv& d) {
   const double kT = KBOL * T;
   const double SA = g_c.deno_sa, SAM = g_c.deno_sam;
@@ -399,7 +428,8 @@ struct LineConst {   // per-line constants prepared on the host
 
 // Line opacity (line_opacity.f90:34-217) with ION_CALC_TEMPX (:1511-1597)
 // and ION_CALC_PGTEMP (:1446-1507).  out: kl, dkl/dT|Pg, dkl/dPg|T, dkl/drho|T.
-__device__ inline int opac_line(const LineConst& L, double T, double NE, double NH, const Saha& s,
+template <class S>
+__device__ inline int opac_line(const LineConst& L, double T, double NE, double NH, const S& s,
                                 const Deriv& d, bool der, double out[4]) {
   const int z = L.zn, ion = L.ion;
   if (ion < 1 || ion > 3) return ERR_ION;
