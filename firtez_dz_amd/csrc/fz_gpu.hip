@@ -33,7 +33,8 @@
 
 namespace fz {
 
-constexpr int MAXL = 16, MAXR = 8, MAXS = 32, MAXC = 1024, MAXNZ = 512;
+constexpr int MAXL = 16, MAXR = 8, MAXS = 32, MAXC = 1024, MAXNZ = 512, MAXBLK = 128;
+constexpr int RT_WAVE = 64;   // wavelengths per radiative-transfer workgroup (one wave)
 constexpr int NATM = 12;
 enum { A_T = 0, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VX, A_VY, A_VZ, A_TAU };
 enum { F_FLAG = 0, F_KC5 = 1, F_REG = 2 };
@@ -51,6 +52,11 @@ struct Cfg {
   float comp_shift[MAXC], comp_str[MAXC];
   float dz_cm[MAXNZ];
   double zz[MAXNZ];    // heights in km, rounded through REAL(4) like MODEL(:,13)
+  // radiative-transfer workgroups of one column: each covers up to RT_WAVE
+  // wavelengths of a single region, so the column and the region (and every
+  // per-height quantity read) are uniform across the wave
+  int nbpc;
+  int blk_reg[MAXBLK], blk_l0[MAXBLK], blk_n[MAXBLK];
 };
 __constant__ Cfg c_cfg;
 
@@ -189,6 +195,9 @@ __global__ void __launch_bounds__(64) tau_kernel(MicroTile a) {
 }
 
 // ---------------------------------------------------------------- radiative transfer
+// Launch description of rt_kernel (host side; the kernel receives the
+// pointers as separate restrict-qualified arguments so that loads whose
+// address is uniform across the wave become scalar loads).
 struct RtArgs {
   int ncol;          // columns in the whole cube (strides of atm/micro)
   int col0;          // first column of this launch
@@ -390,21 +399,38 @@ __device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
 }
 
 template <int RF>
-__global__ void __launch_bounds__(256) rt_kernel(RtArgs a) {
+__global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ atm, const float* __restrict__ micro,
+                                                     const float* __restrict__ gain, const int32_t* __restrict__ status,
+                                                     float* __restrict__ stokes, float* __restrict__ jac,
+                                                     float* __restrict__ scratch, int ncol, int col0, int ntile,
+                                                     int jcol0) {
+  RtArgs a;
+  a.ncol = ncol;
+  a.col0 = col0;
+  a.ntile = ntile;
+  a.atm = atm;
+  a.micro = micro;
+  a.gain = gain;
+  a.status = status;
+  a.stokes = stokes;
+  a.jac = jac;
+  a.jcol0 = jcol0;
+  a.scratch = scratch;
   constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : 0);
   const int numw = c_cfg.numw, nz = c_cfg.nz;
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  // workgroup -> (column, region, first wavelength); lane -> wavelength
+  const int ct = (int)(blockIdx.x / (unsigned)c_cfg.nbpc);
+  const int bc = (int)(blockIdx.x - (unsigned)ct * (unsigned)c_cfg.nbpc);
+  if ((int)threadIdx.x >= c_cfg.blk_n[bc]) return;
+  const int r = c_cfg.blk_reg[bc];
+  const int l = c_cfg.blk_l0[bc] + (int)threadIdx.x;
+  const long t = (long)ct * numw + l;
   const long nthr = (long)a.ntile * numw;
-  if (t >= nthr) return;
-  const int ct = (int)(t / numw);
-  const int l = (int)(t - (long)ct * numw);
   const int col = a.col0 + ct;
   if (a.status && a.status[col] != 0) {
     for (int s = 0; s < 4; ++s) a.stokes[((long)col * 4 + s) * numw + l] = __builtin_nanf("");
     return;
   }
-  int r = 0;
-  while (r + 1 < c_cfg.nreg && l >= c_cfg.reg_pix0[r + 1]) ++r;
   const double wd = c_cfg.reg_wstart[r] + c_cfg.reg_wstep[r] * (double)(l - c_cfg.reg_pix0[r]);
   const float wave = (float)wd;
   // Planck function constants (rtesolver.f90:1008-1035): S = c1 / (exp(c2/T) - 1)
@@ -858,6 +884,15 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
   }
   g.nslot = ns;
   g.numw = numw;
+  g.nbpc = 0;
+  for (int r = 0; r < nreg; ++r)
+    for (int l0 = g.reg_pix0[r]; l0 < g.reg_pix0[r] + g.reg_nwave[r]; l0 += RT_WAVE) {
+      if (g.nbpc >= MAXBLK) return set_err(c, FZ_ECONFIG, "too many wavelengths");
+      g.blk_reg[g.nbpc] = r;
+      g.blk_l0[g.nbpc] = l0;
+      g.blk_n[g.nbpc] = std::min(RT_WAVE, g.reg_pix0[r] + g.reg_nwave[r] - l0);
+      ++g.nbpc;
+    }
   g.nfield = F_REG + 4 * nreg + 8 * ns;
   // layer thickness uses the spacing to the next height, the top one the
   // spacing below (rtesolver.f90:439-443); z rounded to REAL(4) like MODEL(:,13)
@@ -918,6 +953,17 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   *cap = 0;
   if (hipMalloc((void**)p, bytes) != hipSuccess) return set_err(c, FZ_ENOMEM, "device allocation failed");
   *cap = bytes;
+  return FZ_OK;
+}
+
+static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
+  const dim3 grid((unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
+#define FZ_RT_ARGS a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0
+  if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL(rt_kernel<FZ_RF_NONE>, grid, blk, 0, st, FZ_RT_ARGS);
+  else if (rf_mode == FZ_RF_SPHERICAL) hipLaunchKernelGGL(rt_kernel<FZ_RF_SPHERICAL>, grid, blk, 0, st, FZ_RT_ARGS);
+  else hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, grid, blk, 0, st, FZ_RT_ARGS);
+#undef FZ_RT_ARGS
+  HIPCHK(c, hipGetLastError());
   return FZ_OK;
 }
 
@@ -1000,11 +1046,8 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     a.jac = d_jac;
     a.jcol0 = 0;
     a.scratch = c->d_scratch;
-    const long nthr = (long)a.ntile * g.numw;
-    const dim3 grid((unsigned)((nthr + c->block - 1) / c->block)), blk(c->block);
-    if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL(rt_kernel<FZ_RF_NONE>, grid, blk, 0, st, a);
-    else if (rf_mode == FZ_RF_SPHERICAL) hipLaunchKernelGGL(rt_kernel<FZ_RF_SPHERICAL>, grid, blk, 0, st, a);
-    else hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, grid, blk, 0, st, a);
+    r = launch_rt(c, rf_mode, a, st);
+    if (r) return r;
     HIPCHK(c, hipGetLastError());
     c->rt_launches++;
   }
@@ -1305,10 +1348,8 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     a.jac = c->d_jtile;
     a.jcol0 = c0;
     a.scratch = c->d_scratch;
-    const long nthr = (long)a.ntile * g.numw;
-    hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, dim3((unsigned)((nthr + c->block - 1) / c->block)), dim3(c->block),
-                       0, st, a);
-    HIPCHK(c, hipGetLastError());
+    r = launch_rt(c, FZ_RF_CARTESIAN, a, st);
+    if (r) return r;
     e.ncol_tile = a.ntile;
     e.col0 = c0;
     e.jac = c->d_jtile;
