@@ -17,7 +17,9 @@
 //                  dO(I_in - S) + (1-O) dS, and propagates it to the top with
 //                  the running product O_nz...O_{k+1} (forward.f90:511-548).
 //   hsra_kernel / gain_kernel : continuum normalisation (normalization.f90).
-//   he_kernel    : hydrostatic RK4 from the top (hydrostatic_tools.f90:150-396).
+//   saha_mid/he_rk4 kernels : hydrostatic RK4 from the top
+//                  (hydrostatic_tools.f90:150-396), standalone or inside the
+//                  forward with HYDROSTATIC: yes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,6 +59,10 @@ struct Cfg {
   // per-height quantity read) are uniform across the wave
   int nbpc;
   int blk_reg[MAXBLK], blk_l0[MAXBLK], blk_n[MAXBLK];
+  // hydrostatic RK4 steps k+1 -> k (hydrostatic_tools.f90:166-212): step dz
+  // and the two weights of the mid-step temperature, with the running height
+  // t0 accumulated exactly as the reference does
+  double he_dz[MAXNZ], he_a[MAXNZ], he_b[MAXNZ];
 };
 __constant__ Cfg c_cfg;
 
@@ -192,6 +198,88 @@ __global__ void __launch_bounds__(64) tau_kernel(MicroTile a) {
     tau = (float)((double)tau + (double)x * ((c_cfg.zz[k + 1] - c_cfg.zz[k]) * 1000.0 * 100.0));
     a.atm[A_TAU * ps + k * cs + col] = log10f(tau);
   }
+}
+
+// ---------------------------------------------------------------- hydrostatic equilibrium
+// RK4_INTEGRATION_LOG (hydrostatic_tools.f90:150-222, RK4_LOG :350-396) over
+// a tile: the Saha factors at the heights come from saha_kernel, those at the
+// RK4 mid-step temperatures from saha_mid_kernel, and he_rk4_kernel (one
+// thread per column, top -> bottom) runs only the fixed points.
+__device__ __forceinline__ double he_tmid(int k, double tm0, double tm1) {
+  const double dz = c_cfg.he_dz[k];
+  return tm0 / dz * c_cfg.he_a[k] + tm1 / dz * c_cfg.he_b[k];
+}
+
+__global__ void __launch_bounds__(256) saha_mid_kernel(MicroTile a, double* __restrict__ smid) {
+  const int nz = c_cfg.nz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)(nz - 1) * a.ntile) return;
+  const int k = (int)(t / a.ntile), ct = (int)(t - (long)k * a.ntile);
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  const double tm0 = (double)a.atm[A_T * ps + (k + 1) * cs + a.col0 + ct];
+  const double tm1 = (double)a.atm[A_T * ps + k * cs + a.col0 + ct];
+  const double T = he_tmid(k, tm0, tm1);
+  const double lam = g_c.csaha1 * pow(T, -1.5);
+  double* o = smid + (long)k * FZ_NELEM * 4 * a.ntile + ct;
+  for (int i = 0; i < FZ_NELEM; ++i) {
+    double a1, a2, da1, da2;
+    saha_element(i, T, lam, 0.0, false, a1, a2, da1, da2);
+    o[(long)(i * 4 + 0) * a.ntile] = a1;
+    o[(long)(i * 4 + 1) * a.ntile] = a2;
+  }
+}
+
+template <class S>
+__device__ __forceinline__ double he_step_rhs(const S& s, double lnp, double tem, double& nel, int& err) {
+  double nh, pel, dens, mw;
+  const int e = get_rho(s, tem, exp(lnp), nel, nh, pel, dens, mw);
+  if (e) err = e;
+  return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
+}
+
+// writes Pg, rho, Pe, mu of every height and, for the forward, (n_e, N_H)
+// with n_e = Pe/(k T) from the REAL(4) Pe and T (forward.f90:904)
+__global__ void __launch_bounds__(64) he_rk4_kernel(MicroTile a, const double* __restrict__ smid) {
+  const int ct = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ct >= a.ntile) return;
+  const int nz = c_cfg.nz, col = a.col0 + ct;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  auto A = [&](int p, int k) -> float& { return a.atm[p * ps + k * cs + col]; };
+  auto SH = [&](int k) { return SahaView(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile); };
+  auto SM = [&](int k) { return SahaView(smid + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile); };
+  int err = 0;
+  const int top = nz - 1;
+  double u0 = log((double)A(A_PG, top));
+  double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
+  double nh, pel, dens, mw;
+  err = get_rho(SH(top), (double)A(A_T, top), exp(u0), nel, nh, pel, dens, mw);
+  A(A_PG, top) = (float)exp(u0);
+  A(A_RHO, top) = (float)dens;
+  A(A_PEL, top) = (float)pel;
+  A(A_MU, top) = (float)mw;
+  if (a.eos) a.eos[((long)nz + top) * a.ntile + ct] = nh;
+  for (int k = nz - 2; k >= 0 && !err; --k) {
+    const double dz = c_cfg.he_dz[k];
+    const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
+    const double k1 = he_step_rhs(SH(k + 1), u0, tm0, nel, err) * dz;
+    const double tmid = he_tmid(k, tm0, tm1);
+    const double k2 = he_step_rhs(SM(k), u0 + k1 / 2.0, tmid, nel, err) * dz;
+    const double k3 = he_step_rhs(SM(k), u0 + k2 / 2.0, tmid, nel, err) * dz;
+    const double k4 = he_step_rhs(SH(k), u0 + k3, tm1, nel, err) * dz;
+    const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+    const int e = get_rho(SH(k), tm1, exp(u1), nel, nh, pel, dens, mw);
+    if (e) err = e;
+    A(A_PG, k) = (float)exp(u1);
+    A(A_RHO, k) = (float)dens;
+    A(A_PEL, k) = (float)pel;
+    A(A_MU, k) = (float)mw;
+    if (a.eos) a.eos[((long)nz + k) * a.ntile + ct] = nh;
+    u0 = u1;
+  }
+  if (a.eos)
+    for (int k = 0; k < nz; ++k)
+      a.eos[(long)k * a.ntile + ct] = (double)A(A_PEL, k) / KBOL / (double)A(A_T, k);
+  a.status[col] = err;
 }
 
 // ---------------------------------------------------------------- radiative transfer
@@ -626,57 +714,6 @@ __global__ void gain_kernel(const double* __restrict__ kc, float* __restrict__ g
   gain[4 * l + 3] = -g;   // V sign convention (normalization.f90:743,751)
 }
 
-// ---------------------------------------------------------------- hydrostatic equilibrium
-__device__ double he_rhs(Saha& s, double lnp, double tem, double& nel, int& err) {
-  double nh, pel, dens, mw;
-  saha_init(s, tem, false);
-  const int e = get_rho(s, tem, exp(lnp), nel, nh, pel, dens, mw);
-  if (e) err = e;
-  return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
-}
-
-__global__ void __launch_bounds__(64) he_kernel(int ncol, float* __restrict__ atm, int32_t* __restrict__ status) {
-  const int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  const int nz = c_cfg.nz;
-  const long cs = ncol, ps = (long)nz * ncol;
-  auto A = [&](int p, int k) -> float& { return atm[p * ps + k * cs + col]; };
-  Saha s;
-  int err = 0;
-  const int top = nz - 1;
-  double u0 = log((double)A(A_PG, top));
-  double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
-  double nh, pel, dens, mw;
-  saha_init(s, (double)A(A_T, top), false);
-  err = get_rho(s, (double)A(A_T, top), exp(u0), nel, nh, pel, dens, mw);
-  A(A_PG, top) = (float)exp(u0);
-  A(A_RHO, top) = (float)dens;
-  A(A_PEL, top) = (float)pel;
-  A(A_MU, top) = (float)mw;
-  double t0 = c_cfg.zz[top] * 1000.0 * 100.0;
-  for (int k = nz - 2; k >= 0 && !err; --k) {
-    // RK4_LOG (hydrostatic_tools.f90:350-396); z decreasing, dz < 0
-    const double dz = (c_cfg.zz[k] - c_cfg.zz[k + 1]) * 1000.0 * 100.0;
-    const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
-    const double k1 = he_rhs(s, u0, tm0, nel, err) * dz;
-    const double it1 = t0 + dz / 2.0;
-    const double tmid = tm0 / dz * (t0 + dz - it1) + tm1 / dz * (it1 - t0);
-    const double k2 = he_rhs(s, u0 + k1 / 2.0, tmid, nel, err) * dz;
-    const double k3 = he_rhs(s, u0 + k2 / 2.0, tmid, nel, err) * dz;
-    const double k4 = he_rhs(s, u0 + k3, tm1, nel, err) * dz;
-    const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
-    saha_init(s, tm1, false);
-    const int e = get_rho(s, tm1, exp(u1), nel, nh, pel, dens, mw);
-    if (e) err = e;
-    A(A_PG, k) = (float)exp(u1);
-    A(A_RHO, k) = (float)dens;
-    A(A_PEL, k) = (float)pel;
-    A(A_MU, k) = (float)mw;
-    u0 = u1;
-    t0 = t0 + dz;
-  }
-  if (status) status[col] = err;
-}
 
 // ---------------------------------------------------------------- layout transposes for FORWARD3D
 // Fortran cube (NZ,NY,NX) -> device SoA [z][col] with col = y + NY*x
@@ -733,6 +770,9 @@ struct fz_ctx {
   size_t eosb_cap = 0;
   float* d_stat = nullptr;    // per-column status when the caller passes none
   size_t stat_cap = 0;
+  float* d_smid = nullptr;    // Saha factors at the RK4 mid-step temperatures (double)
+  size_t smid_cap = 0;
+  bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
 };
 
@@ -791,6 +831,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_sbuf) (void)hipFree(c->d_sbuf);
   if (c->d_eosb) (void)hipFree(c->d_eosb);
   if (c->d_stat) (void)hipFree(c->d_stat);
+  if (c->d_smid) (void)hipFree(c->d_smid);
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -902,6 +943,17 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
     g.dz_cm[k] = (float)(1.0e5 * fabs(zo - zk));
     g.zz[k] = zk;
   }
+  {
+    double t0 = g.zz[nz - 1] * 1000.0 * 100.0;
+    for (int k = nz - 2; k >= 0; --k) {
+      const double dz = (g.zz[k] - g.zz[k + 1]) * 1000.0 * 100.0;
+      const double it1 = t0 + dz / 2.0;
+      g.he_dz[k] = dz;
+      g.he_a[k] = t0 + dz - it1;
+      g.he_b[k] = it1 - t0;
+      t0 = t0 + dz;
+    }
+  }
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &g, sizeof(Cfg)));
   if (c->device < 64) g_owner[c->device] = c;
@@ -968,8 +1020,12 @@ static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
 }
 
 // Microphysics over all columns, tile by tile; status[ncol] receives each
-// column's FZ_E* code (the caller's array, or the context's own).
-static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool der, hipStream_t st) {
+// column's FZ_E* code (the caller's array, or the context's own).  With
+// hydro, Pg is first re-integrated from the top (forward.f90:891-906) and the
+// EOS takes N_H and n_e from that integration; with physics == false only the
+// hydrostatic integration runs (fz_hydrostatic_dev).
+static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool der, bool hydro, bool physics,
+                     hipStream_t st) {
   const int nz = c->cfg.nz;
   int32_t* stv = *status;
   if (!stv) {
@@ -983,24 +1039,43 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
   if (r) return r;
   r = ensure(c, &c->d_eosb, &c->eosb_cap, sizeof(double) * (size_t)2 * nz * mt);
   if (r) return r;
+  if (hydro) {
+    r = ensure(c, &c->d_smid, &c->smid_cap, sizeof(double) * (size_t)(nz - 1) * FZ_NELEM * 4 * mt);
+    if (r) return r;
+  }
   for (int c0 = 0; c0 < ncol; c0 += mt) {
     MicroTile a;
     a.ncol = ncol;
     a.col0 = c0;
     a.ntile = std::min(mt, ncol - c0);
-    a.der = der ? 1 : 0;
+    a.der = (der && physics) ? 1 : 0;
     a.atm = d_atm;
     a.micro = c->d_micro;
     a.sbuf = (double*)c->d_sbuf;
-    a.eos = (double*)c->d_eosb;
+    a.eos = physics ? (double*)c->d_eosb : nullptr;
     a.status = stv;
     const long nkt = (long)nz * a.ntile;
     hipLaunchKernelGGL(saha_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(rho_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
-    hipLaunchKernelGGL(opac_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(tau_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+    if (hydro) {
+      const long nmt = (long)(nz - 1) * a.ntile;
+      hipLaunchKernelGGL(saha_mid_kernel, dim3((unsigned)((nmt + 255) / 256)), dim3(256), 0, st, a,
+                         (double*)c->d_smid);
+      hipLaunchKernelGGL(he_rk4_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a, (const double*)c->d_smid);
+    } else {
+      hipLaunchKernelGGL(rho_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+    }
+    if (physics) {
+      hipLaunchKernelGGL(opac_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(tau_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+    }
     HIPCHK(c, hipGetLastError());
   }
+  return FZ_OK;
+}
+
+int fz_set_hydrostatic(fz_ctx* c, int on) {
+  if (!c) return FZ_EARG;
+  c->hydro = on != 0;
   return FZ_OK;
 }
 
@@ -1029,7 +1104,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
-  r = run_micro(c, ncol, d_atm, &d_status, rf_mode != FZ_RF_NONE, st);
+  r = run_micro(c, ncol, d_atm, &d_status, rf_mode != FZ_RF_NONE, c->hydro, true, st);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
   c->rt_launches = 0;
@@ -1079,11 +1154,10 @@ int fz_last_timing(fz_ctx* c, float* micro_ms, float* rt_ms, int* launches) {
 int fz_hydrostatic_dev(fz_ctx* c, int ncol, float* d_atm, int32_t* d_status, void* stream) {
   if (!c || !c->configured || ncol < 1 || !d_atm) return FZ_EARG;
   HIPCHK(c, hipSetDevice(c->device));
-  int ru = upload_cfg(c);
-  if (ru) return ru;
-  hipLaunchKernelGGL(he_kernel, dim3((ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, ncol, d_atm, d_status);
-  HIPCHK(c, hipGetLastError());
-  return FZ_OK;
+  int r = upload_cfg(c);
+  if (r) return r;
+  r = run_micro(c, ncol, d_atm, &d_status, false, true, false, (hipStream_t)stream);
+  return r;
 }
 
 int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
@@ -1317,7 +1391,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * 7 * g.nz * 4 * g.numw);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
-  r = run_micro(c, ncol, d_atm, &d_status, true, st);
+  r = run_micro(c, ncol, d_atm, &d_status, true, c->hydro, true, st);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[1], st));
   c->rt_launches = 0;
