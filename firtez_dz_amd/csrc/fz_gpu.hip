@@ -63,6 +63,8 @@ struct Cfg {
   // and the two weights of the mid-step temperature, with the running height
   // t0 accumulated exactly as the reference does
   double he_dz[MAXNZ], he_a[MAXNZ], he_b[MAXNZ];
+  // m_u g dz / k per height (GET_HYD_DERIVATIVES, forward.f90:1033-1061)
+  float he_cte[MAXNZ];
 };
 __constant__ Cfg c_cfg;
 
@@ -486,7 +488,16 @@ __device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
   return ly;
 }
 
-template <int RF>
+// HE (HYDROSTATIC: yes, response functions on): the T response of height K
+// also carries the change of Pg below K through hydrostatic equilibrium,
+// dPg_i/dT_K = Pg_i (-mu_K m_u g dz_K / (k T_K^2)) (GET_HYD_DERIVATIVES), i.e.
+// g_K * W_K with W_K = sum_{i<K} O_K..O_{i+1} (local Pg|T response of i) Pg_i,
+// skipping layers with O_11 <= 1e-5 (rtesolver.f90:218-231).  W is built in
+// pass 1 with one extra directional derivative per layer, so the term costs
+// O(nz) instead of the reference's O(nz^2).  In that mode the reference
+// evaluates the rho, B and v responses with the layer's outgoing Stokes
+// vector (STKIN is reset to SYN1D after the hydrostatic block); reproduced.
+template <int RF, bool HE>
 __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ atm, const float* __restrict__ micro,
                                                      const float* __restrict__ gain, const int32_t* __restrict__ status,
                                                      float* __restrict__ stokes, float* __restrict__ jac,
@@ -526,23 +537,46 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   const float c1 = (float)(2.0 * HPLA * LIGHT * LIGHT / (lam * lam * lam * lam * lam));
   const float c2 = (float)(HPLA * LIGHT / (lam * KBOL));
   const long ps = (long)nz * a.ncol, cs = a.ncol;
+  constexpr bool HEX = HE && RF != 0;
   float I[4] = {0.f, 0.f, 0.f, 0.f};
+  float Ahe[4] = {0.f, 0.f, 0.f, 0.f};
   float* scr = a.scratch + t;
   const long sstr = nthr;
   for (int k = 0; k < nz; ++k) {
     const Layer ly = load_layer(a, col, k);
     const float S = c1 / (expf(c2 / ly.T) - 1.0f);
+    if (RF) {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) scr[(long)(k * 4 + s) * sstr] = I[s];
+    }
     if (a.micro[F_FLAG * ps + k * cs + col] != 0.f) {
       I[0] = S; I[1] = 0.f; I[2] = 0.f; I[3] = 0.f;
       continue;
     }
     float x7[7];
-    absorb(a, col, k, r, wave, ly, x7);
     Evol E;
-    evol_build(x7, c_cfg.dz_cm[k], E);
-    if (RF) {
+    if constexpr (HEX) {
+      float dx[NDIR][7];
+      absorb_der(a, col, k, r, wave, ly, x7, dx);
+      const float dz = c_cfg.dz_cm[k];
+      evol_build(x7, dz, E);
+      float v[4] = {I[0] - S, I[1], I[2], I[3]}, m4v[4], d2[4];
+      m4mul(E.e, E.r, E.s0, E.c, v, m4v);
+      evol_dir(E, dx[D_P], dz, v, m4v, d2);
+      float W[4];
+      const bool thin = E.O[0][0] > 1.0e-5f;
 #pragma unroll
-      for (int s = 0; s < 4; ++s) scr[(long)(k * 4 + s) * sstr] = I[s];
+      for (int i = 0; i < 4; ++i)
+        W[i] = thin ? E.O[i][0] * Ahe[0] + E.O[i][1] * Ahe[1] + E.O[i][2] * Ahe[2] + E.O[i][3] * Ahe[3] : Ahe[i];
+      const float pg = a.atm[A_PG * ps + k * cs + col];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        scr[(long)((nz + k) * 4 + i) * sstr] = W[i];
+        if (thin) Ahe[i] = W[i] + d2[i] * pg;
+      }
+    } else {
+      absorb(a, col, k, r, wave, ly, x7);
+      evol_build(x7, c_cfg.dz_cm[k], E);
     }
     float In[4];
 #pragma unroll
@@ -591,17 +625,30 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
         for (int s = 0; s < 4; ++s) v[s] = scr[(long)(k * 4 + s) * sstr] - (s == 0 ? S : 0.f);
         m4mul(E.e, E.r, E.s0, E.c, v, m4v);
+        float vo[4], m4vo[4];
+        if constexpr (HEX) {
+#pragma unroll
+          for (int s = 0; s < 4; ++s) vo[s] = (k == nz - 1 ? I[s] : scr[(long)((k + 1) * 4 + s) * sstr]) - (s == 0 ? S : 0.f);
+          m4mul(E.e, E.r, E.s0, E.c, vo, m4vo);
+        }
         float loc[NDIR][4];
         constexpr int dirs_sph[5] = {D_T, D_B, D_G, D_F, D_V};
         constexpr int dirs_car[7] = {D_T, D_P, D_R, D_B, D_G, D_F, D_V};
 #pragma unroll
         for (int q = 0; q < NP; ++q) {
           const int dd = RF == FZ_RF_SPHERICAL ? dirs_sph[q] : dirs_car[q];
-          evol_dir(E, dx[dd], dz, v, m4v, loc[dd]);
+          if (HEX && dd != D_T && dd != D_P) evol_dir(E, dx[dd], dz, vo, m4vo, loc[dd]);
+          else evol_dir(E, dx[dd], dz, v, m4v, loc[dd]);
         }
         // source-function term (1 - O) e0 dS/dT for T
 #pragma unroll
         for (int i = 0; i < 4; ++i) loc[D_T][i] += ((i == 0 ? 1.f : 0.f) - E.O[i][0]) * dS;
+        if constexpr (HEX) {
+          const float mu = a.atm[A_MU * ps + k * cs + col];
+          const float gk = -mu * c_cfg.he_cte[k] / ly.T / ly.T;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) loc[D_T][i] += gk * scr[(long)((nz + k) * 4 + i) * sstr];
+        }
         float lp[NP][4];
         if constexpr (RF == FZ_RF_SPHERICAL) {
 #pragma unroll
@@ -953,6 +1000,8 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
       g.he_b[k] = it1 - t0;
       t0 = t0 + dz;
     }
+    g.he_cte[0] = 0.f;
+    for (int k = 1; k < nz; ++k) g.he_cte[k] = (float)(MAMU / KBOL * GRAV * (g.zz[k] - g.zz[k - 1]) * 1.0e3 * 1.0e2);
   }
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &g, sizeof(Cfg)));
@@ -1011,9 +1060,13 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
 static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
   const dim3 grid((unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
 #define FZ_RT_ARGS a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0
-  if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL(rt_kernel<FZ_RF_NONE>, grid, blk, 0, st, FZ_RT_ARGS);
-  else if (rf_mode == FZ_RF_SPHERICAL) hipLaunchKernelGGL(rt_kernel<FZ_RF_SPHERICAL>, grid, blk, 0, st, FZ_RT_ARGS);
-  else hipLaunchKernelGGL(rt_kernel<FZ_RF_CARTESIAN>, grid, blk, 0, st, FZ_RT_ARGS);
+  if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (!c->hydro && rf_mode == FZ_RF_SPHERICAL)
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (!c->hydro) hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (rf_mode == FZ_RF_SPHERICAL)
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, true>), grid, blk, 0, st, FZ_RT_ARGS);
+  else hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, true>), grid, blk, 0, st, FZ_RT_ARGS);
 #undef FZ_RT_ARGS
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
@@ -1100,7 +1153,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
   if (r) return r;
   const int tile = std::min(c->tile, ncol);
   if (rf_mode != FZ_RF_NONE) {
-    r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * 4 * tile * g.numw);
+    r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * tile * g.numw);
     if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
@@ -1386,7 +1439,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
   if (r) return r;
   const int tile = std::min(c->tile, ncol);
-  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * 4 * tile * g.numw);
+  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * tile * g.numw);
   if (r) return r;
   r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * 7 * g.nz * 4 * g.numw);
   if (r) return r;

@@ -86,6 +86,7 @@ def library():
     L.fz_hydrostatic_dev.argtypes = [vp, ip, vp, vp, vp]
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
+    L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_synchronize.argtypes = [vp]
     L.fz_set_profiling.argtypes = [vp, ip]
     L.fz_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
@@ -106,6 +107,7 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
+            "fz_set_hydrostatic",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev"]
 
 
@@ -208,6 +210,11 @@ class Context:
     def set_gain(self, gain):
         g = np.ascontiguousarray(np.asarray(gain, np.float32).T)
         self._check(self.lib.fz_set_gain(self.h, _f(g)), "fz_set_gain")
+
+    def set_hydrostatic(self, on: bool = True):
+        """HYDROSTATIC: yes -- Pg re-integrated from the top inside every
+        forward, and the hydrostatic term in dI/dT."""
+        self._check(self.lib.fz_set_hydrostatic(self.h, int(on)), "fz_set_hydrostatic")
 
     def set_tile(self, columns: int, block: int = 256):
         self._check(self.lib.fz_set_tile(self.h, columns, block), "fz_set_tile")

@@ -115,6 +115,13 @@ int fz_synchronize(fz_ctx* ctx);
 int fz_set_profiling(fz_ctx* ctx, int on);
 int fz_last_timing(fz_ctx* ctx, float* micro_ms, float* rt_ms, int* rt_launches);
 
+/* HYDROSTATIC: yes / no (input_file.f90:516-520).  With it every forward
+ * re-integrates Pg from the top pressure of each column (RK4_INTEGRATION_LOG,
+ * forward.f90:891-906), takes N_H and n_e from that integration, and adds the
+ * hydrostatic term to dI/dT (rtesolver.f90:218-231, HYDROSTATICDER).  The P0
+ * response (forward.f90:543-545) is not produced. */
+int fz_set_hydrostatic(fz_ctx* ctx, int on);
+
 /* Columns processed per radiative-transfer launch (scratch is sized for this
  * tile); the autotuner sets it.  Default 8192. */
 int fz_set_tile(fz_ctx* ctx, int columns_per_launch, int block_threads);

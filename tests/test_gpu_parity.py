@@ -202,3 +202,68 @@ def test_forward3d_host_mirror(gpu_ctx, golden):
         worst = max(worst, np.max(np.abs(ours - g["stokes"][k])))
     report("FORWARD3D host mirror Stokes vs reference", worst, STOKES_TOL)
     assert dsyn.shape == (4 * numw, 7 * nz, ny, nx)
+
+
+def test_hydrostatic_mode_vs_reference(gpu_ctx, golden):
+    """HYDROSTATIC: yes against the reference's own output (tests/test_he,
+    test_syn2 set-up with a field and a flow): the re-integrated model, Stokes
+    and the seven Cartesian response functions, including the hydrostatic T
+    term and the outgoing-vector quirk of the rho, B and v responses."""
+    g = golden["he"]
+    ls = lines.setup([((4, 5), int(g["nwave"]), float(g["wave_start"]), float(g["wave_step"]))])
+    gpu_ctx.configure(ls, g["model_in"][12].astype(np.float64))
+    gpu_ctx.set_gain(g["gain"])
+    gpu_ctx.set_hydrostatic(True)
+    try:
+        atm_out, st, jac, status = run_gpu(gpu_ctx, model_rows(g["model_in"][None]), 2)
+    finally:
+        gpu_ctx.set_hydrostatic(False)
+    assert status[0] == 0
+    for row, name in ((1, "Pg"), (2, "rho"), (3, "Pe"), (4, "mu")):
+        report("HE model %s (relative)" % name,
+               np.max(np.abs(atm_out[row, :, 0] - g["model_out"][row]) / np.abs(g["model_out"][row])), 1e-6)
+    report("HE Stokes vs reference", np.max(np.abs(st[0] - g["stokes"])), STOKES_TOL)
+    names = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz")
+    for p in range(7):
+        ours = jac[0, p][g["rf_heights"]]
+        ref = np.transpose(g["rf"][:, :, p, :], (0, 2, 1))
+        sc = np.abs(ref).max()
+        for s in range(4):
+            report("HE RF d/d%s Stokes %d" % (names[p], s), np.max(np.abs(ours[:, s] - ref[:, s])) / sc, RF_TOL)
+
+
+def test_hydrostatic_mode_seeded_vs_oracle(gpu_ctx, cpu):
+    """HYDROSTATIC: yes on seeded columns (2 lines, 112 wavelengths), Stokes
+    and Cartesian response functions against the oracle in the same mode."""
+    import torch
+    ls = lines.fe6301_6302()
+    atm, z = atmos.seeded(4, 4, 64, seed=9)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    gpu_ctx.set_hydrostatic(True)
+    try:
+        t = torch.from_numpy(atm.copy()).cuda()
+        st, jac, status = gpu_ctx.forward_torch(t, 2)
+        torch.cuda.synchronize()
+    finally:
+        gpu_ctx.set_hydrostatic(False)
+    st, jac, status, atm_h = st.cpu().numpy(), jac.cpu().numpy(), status.cpu().numpy(), t.cpu().numpy()
+    assert np.all(status == 0)
+    cpu.setup(ls, resp=True, atmpar=CART)
+    cpu.set_gain(gain)
+    cpu.set_hydrostatic(True)
+    try:
+        worst = 0.0
+        for c in (0, 5, 15):
+            m = atmos.column_model(atm, z, c)       # Pg only at the top, as the GPU got it
+            mo, syn, dsyn, _ = cpu.forward1d(m)
+            report("HE seeded col %d Pg vs oracle (relative)" % c,
+                   np.max(np.abs(atm_h[1, :, c] - mo[1]) / mo[1]), 1e-6)
+            worst = max(worst, np.max(np.abs(st[c] - syn)))
+            for p in range(7):
+                ref = np.transpose(dsyn[:, :, p, :], (0, 2, 1))
+                sc = np.abs(ref).max()
+                report("HE seeded col %d RF p%d" % (c, p), np.max(np.abs(jac[c, p] - ref)) / sc, RF_TOL)
+        report("HE seeded Stokes vs oracle", worst, STOKES_TOL)
+    finally:
+        cpu.set_hydrostatic(False)
