@@ -660,13 +660,18 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
             lp[4][i] = loc[D_V][i];
           }
         } else {
-          // chain rule (|B|, gamma, phi) -> (Bx, By, Bz)
+          // chain rule (|B|, gamma, phi) -> (Bx, By, Bz).  With B = 0 every
+          // field derivative is zero, as in the reference (absorption_matrix.f90:1842-1866);
+          // with a vertical field the azimuth terms are dropped (the reference's
+          // d cos2phi / dBx is 0/0 there).
           const float* at = a.atm + k * cs + col;
           const float bx = at[A_BX * ps], by = at[A_BY * ps], bz = at[A_BZ * ps];
           const float bt2 = bx * bx + by * by, bt = sqrtf(bt2), B = ly.g.B, B2 = B * B;
-          const float dB[3] = {bx / B, by / B, bz / B};
-          const float dG[3] = {bz * bx / (B2 * bt), bz * by / (B2 * bt), -bt / B2};
-          const float dF[3] = {-by / bt2, bx / bt2, 0.f};
+          const bool hasB = B > 0.f, hasT = bt2 > 0.f;
+          const float iB = hasB ? 1.f / B : 0.f, iBt = hasT ? 1.f / (B2 * bt) : 0.f, ibt2 = hasT ? 1.f / bt2 : 0.f;
+          const float dB[3] = {bx * iB, by * iB, bz * iB};
+          const float dG[3] = {bz * bx * iBt, bz * by * iBt, hasB ? -bt / B2 : 0.f};
+          const float dF[3] = {-by * ibt2, bx * ibt2, 0.f};
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             lp[0][i] = loc[D_T][i];
