@@ -9,6 +9,14 @@ each step runs the microphysics and radiative-transfer kernels and writes
 the 71 GB Jacobian to HBM.
 
 Prints one JSON line (rank 0).  Only N = 1 GPU is supported.
+
+--lm measures the inversion instead: LM-converged pixels per second of the
+whole INVERT3D schedule (fz_invert3d_dev) on a seeded cube (default 512x512,
+nz = 64, the same two lines and 112 wavelengths), observations = forward of
+the seeded truth + N(0, 1e-3 I_c) noise, T, Bx, By, Bz, v_z with at most 4
+nodes each (two cycles: T 2 / others 1 node, then 4), started from the
+plain initial model: the column-mean T(z) of the truth in hydrostatic
+equilibrium, B = (100, 100, 100) G, v = 0.
 """
 from __future__ import annotations
 
@@ -68,11 +76,15 @@ def main():
     ap.add_argument("--tile", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--lm", action="store_true", help="measure LM-converged pixels/s instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (columns are independent; "
               "see the sharding design).", file=sys.stderr)
         sys.exit(2)
+
+    if args.lm:
+        return bench_lm(args)
 
     import torch
     from firtez_dz_amd import atmos, flops, gpu, lines
@@ -139,6 +151,59 @@ def main():
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+def bench_lm(args):
+    import torch
+    from firtez_dz_amd import atmos, gpu, lines
+
+    nx = ny = 512 if args.nx == 288 else args.nx
+    nz = 64 if args.nz == 96 else args.nz
+    ncol = nx * ny
+    ls = lines.fe6301_6302(nwave=112, start=-624.0, step=20.0)
+    print("[bench-lm] seeding %dx%dx%d truth" % (nx, ny, nz), file=sys.stderr, flush=True)
+    atm, z = atmos.seeded(nx, ny, nz, dz_km=15.0, seed=7)
+    ctx = gpu.Context(0)
+    numw = ctx.configure(ls, z)
+    ctx.gain_hsra()
+    truth = torch.from_numpy(atm).cuda()
+    assert int(ctx.hydrostatic_torch(truth).max().item()) == 0
+    st, _, status = ctx.forward_torch(truth, gpu.RF_NONE)
+    torch.cuda.synchronize()
+    assert int(status.max().item()) == 0
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    obs = (st.double().reshape(ncol, 4 * numw) +
+           1e-3 * torch.randn((ncol, 4 * numw), generator=gen, device="cuda", dtype=torch.float64)).contiguous()
+    # plain initial model: column-mean T(z), hydrostatic Pg from the mean top pressure
+    init = torch.zeros_like(truth)
+    init[0] = truth[0].mean(dim=1, keepdim=True)
+    init[1, -1] = truth[1, -1].mean()
+    init[5] = 100.0
+    init[6] = 100.0
+    init[7] = 100.0
+    assert int(ctx.hydrostatic_torch(init).max().item()) == 0
+    torch.cuda.synchronize()
+    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3)
+    print("[bench-lm] inverting %d pixels" % ncol, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    syn, chi2, its = ctx.invert3d_torch(init, obs, ctl, nx, ny)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    chi = chi2.cpu().numpy()
+    err_t = float((init[0] - truth[0]).abs().mean().item())
+    err_v = float((init[10] - truth[10]).abs().mean().item())
+    line = {"metric": "LM-converged pixels/s (plain init)", "value": ncol / dt, "unit": "pixel/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer",
+            "data": "synthetic (seeded truth + 1e-3 noise)",
+            "config": {"workload": "INVERT3D %dx%d pixels, nz=%d, Fe I 6301.5+6302.5, %d wavelengths, "
+                                   "T/Bx/By/Bz/vz up to 4 nodes (2 cycles)" % (nx, ny, nz, numw),
+                       "lm_iterations": its, "median_chi2": float(np.median(chi)),
+                       "p90_chi2": float(np.percentile(chi, 90)),
+                       "mean_abs_T_error_K": err_t, "mean_abs_vz_error_cm_s": err_v}}
     print(json.dumps(line), flush=True)
 
 
