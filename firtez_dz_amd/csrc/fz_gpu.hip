@@ -1684,4 +1684,50 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   return FZ_OK;
 }
 
+int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
+                float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d, float* vz3d,
+                float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
+                int* iterations) {
+  if (!c || !c->configured || nx < 1 || ny < 1 || !obs3d || !ctl || !syn3d) return FZ_EARG;
+  const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
+  int nstk = 0;
+  for (int s = 0; s < 4; ++s) nstk += ctl->invstk[s] != 0;
+  const size_t nfreq = (size_t)nstk * numw;
+  float* cubes[NATM] = {tem3d, pg3d, rho3d, pel3d, mw3d, bx3d, by3d, bz3d, (float*)vx3d, (float*)vy3d, vz3d, tau3d5};
+  HIPCHK(c, hipSetDevice(c->device));
+  DevBuf b_atm, b_tmp, b_obs, b_st, b_chi;
+  const size_t cube = sizeof(float) * (size_t)nz * ncol;
+  HIPCHK(c, hipMalloc(&b_atm.p, cube * NATM));
+  HIPCHK(c, hipMalloc(&b_tmp.p, cube));
+  HIPCHK(c, hipMalloc(&b_obs.p, sizeof(double) * nfreq * ncol));
+  HIPCHK(c, hipMalloc(&b_st.p, sizeof(float) * 4 * numw * (size_t)ncol));
+  HIPCHK(c, hipMalloc(&b_chi.p, sizeof(double) * ncol));
+  float* d_atm = (float*)b_atm.p;
+  const unsigned tb = 256, tg = (unsigned)(((long)nz * ncol + tb - 1) / tb);
+  // the cubes (NZ,NY,NX) are column-major with column index y + ny*x, the
+  // same column order as the device layout [parameter][height][column]
+  for (int p = 0; p < NATM; ++p) {
+    HIPCHK(c, hipMemcpy(b_tmp.p, cubes[p], cube, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(cube_to_soa, dim3(tg), dim3(tb), 0, 0, (const float*)b_tmp.p, d_atm + (size_t)p * nz * ncol,
+                       nz, ncol);
+  }
+  // OBS3D(NFREQ,NY,NX) is already [column][NFREQ]
+  HIPCHK(c, hipMemcpy(b_obs.p, obs3d, sizeof(double) * nfreq * ncol, hipMemcpyHostToDevice));
+  int rc = fz_invert3d_dev(c, nx, ny, d_atm, (const double*)b_obs.p, ctl, (float*)b_st.p, (double*)b_chi.p,
+                           iterations, nullptr);
+  if (rc) return rc;
+  // the model (every parameter the inversion or the EOS changed) back to the cubes
+  const int outp[] = {A_T, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VZ, A_TAU};
+  for (int p : outp) {
+    hipLaunchKernelGGL(soa_to_cube, dim3(tg), dim3(tb), 0, 0, d_atm + (size_t)p * nz * ncol, (float*)b_tmp.p, nz,
+                       ncol);
+    HIPCHK(c, hipMemcpy(cubes[p], b_tmp.p, cube, hipMemcpyDeviceToHost));
+  }
+  std::vector<float> st((size_t)4 * numw * ncol);
+  HIPCHK(c, hipMemcpy(st.data(), b_st.p, sizeof(float) * st.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < (size_t)4 * numw * ncol; ++i) syn3d[i] = (double)st[i];
+  if (chi2) HIPCHK(c, hipMemcpy(chi2, b_chi.p, sizeof(double) * ncol, hipMemcpyDeviceToHost));
+  return FZ_OK;
+}
+
 }  // extern "C"

@@ -98,6 +98,8 @@ def library():
     L.fz_lm_step_dev.argtypes = [vp, ip] + [vp] * 12
     L.fz_invert3d_dev.argtypes = [vp, ip, ip, vp, vp, ctypes.POINTER(CInvertControl), vp, vp,
                                   ctypes.POINTER(ctypes.c_int), vp]
+    L.fz_invert3d.argtypes = [vp, ip, ip] + [fp] * 12 + [dp, ctypes.POINTER(CInvertControl), dp, dp,
+                                                       ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -108,7 +110,8 @@ def exported_symbols():
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
             "fz_set_hydrostatic",
-            "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev"]
+            "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
+            "fz_invert3d"]
 
 
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
@@ -340,6 +343,27 @@ class Context:
                     "fz_invert3d_dev")
         return stokes, chi2, its.value
 
+    def invert3d(self, cube: dict, obs3d, control: CInvertControl):
+        """INVERT3D on host cubes (invert3d.f90:57-178): cube as for forward3d
+        (updated in place with the inverted model), obs3d (NFREQ, ny, nx) float64
+        Fortran order.  Returns (SYN3D (4*numw, ny, nx), chi2 (ny, nx), iterations)."""
+        nz, ny, nx = cube["T"].shape
+        arrs = []
+        for name in ATM:
+            a = cube[name]
+            if not (a.dtype == np.float32 and a.flags.f_contiguous):
+                raise FzError("cube['%s'] must be float32 Fortran-contiguous" % name)
+            arrs.append(a)
+        obs = np.asfortranarray(obs3d, dtype=np.float64)
+        syn = np.zeros((4 * self.numw, ny, nx), np.float64, order="F")
+        chi = np.zeros((ny, nx), np.float64, order="F")
+        its = ctypes.c_int()
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.fz_invert3d(self.h, nx, ny, *[_f(a) for a in arrs], obs.ctypes.data_as(dp),
+                                         ctypes.byref(control), syn.ctypes.data_as(dp), chi.ctypes.data_as(dp),
+                                         ctypes.byref(its)), "fz_invert3d")
+        return syn, chi, its.value
+
     # ---------------------------------------------------------------- FORWARD3D mirror
     def forward3d(self, cube: dict, am_i_done=None, rf_mode=RF_NONE):
         """FORWARD3D on host cubes (forward.f90:43-306).
@@ -371,3 +395,4 @@ class Context:
                                    _f(dsyn) if dsyn is not None else None)
         self._check(rc, "fz_forward3d")
         return syn, dsyn
+

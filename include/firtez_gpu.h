@@ -217,6 +217,17 @@ typedef struct {
 int fz_invert3d_dev(fz_ctx* ctx, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
                     float* d_stokes, double* d_chi2, int* iterations, void* stream);
 
+/* INVERT3D on host arrays in the reference's layout (invert3d.f90:57-178
+ * with its module globals): the twelve cubes (NZ,NY,NX) float are the
+ * initial model on entry and the inverted model on return; OBS3D(NFREQ,NY,NX)
+ * double holds the observed profiles of the inverted Stokes parameters
+ * (Stokes-major blocks of NUMW); SYN3D(4*NUMW,NY,NX) double receives the
+ * synthesis of the inverted model, CHI2(NY,NX) (or NULL) its chi^2. */
+int fz_invert3d(fz_ctx* ctx, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
+                float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d, float* vz3d,
+                float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
+                int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

@@ -288,3 +288,41 @@ def test_invert3d_vs_oracle(gpu_ctx, cpu, case):
             scale = max(1.0, float(np.abs(atm[row]).max()))
             report("INVERT3D one-node model row %d vs oracle (relative to max)" % row,
                    float(np.max(np.abs(g_model[row] - o_model[row]))) / scale, 2e-3)
+
+
+def test_invert3d_host_arrays_match_device_path(gpu_ctx, cpu):
+    """fz_invert3d on Fortran-order host cubes (the INVERT3D drop-in) gives
+    exactly what the device-resident driver gives on the same data."""
+    import torch
+    ls = lines.fe6301_6302(nwave=48, start=-200.0, step=30.0)
+    nx, ny = 3, 2
+    atm, z, rng = _invert_problem(cpu, nx, ny, 31, ls)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    cpu.set_gain(gain)
+    ncol = nx * ny
+    obs = []
+    for c in range(ncol):
+        mo, syn, _, _ = cpu.forward1d(atmos.column_model(atm, z, c))
+        obs.append(syn.reshape(-1) + rng.normal(0, NOISE, syn.size))
+    obs = np.array(obs)
+    guess = atm.copy()
+    guess[0] += np.float32(120.0)
+    guess[10] += np.float32(4e4)
+    ctl = gpu.invert_control({"T": 2, "vz": 1, "Bz": 1}, noise=NOISE)
+    d_atm = torch.from_numpy(guess.copy()).cuda()
+    st_d, chi_d, its_d = gpu_ctx.invert3d_torch(d_atm, torch.from_numpy(obs.copy()).cuda(), ctl, nx, ny)
+    torch.cuda.synchronize()
+    nz = len(z)
+    cube = {}
+    for i, nm in enumerate(gpu.ATM):
+        cube[nm] = np.asfortranarray(guess[i].reshape(nz, nx, ny).transpose(0, 2, 1))   # (nz, ny, nx)
+    obs3d = np.asfortranarray(obs.reshape(nx, ny, -1).transpose(2, 1, 0))             # (nfreq, ny, nx)
+    syn3d, chi3d, its_h = gpu_ctx.invert3d(cube, obs3d, ctl)
+    assert its_h == its_d
+    back = np.stack([np.ascontiguousarray(cube[nm].transpose(0, 2, 1)).reshape(nz, ncol) for nm in gpu.ATM])
+    dev = d_atm.cpu().numpy()
+    for row in (0, 1, 2, 3, 4, 7, 10, 11):
+        assert np.array_equal(back[row], dev[row]), row
+    assert np.array_equal(chi3d.T.reshape(-1), chi_d.cpu().numpy())
+    assert np.array_equal(syn3d.transpose(2, 1, 0).reshape(ncol, -1), st_d.cpu().numpy().reshape(ncol, -1))
