@@ -65,6 +65,9 @@ struct Cfg {
   double he_dz[MAXNZ], he_a[MAXNZ], he_b[MAXNZ];
   // m_u g dz / k per height (GET_HYD_DERIVATIVES, forward.f90:1033-1061)
   float he_cte[MAXNZ];
+  // spectral LSF (MLSF): on/off and LSF_VALID per region
+  int lsf_on;
+  int reg_lsf[MAXR];
 };
 __constant__ Cfg c_cfg;
 
@@ -561,7 +564,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
       const float dz = c_cfg.dz_cm[k];
       evol_build(x7, dz, E);
       float v[4] = {I[0] - S, I[1], I[2], I[3]}, m4v[4], d2[4];
-      m4mul(E.e, E.r, E.s0, E.c, v, m4v);
+      m4mul(E.e, E.r, E.s0, v, m4v);
       evol_dir(E, dx[D_P], dz, v, m4v, d2);
       float W[4];
       const bool thin = E.O[0][0] > 1.0e-5f;
@@ -624,12 +627,12 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
         float v[4], m4v[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) v[s] = scr[(long)(k * 4 + s) * sstr] - (s == 0 ? S : 0.f);
-        m4mul(E.e, E.r, E.s0, E.c, v, m4v);
+        m4mul(E.e, E.r, E.s0, v, m4v);
         float vo[4], m4vo[4];
         if constexpr (HEX) {
 #pragma unroll
           for (int s = 0; s < 4; ++s) vo[s] = (k == nz - 1 ? I[s] : scr[(long)((k + 1) * 4 + s) * sstr]) - (s == 0 ? S : 0.f);
-          m4mul(E.e, E.r, E.s0, E.c, vo, m4vo);
+          m4mul(E.e, E.r, E.s0, vo, m4vo);
         }
         float loc[NDIR][4];
         constexpr int dirs_sph[5] = {D_T, D_B, D_G, D_F, D_V};
@@ -787,6 +790,48 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 #include "fz_lm.hpp"
 #include "fz_invert.hpp"
 
+namespace fz {
+// Spectral line-spread function (MLSF) on rows of numw floats (Stokes rows,
+// per-height or per-node response functions).  Per valid region of n pixels
+// the reference multiplies the r2c DFT of the row by the DFT of LSF_KERNEL
+// and transforms back with an unnormalised c2r (forward.f90:621-657,
+// 814-844), i.e. the circular convolution
+//     y[i] = sum_t tap[t] x[(i - t) mod n],  tap = n * LSF_KERNEL.
+// One wave per row: the row is staged in LDS (the update is in place), lane i
+// accumulates output i over the taps, the tap index is wave-uniform (scalar
+// loads) and the LDS reads of consecutive lanes are consecutive.  FP32
+// accumulation over n taps of a unit-sum kernel.  blockDim.x / 64 rows per
+// workgroup (up to 4, with numw * rows floats of LDS <= 64 KiB).
+__global__ void __launch_bounds__(256) lsf_kernel(float* __restrict__ rows, long nrows, const float* __restrict__ taps) {
+  extern __shared__ float xs[];
+  const int numw = c_cfg.numw;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long row = (long)blockIdx.x * (blockDim.x >> 6) + w;
+  float* x = xs + w * numw;
+  float* out = rows + row * numw;
+  if (row < nrows)
+    for (int i = lane; i < numw; i += 64) x[i] = out[i];
+  __syncthreads();
+  if (row >= nrows) return;
+  for (int r = 0; r < c_cfg.nreg; ++r) {
+    if (!c_cfg.reg_lsf[r]) continue;
+    const int p0 = c_cfg.reg_pix0[r], n = c_cfg.reg_nwave[r];
+    const float* xr = x + p0;
+    const float* tr = taps + p0;
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      const int i = i0 + lane;
+      float acc = 0.f;
+      int j = i < n ? i : 0;           // (i - t) mod n for t = 0; lanes past n idle in range
+      for (int t = 0; t < n; ++t) {
+        acc += tr[t] * xr[j];
+        j = j == 0 ? n - 1 : j - 1;
+      }
+      if (i < n) out[p0 + i] = acc;
+    }
+  }
+}
+}  // namespace fz
+
 // ==================================================================== C ABI
 using namespace fz;
 
@@ -826,7 +871,10 @@ struct fz_ctx {
   size_t smid_cap = 0;
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
+  float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
 };
+
+
 
 // the context whose configuration is in a device's constant memory
 static const fz_ctx* g_owner[64];
@@ -841,6 +889,18 @@ static int set_err(fz_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (x);                                                                 \
     if (e_ != hipSuccess) return set_err((c), FZ_EHIP, std::string(#x ": ") + hipGetErrorString(e_)); \
   } while (0)
+
+// rows of numw floats through the LSF (no-op when it is off)
+static int run_lsf(fz_ctx* c, float* d_rows, long nrows, hipStream_t st) {
+  if (!c->cfg.lsf_on || nrows <= 0) return FZ_OK;
+  const int rpb = std::max(1, std::min(4, 16384 / c->cfg.numw));
+  const long nb = (nrows + rpb - 1) / rpb;
+  if (nb > 0x7fffffffL) return set_err(c, FZ_EARG, "too many rows for the LSF launch");
+  hipLaunchKernelGGL(lsf_kernel, dim3((unsigned)nb), dim3(64 * rpb), sizeof(float) * rpb * c->cfg.numw, st, d_rows,
+                     nrows, (const float*)c->d_lsf);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
 
 static int upload_cfg(fz_ctx* c) {
   if (c->device < 64 && g_owner[c->device] == c) return FZ_OK;
@@ -884,6 +944,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_eosb) (void)hipFree(c->d_eosb);
   if (c->d_stat) (void)hipFree(c->d_stat);
   if (c->d_smid) (void)hipFree(c->d_smid);
+  if (c->d_lsf) (void)hipFree(c->d_lsf);
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -1028,6 +1089,31 @@ int fz_set_gain(fz_ctx* c, const float* gain) {
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipMemcpy(c->d_gain, gain, sizeof(float) * 4 * c->cfg.numw, hipMemcpyHostToDevice));
   c->have_gain = true;
+  return FZ_OK;
+}
+
+int fz_set_lsf(fz_ctx* c, const double* lsf_kernel, const int32_t* lsf_valid) {
+  if (!c || !c->configured) return FZ_EARG;
+  Cfg& g = c->cfg;
+  HIPCHK(c, hipSetDevice(c->device));
+  g.lsf_on = 0;
+  for (int r = 0; r < g.nreg; ++r) g.reg_lsf[r] = 0;
+  if (lsf_kernel) {
+    std::vector<float> taps(g.numw, 0.f);
+    for (int r = 0; r < g.nreg; ++r) {
+      g.reg_lsf[r] = lsf_valid ? (lsf_valid[r] != 0) : 1;
+      const int p0 = g.reg_pix0[r], n = g.reg_nwave[r];
+      for (int i = 0; i < n; ++i) {
+        if (!std::isfinite(lsf_kernel[p0 + i])) return set_err(c, FZ_EARG, "non-finite LSF kernel");
+        taps[p0 + i] = (float)(lsf_kernel[p0 + i] * (double)n);
+      }
+    }
+    if (!c->d_lsf) HIPCHK(c, hipMalloc(&c->d_lsf, sizeof(float) * RT_WAVE * MAXBLK));
+    HIPCHK(c, hipMemcpy(c->d_lsf, taps.data(), sizeof(float) * g.numw, hipMemcpyHostToDevice));
+    g.lsf_on = 1;
+  }
+  HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &g, sizeof(Cfg)));
+  if (c->device < 64) g_owner[c->device] = c;
   return FZ_OK;
 }
 
@@ -1183,6 +1269,12 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     if (r) return r;
     HIPCHK(c, hipGetLastError());
     c->rt_launches++;
+  }
+  r = run_lsf(c, d_stokes, (long)ncol * 4, st);
+  if (r) return r;
+  if (rf_mode != FZ_RF_NONE) {
+    r = run_lsf(c, d_jac, (long)ncol * (rf_mode == FZ_RF_SPHERICAL ? 5 : 7) * g.nz * 4, st);
+    if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
   return FZ_OK;
@@ -1490,6 +1582,12 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     HIPCHK(c, hipGetLastError());
     c->rt_launches++;
   }
+  // the LSF acts on the node-contracted response functions, after GAIN1D
+  // (forward.f90:608-657), and on the Stokes rows (:810-844)
+  r = run_lsf(c, d_stokes, (long)ncol * 4, st);
+  if (r) return r;
+  r = run_lsf(c, d_eqv, (long)ncol * e.np * e.nstk, st);
+  if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
   return FZ_OK;
 }

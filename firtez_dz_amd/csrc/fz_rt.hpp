@@ -86,7 +86,8 @@ struct Evol {
   v3 e, r;
   float ei, th, d, er, l1, l2, sig;
   float A, C2, B3, D4, che, she, ce, se, s0;
-  v3 a2, w2, a3, w3, c;
+  // al = -(ga e + gb r) and om = -(gb e - ga r)
+  float ga, gb;
 };
 
 __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E) {
@@ -127,18 +128,20 @@ __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E)
   E.C2 = E.se * ith;
   E.B3 = E.she * ith;
   E.D4 = (E.che - E.ce) * ith;
+  // With a2 = l2 e - s l1 r, a3 = l1 e + s l2 r, w2 = -(s l1 e + l2 r) and
+  // w3 = s l2 e - l1 r, the vectors al = -(C2 a2 + B3 a3) and
+  // om = -(C2 w2 + B3 w3) take two scalars: ga = C2 l2 + B3 l1 and
+  // gb = s (B3 l2 - C2 l1).
   const float s = E.sig;
-  E.a2 = l2 * E.e - (s * l1) * E.r;
-  E.w2 = -((s * l1) * E.e + l2 * E.r);
-  E.a3 = l1 * E.e + (s * l2) * E.r;
-  E.w3 = (s * l2) * E.e - l1 * E.r;
-  const v3 al = -(E.C2 * E.a2 + E.B3 * E.a3);
-  const v3 om = -(E.C2 * E.w2 + E.B3 * E.w3);
+  E.ga = E.C2 * l2 + E.B3 * l1;
+  E.gb = s * (E.B3 * l2 - E.C2 * l1);
+  const v3 al = -(E.ga * E.e + E.gb * E.r);
+  const v3 om = -(E.gb * E.e - E.ga * E.r);
   E.s0 = 0.5f * (e2 + r2);
-  E.c = cross(E.e, E.r);
+  const v3 c = cross(E.e, E.r);
   const float D4 = E.D4;
   const float ev[3] = {E.e.x, E.e.y, E.e.z}, rv[3] = {E.r.x, E.r.y, E.r.z};
-  const float alv[3] = {al.x, al.y, al.z}, cv[3] = {E.c.x, E.c.y, E.c.z};
+  const float alv[3] = {al.x, al.y, al.z}, cv[3] = {c.x, c.y, c.z};
   E.O[0][0] = E.A + D4 * E.s0;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -154,8 +157,9 @@ __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E)
   E.O[3][2] = -om.x + D4 * (ev[1] * ev[2] + rv[1] * rv[2]);
 }
 
-// M4raw v and its pieces
-__device__ __forceinline__ void m4mul(v3 e, v3 r, float s0, v3 c, const float* v, float* o) {
+// M4raw v and its pieces (c = e x r)
+__device__ __forceinline__ void m4mul(v3 e, v3 r, float s0, const float* v, float* o) {
+  const v3 c = cross(e, r);
   const v3 vp = mk(v[1], v[2], v[3]);
   const float ev = dot(e, vp), rv = dot(r, vp);
   o[0] = s0 * v[0] + dot(c, vp);
@@ -191,13 +195,11 @@ __device__ __forceinline__ void evol_dir(const Evol& E, const float dx7[7], floa
   const float dC2 = -dei * E.C2 + (E.ce * dl2 - E.C2 * dth) * ith;
   const float dB3 = -dei * E.B3 + (E.che * dl1 - E.B3 * dth) * ith;
   const float dD4 = -dei * E.D4 + (E.she * dl1 + E.se * dl2 - E.D4 * dth) * ith;
-  const float s = E.sig;
-  const v3 da2 = dl2 * E.e + E.l2 * de - s * (dl1 * E.r + E.l1 * dr);
-  const v3 dw2 = -(s * (dl1 * E.e + E.l1 * de) + dl2 * E.r + E.l2 * dr);
-  const v3 da3 = dl1 * E.e + E.l1 * de + s * (dl2 * E.r + E.l2 * dr);
-  const v3 dw3 = s * (dl2 * E.e + E.l2 * de) - (dl1 * E.r + E.l1 * dr);
-  const v3 dal = -(dC2 * E.a2 + E.C2 * da2 + dB3 * E.a3 + E.B3 * da3);
-  const v3 dom = -(dC2 * E.w2 + E.C2 * dw2 + dB3 * E.w3 + E.B3 * dw3);
+  // d(al) = -(dga e + ga de + dgb r + gb dr), d(om) = -(dgb e + gb de - dga r - ga dr)
+  const float dga = dC2 * E.l2 + E.C2 * dl2 + dB3 * E.l1 + E.B3 * dl1;
+  const float dgb = E.sig * (dB3 * E.l2 + E.B3 * dl2 - dC2 * E.l1 - E.C2 * dl1);
+  const v3 dal = -(dga * E.e + E.ga * de + dgb * E.r + E.gb * dr);
+  const v3 dom = -(dgb * E.e + E.gb * de - dga * E.r - E.ga * dr);
   float p[4];
   pmul(dal, dom, v, p);
   // d(M4raw) v

@@ -87,6 +87,7 @@ def library():
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
+    L.fz_set_lsf.argtypes = [vp, vp, vp]
     L.fz_synchronize.argtypes = [vp]
     L.fz_set_profiling.argtypes = [vp, ip]
     L.fz_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
@@ -109,7 +110,7 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
-            "fz_set_hydrostatic",
+            "fz_set_hydrostatic", "fz_set_lsf",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d"]
 
@@ -218,6 +219,19 @@ class Context:
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside every
         forward, and the hydrostatic term in dI/dT."""
         self._check(self.lib.fz_set_hydrostatic(self.h, int(on)), "fz_set_hydrostatic")
+
+    def set_lsf(self, lsf_kernel=None, valid=None):
+        """MLSF: LSF_KERNEL(NUMW) and LSF_VALID(NUML) as LSF_INIT leaves them
+        (lines.lsf_kernel builds both); None switches the convolution off."""
+        if lsf_kernel is None:
+            self._check(self.lib.fz_set_lsf(self.h, None, None), "fz_set_lsf")
+            return
+        k = np.ascontiguousarray(lsf_kernel, dtype=np.float64)
+        if k.size != self.numw:
+            raise ValueError("LSF kernel has %d entries, NUMW is %d" % (k.size, self.numw))
+        v = None if valid is None else np.ascontiguousarray(valid, dtype=np.int32)
+        self._check(self.lib.fz_set_lsf(self.h, k.ctypes.data, None if v is None else v.ctypes.data), "fz_set_lsf")
+        self._lsf = (k, v)
 
     def set_tile(self, columns: int, block: int = 256):
         self._check(self.lib.fz_set_tile(self.h, columns, block), "fz_set_tile")

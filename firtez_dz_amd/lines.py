@@ -133,3 +133,44 @@ def c_arrays(ls: LineSetup):
         regs[i].wave_start = r.wave_start
         regs[i].wave_step = r.wave_step
     return lines, regs
+
+
+def lsf_kernel(ls: LineSetup, sigma: Sequence[float], w0: Sequence[float], taps: dict | None = None):
+    """LSF_KERNEL(NUMW) and LSF_VALID(NUML) as LSF_INIT builds them
+    (pre_post_duties.f90:770-898).
+
+    Region r with sigma[r] != -1 and w0[r] != -1 gets a Gaussian of that
+    width and centre (mA) on its own pixel grid, centred on the region's middle
+    pixel (half a step to the red for an even count, :838-843), normalised to
+    unit sum and circularly shifted by SKI/2 so that tap 0 is the centre
+    (:844-846).  Otherwise taps[r] (an LSF file's samples, modelio.read_lsf)
+    is used when given.  A region is valid when its taps sum to 1 within 0.01
+    (:857); the stored kernel is the normalised taps divided by SKI (:862-864),
+    the form whose unnormalised inverse DFT the forward pass applies."""
+    taps = taps or {}
+    out = np.zeros(ls.numw)
+    valid = np.zeros(len(ls.regions), np.int32)
+    w = ls.wave()
+    p0 = 0
+    for r, reg in enumerate(ls.regions):
+        n = reg.nwave
+        if sigma[r] != -1 and w0[r] != -1:
+            xk = np.arange(1, n + 1, dtype=np.float64)
+            xk = xk - xk.sum() / n
+            xk = xk * (w[p0 + 1] - w[p0])
+            woff = (xk[1] - xk[0]) / 2.0 if n % 2 == 0 else 0.0
+            yk = np.exp(-((xk - w0[r] - woff) ** 2 / (2.0 * sigma[r] ** 2)))
+            yk = yk / yk.sum()
+            yk = np.roll(yk, -(n // 2))          # CSHIFT(YK, SHIFT=SKI/2)
+        elif r in taps:
+            yk = np.asarray(taps[r], np.float64)
+            if yk.size != n:
+                raise ValueError("LSF for region %d has %d samples, the region %d" % (r, yk.size, n))
+        else:
+            yk = np.zeros(n)
+        if abs(yk.sum() - 1.0) < 0.01:
+            valid[r] = 1
+            yk = yk / yk.sum()
+            out[p0:p0 + n] = yk / n
+        p0 += n
+    return out, valid

@@ -73,6 +73,16 @@ int fz_gain_hsra(fz_ctx* ctx, float* gain);
 /* Use a given GAIN1D instead (e.g. the reference's). */
 int fz_set_gain(fz_ctx* ctx, const float* gain);
 
+/* Spectral line-spread function (MLSF).  lsf_kernel: LSF_KERNEL(NUMW) and
+ * lsf_valid: LSF_VALID(NUML) (as int32) exactly as LSF_INIT leaves them
+ * (pre_post_duties.f90:770-898): per region the unit-sum taps divided by the
+ * region's pixel count, tap 0 (the centre) first.  Every later forward then
+ * convolves the Stokes rows and the response functions of each valid region
+ * circularly with the taps, after GAIN1D, like the FFT products of
+ * forward.f90:621-657 and 814-844.  lsf_kernel NULL switches it off;
+ * fz_configure resets it. */
+int fz_set_lsf(fz_ctx* ctx, const double* lsf_kernel, const int32_t* lsf_valid);
+
 /* FORWARD3D on device-resident data (forward.f90:43-306, per column the
  * FORWARD1D of forward.f90:857-1029 with HYDROSTATIC=no).
  *   d_atm    [12][nz][ncol] float, in/out: T, Pg, rho, Pe, mu, Bx, By, Bz, vx,

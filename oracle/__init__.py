@@ -98,6 +98,7 @@ class Cpu:
         L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
         L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
+        L.fzc_set_lsf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
         L.fzc_int_coefs.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.fzc_njevals.argtypes = [ctypes.c_int] * 6
@@ -115,6 +116,7 @@ class Cpu:
         if n < 0:
             raise RuntimeError("fzc_setup failed %d" % n)
         self.numw, self.resp, self.atmpar = n, bool(resp), tuple(atmpar)
+        self.nreg = len(ls.regions)
         return n
 
     def partition(self, elem, T):
@@ -188,6 +190,20 @@ class Cpu:
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside forward1d
         and the hydrostatic term in dI/dT."""
         self.lib.fzc_set_hydrostatic(int(on))
+
+    def set_lsf(self, lsf_kernel=None, valid=None):
+        """MLSF: LSF_KERNEL(NUMW) (region taps YK/SKI, tap 0 first) and
+        LSF_VALID(NUML); None switches the convolution off."""
+        if lsf_kernel is None:
+            self._lsf = None
+            return self.lib.fzc_set_lsf(None, None)
+        k = np.ascontiguousarray(lsf_kernel, dtype=np.float64)
+        v = np.ascontiguousarray(np.ones(self.nreg, np.int32) if valid is None else valid, dtype=np.int32)
+        self._lsf = (k, v)
+        r = self.lib.fzc_set_lsf(k.ctypes.data, v.ctypes.data)
+        if r:
+            raise ValueError("fzc_set_lsf: region longer than 4096 points")
+        return r
 
     def forward1d(self, model):
         """model: (13, nz) float32 [T,Pg,rho,Pe,mu,Bx,By,Bz,vx,vy,vz,tau5,z];

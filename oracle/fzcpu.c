@@ -43,6 +43,11 @@ static int ATMPAR[8];
 static int PIX0[FZC_MAXREG], PIX1[FZC_MAXREG];
 static double *WAVE;
 static float *GAIN;
+/* spectral LSF (MLSF): LSF_KERNEL(NUMW) and LSF_VALID(NUML) as LSF_INIT leaves
+ * them (pre_post_duties.f90:770-898) */
+static int LSF_ON;
+static double *LSFK;
+static int LSFV[FZC_MAXREG];
 static double BLDIFF[FZC_MAXREG][FZC_MAXBLEND];
 /* Zeeman pattern per line (magsplit.f90:38-187), precomputed once */
 static int ZN_N[FZC_MAXLINES];
@@ -796,6 +801,7 @@ int fzc_setup(const fzc_line* lines, int nlines, const fzc_region* regs, int nre
   }
   free(WAVE);
   free(GAIN);
+  fzc_set_lsf(NULL, NULL);
   WAVE = (double*)malloc(sizeof(double) * NUMW);
   GAIN = (float*)malloc(sizeof(float) * 4 * NUMW);
   for (int r = 0; r < nreg; ++r) /* pre_post_duties.f90:592-621 */
@@ -812,6 +818,46 @@ int fzc_numw(void) { return NUMW; }
 void fzc_wave(double* w) { memcpy(w, WAVE, sizeof(double) * NUMW); }
 void fzc_set_gain(const float* g) { memcpy(GAIN, g, sizeof(float) * 4 * NUMW); }
 void fzc_set_hydrostatic(int on) { HYDRO = on ? 1 : 0; }
+
+/* lsf_kernel: NUMW doubles, region r's taps YK/SKI at PIX0[r]..PIX1[r]-1 with
+ * tap 0 first (LSF_INIT builds them so: pre_post_duties.f90:855-876); valid:
+ * NRG flags; NULL switches the convolution off. */
+int fzc_set_lsf(const double* lsf_kernel, const int32_t* valid) {
+  free(LSFK);
+  LSFK = NULL;
+  LSF_ON = 0;
+  if (!lsf_kernel) return 0;
+  for (int r = 0; r < NRG; ++r)
+    if (PIX1[r] - PIX0[r] > 4096) return -E_CONFIG;
+  LSFK = (double*)malloc(sizeof(double) * NUMW);
+  memcpy(LSFK, lsf_kernel, sizeof(double) * NUMW);
+  for (int r = 0; r < NRG; ++r) LSFV[r] = valid ? (valid[r] != 0) : 1;
+  LSF_ON = 1;
+  return 0;
+}
+
+/* One Stokes row x[NUMW] (stride st) through the LSF: per valid region the
+ * reference multiplies the r2c DFT of the row by the DFT of LSF_KERNEL and
+ * returns with an unnormalised c2r (forward.f90:621-657, 814-844), i.e. the
+ * circular convolution y[i] = SKI * sum_j x[j] LSF_KERNEL[(i - j) mod SKI];
+ * the result is stored through REAL() (single precision). */
+static void lsf_row(double* x, size_t st) {
+  double y[4096];
+  for (int r = 0; r < NRG; ++r) {
+    if (!LSFV[r]) continue;
+    const int w0 = PIX0[r], n = PIX1[r] - PIX0[r];
+    for (int i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int j = 0; j < n; ++j) {
+        int t = i - j;
+        if (t < 0) t += n;
+        acc += x[(size_t)(w0 + j) * st] * LSFK[w0 + t];
+      }
+      y[i] = acc * (double)n;
+    }
+    for (int i = 0; i < n; ++i) x[(size_t)(w0 + i) * st] = (double)(float)y[i];
+  }
+}
 
 /* ------------------------------------------------ unit entry points */
 int fzc_eos(float T, float P, double* ne, double* out, double* d) {
@@ -1506,9 +1552,15 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
               size_t i = (((size_t)k * NUMW + l) * 8 + p) * 4 + q;
               dsyn_out[i] = dsyn[i] * (double)GAIN[4 * l + q];
             }
+      if (LSF_ON)
+        for (int k = 0; k < nz; ++k)
+          for (int p = 0; p < 8; ++p)
+            for (int q = 0; q < 4; ++q) lsf_row(&dsyn_out[((size_t)k * NUMW * 8 + p) * 4 + q], 32);
     }
     for (int l = 0; l < NUMW; ++l)
       for (int q = 0; q < 4; ++q) syn_out[4 * l + q] = syn[4 * l + q] * (double)GAIN[4 * l + q];
+    if (LSF_ON)
+      for (int q = 0; q < 4; ++q) lsf_row(&syn_out[q], 4);
     /* log tau_5000, REAL(4) trapezoid (forward.f90:1000-1006) */
     TAU5[nz - 1] = 1e-9f;
     for (int k = nz - 2; k >= 0; --k) {

@@ -90,6 +90,36 @@ def test_eqvdsyn_vs_oracle(gpu_ctx, cpu, nodes):
     report("EQVDSYN (%d free parameters) vs oracle" % P["ifreep"], worst_j, RF_TOL)
 
 
+def test_eqvdsyn_lsf_vs_oracle(gpu_ctx, cpu):
+    """MLSF on in the inversion forward: the LSF acts on the node-contracted
+    response functions and the Stokes rows (forward.f90:608-657, 810-844)."""
+    import torch
+    P = problem(gpu_ctx, cpu, NODES_C2, nx=2, ny=2)
+    kern, valid = lines.lsf_kernel(P["ls"], [50.0], [0.0])
+    try:
+        gpu_ctx.set_lsf(kern, valid)
+        cpu.set_lsf(kern, valid)
+        t = torch.from_numpy(P["guess"].copy()).cuda()
+        st, eqv, status = gpu_ctx.forward_nodes_torch(t)
+        torch.cuda.synchronize()
+        st, eqv, status, atm = st.cpu().numpy(), eqv.cpu().numpy(), status.cpu().numpy(), t.cpu().numpy()
+        assert np.all(status == 0)
+        worst_s = worst_j = 0.0
+        for c in range(P["ncol"]):
+            _, syn, dsyn, _ = cpu.forward1d(atmos.column_model(atm, P["z"], c))
+            worst_s = max(worst_s, np.max(np.abs(st[c] - syn)))
+            ref = cpu.eqv_dsyn(P["cfg"], P["coefs"], dsyn)
+            for j in range(P["ifreep"]):
+                sc = np.abs(ref[:, j]).max()
+                if sc > 0:
+                    worst_j = max(worst_j, np.max(np.abs(eqv[c, j] - ref[:, j])) / sc)
+        report("LSF inversion forward Stokes vs oracle", worst_s, STOKES_TOL)
+        report("LSF EQVDSYN vs oracle", worst_j, RF_TOL)
+    finally:
+        gpu_ctx.set_lsf(None)
+        cpu.set_lsf(None)
+
+
 def _oracle_step(cpu, P, obs, syn, bsyn, sla, eqv, ebest, isig, cur, best):
     out = []
     for c in range(P["ncol"]):

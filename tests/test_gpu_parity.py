@@ -267,3 +267,43 @@ def test_hydrostatic_mode_seeded_vs_oracle(gpu_ctx, cpu):
         report("HE seeded Stokes vs oracle", worst, STOKES_TOL)
     finally:
         cpu.set_hydrostatic(False)
+
+
+def test_lsf_forward_vs_oracle(gpu_ctx, cpu):
+    """MLSF on: two regions (odd and even pixel counts, the second with an
+    off-centre Gaussian), Stokes and Cartesian response functions against the
+    oracle with the same LSF_KERNEL; then the second region switched off
+    (LSF_VALID = 0) must leave its pixels as without the LSF."""
+    import torch
+    ls = lines.setup([((4,), 61, -600.0, 20.0), ((5,), 52, -520.0, 20.0)])
+    atm, z = atmos.seeded(3, 3, 64, seed=13)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t).cpu().numpy() == 0)
+    atm = t.cpu().numpy()
+    kern, valid = lines.lsf_kernel(ls, [45.0, 35.0], [0.0, -15.0])
+    cpu.setup(ls, resp=True, atmpar=CART)
+    cpu.set_gain(gain)
+    try:
+        gpu_ctx.set_lsf(kern, valid)
+        cpu.set_lsf(kern, valid)
+        _, st, jac, status = run_gpu(gpu_ctx, atm, 2)
+        assert np.all(status == 0)
+        worst = 0.0
+        for c in (0, 4, 8):
+            _, syn, dsyn, _ = cpu.forward1d(atmos.column_model(atm, z, c))
+            worst = max(worst, np.max(np.abs(st[c] - syn)))
+            for p in range(7):
+                ref = np.transpose(dsyn[:, :, p, :], (0, 2, 1))
+                report("LSF col %d RF p%d" % (c, p), np.max(np.abs(jac[c, p] - ref)) / np.abs(ref).max(), RF_TOL)
+        report("LSF Stokes vs oracle", worst, STOKES_TOL)
+        gpu_ctx.set_lsf(kern, [1, 0])
+        _, st1, _, _ = run_gpu(gpu_ctx, atm, 0)
+        gpu_ctx.set_lsf(None)
+        _, st0, _, _ = run_gpu(gpu_ctx, atm, 0)
+        report("LSF_VALID=0 region unchanged", np.max(np.abs(st1[:, :, 61:] - st0[:, :, 61:])), 0.0)
+        assert np.max(np.abs(st1[:, 0, :61] - st0[:, 0, :61])) > 1e-3
+    finally:
+        gpu_ctx.set_lsf(None)
+        cpu.set_lsf(None)
