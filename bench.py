@@ -17,6 +17,13 @@ the seeded truth + N(0, 1e-3 I_c) noise, T, Bx, By, Bz, v_z with at most 4
 nodes each (two cycles: T 2 / others 1 node, then 4), started from the
 plain initial model: the column-mean T(z) of the truth in hydrostatic
 equilibrium, B = (100, 100, 100) G, v = 0.
+
+--lm --learned adds the learned initial model: a 448-512-512-512-20 BF16
+network trained on the device (pairs from the device forward on 8 seeded
+512x512 cubes of other seeds, 2M pairs), its proposal inverted with the same
+schedule, pixels above twice the chi^2 level of plain LM (calibrated on a
+held-out 64x64 cube) inverted again from the plain model; the value is the
+learned pipeline's pixels/s (network + LM + fallback), the plain run beside it.
 """
 from __future__ import annotations
 
@@ -77,6 +84,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--lm", action="store_true", help="measure LM-converged pixels/s instead")
+    ap.add_argument("--learned", action="store_true", help="with --lm: learned initial model vs plain")
+    ap.add_argument("--train-cubes", type=int, default=8)
+    ap.add_argument("--train-nx", type=int, default=512)
+    ap.add_argument("--epochs", type=int, default=6)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (columns are independent; "
@@ -154,7 +165,9 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_lm(args):
+def lm_problem(args):
+    """Seeded truth cube, its observations (forward + 1e-3 noise) and the
+    plain initial model, all on the device."""
     import torch
     from firtez_dz_amd import atmos, gpu, lines
 
@@ -176,7 +189,15 @@ def bench_lm(args):
     gen.manual_seed(11)
     obs = (st.double().reshape(ncol, 4 * numw) +
            1e-3 * torch.randn((ncol, 4 * numw), generator=gen, device="cuda", dtype=torch.float64)).contiguous()
-    # plain initial model: column-mean T(z), hydrostatic Pg from the mean top pressure
+    init = plain_init(ctx, truth)
+    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3)
+    return ctx, truth, obs, init, ctl, nx, ny, nz, numw
+
+
+def plain_init(ctx, truth):
+    """Plain initial model: column-mean T(z), hydrostatic Pg from the mean top
+    pressure, B = (100, 100, 100) G, v = 0."""
+    import torch
     init = torch.zeros_like(truth)
     init[0] = truth[0].mean(dim=1, keepdim=True)
     init[1, -1] = truth[1, -1].mean()
@@ -185,25 +206,93 @@ def bench_lm(args):
     init[7] = 100.0
     assert int(ctx.hydrostatic_torch(init).max().item()) == 0
     torch.cuda.synchronize()
-    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3)
-    print("[bench-lm] inverting %d pixels" % ncol, file=sys.stderr, flush=True)
+    return init
+
+
+def lm_stats(model, truth, chi):
+    return {"median_chi2": float(np.median(chi)), "p90_chi2": float(np.percentile(chi, 90)),
+            "mean_abs_T_error_K": float((model[0] - truth[0]).abs().mean().item()),
+            "mean_abs_vz_error_cm_s": float((model[10] - truth[10]).abs().mean().item())}
+
+
+def bench_lm(args):
+    import torch
+    from firtez_dz_amd import learned
+
+    ctx, truth, obs, init, ctl, nx, ny, nz, numw = lm_problem(args)
+    ncol = nx * ny
+    print("[bench-lm] inverting %d pixels (plain init)" % ncol, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    syn, chi2, its = ctx.invert3d_torch(init, obs, ctl, nx, ny)
+    model, _, chi2, rp = learned.invert(ctx, init, obs, ctl, nx, ny, None, learned=False)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    chi = chi2.cpu().numpy()
-    err_t = float((init[0] - truth[0]).abs().mean().item())
-    err_v = float((init[10] - truth[10]).abs().mean().item())
-    line = {"metric": "LM-converged pixels/s (plain init)", "value": ncol / dt, "unit": "pixel/s", "n_gpus": 1,
-            "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer",
-            "data": "synthetic (seeded truth + 1e-3 noise)",
-            "config": {"workload": "INVERT3D %dx%d pixels, nz=%d, Fe I 6301.5+6302.5, %d wavelengths, "
-                                   "T/Bx/By/Bz/vz up to 4 nodes (2 cycles)" % (nx, ny, nz, numw),
-                       "lm_iterations": its, "median_chi2": float(np.median(chi)),
-                       "p90_chi2": float(np.percentile(chi, 90)),
-                       "mean_abs_T_error_K": err_t, "mean_abs_vz_error_cm_s": err_v}}
+    plain = dict(lm_stats(model, truth, chi2.cpu().numpy()), lm_iterations=rp.iterations, seconds=dt,
+                 pixels_per_s=ncol / dt)
+    print("[bench-lm] plain: %.1f s, %d iterations, median chi2 %.3g" % (dt, rp.iterations, plain["median_chi2"]),
+          file=sys.stderr, flush=True)
+    workload = ("INVERT3D %dx%d pixels, nz=%d, Fe I 6301.5+6302.5, %d wavelengths, T/Bx/By/Bz/vz up to 4 nodes "
+                "(2 cycles)" % (nx, ny, nz, numw))
+    if not args.learned:
+        line = {"metric": "LM-converged pixels/s (plain init)", "value": ncol / dt, "unit": "pixel/s", "n_gpus": 1,
+                "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+                "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer",
+                "data": "synthetic (seeded truth + 1e-3 noise)",
+                "config": dict({"workload": workload}, **plain)}
+        print(json.dumps(line), flush=True)
+        return
+
+    # the network: trained on pairs from the device forward (cubes of other seeds)
+    from firtez_dz_amd import gpu
+    ctx.mlp_create(4 * numw, 512, 3, 4, seed=1)
+    tc = args.train_cubes
+    print("[bench-lm] training set: %d cubes of %dx%d" % (tc, args.train_nx, args.train_nx), file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    X, T = learned.training_set(ctx, tc, args.train_nx, args.train_nx, nz, seed0=1000, noise=1e-3,
+                                log=lambda m: print("[bench-lm] " + m, file=sys.stderr, flush=True))
+    t_data = time.perf_counter() - t0
+    rep = learned.train(ctx, X, T, epochs=args.epochs, batch=1024, lr=1e-3,
+                        log=lambda m: print("[bench-lm] " + m, file=sys.stderr, flush=True))
+    del X, T
+    # the fallback threshold: twice the chi^2 level plain LM reaches on a
+    # held-out 64x64 cube of the same statistics
+    from firtez_dz_amd import atmos
+    ca, _ = atmos.seeded(64, 64, nz, dz_km=15.0, seed=4242)
+    ctruth = torch.from_numpy(ca).cuda()
+    ctx.hydrostatic_torch(ctruth)
+    cst, _, _ = ctx.forward_torch(ctruth, gpu.RF_NONE)
+    g2 = torch.Generator(device="cuda")
+    g2.manual_seed(12)
+    cobs = (cst.double().reshape(64 * 64, -1) + 1e-3 * torch.randn(cst.reshape(64 * 64, -1).shape, generator=g2,
+                                                                   device="cuda", dtype=torch.float64)).contiguous()
+    _, _, cchi, _ = learned.invert(ctx, plain_init(ctx, ctruth), cobs, ctl, 64, 64, None, learned=False)
+    noise_chi2 = float(cchi.median())
+    threshold = 2.0 * noise_chi2
+    print("[bench-lm] network trained in %.1f s (data %.1f s); calibration chi2 %.3g -> threshold %.3g"
+          % (rep.seconds, t_data, noise_chi2, threshold), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model, _, chi2, rl = learned.invert(ctx, init, obs, ctl, nx, ny, threshold, learned=True)
+    torch.cuda.synchronize()
+    dtl = time.perf_counter() - t0
+    lrn = dict(lm_stats(model, truth, chi2.cpu().numpy()), lm_iterations=rl.iterations, seconds=dtl,
+               pixels_per_s=ncol / dtl, fallback_pixels=rl.fallback_pixels,
+               fallback_iterations=rl.fallback_iterations, seconds_network=rl.seconds_init,
+               seconds_lm=rl.seconds_lm, seconds_fallback=rl.seconds_fallback)
+    print("[bench-lm] learned: %.1f s, %d iterations, median chi2 %.3g, %d fallback pixels"
+          % (dtl, rl.iterations, lrn["median_chi2"], rl.fallback_pixels), file=sys.stderr, flush=True)
+    line = {"metric": "LM-converged pixels/s (learned init)", "value": ncol / dtl, "unit": "pixel/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": dtl * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer, bf16 network",
+            "data": "synthetic (seeded truth + 1e-3 noise; network trained on %d other seeded pairs)"
+                    % (tc * args.train_nx ** 2),
+            "config": {"workload": workload, "learned": lrn, "plain": plain,
+                       "speedup_vs_plain": plain["seconds"] / dtl,
+                       "network": {"shape": "%d-512-512-512-20" % (4 * numw), "pairs": tc * args.train_nx ** 2,
+                                   "epochs": args.epochs, "train_seconds": rep.seconds, "data_seconds": t_data,
+                                   "train_loss": rep.train_loss, "valid_loss": rep.valid_loss,
+                                   "fallback_threshold_chi2": threshold}}}
     print(json.dumps(line), flush=True)
 
 

@@ -789,6 +789,7 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 
 #include "fz_lm.hpp"
 #include "fz_invert.hpp"
+#include "fz_mlp.hpp"
 
 namespace fz {
 // Spectral line-spread function (MLSF) on rows of numw floats (Stokes rows,
@@ -835,6 +836,53 @@ __global__ void __launch_bounds__(256) lsf_kernel(float* __restrict__ rows, long
 // ==================================================================== C ABI
 using namespace fz;
 
+// Learned initial model: network shape, FP32 master parameters (flat:
+// W_0, b_0, W_1, b_1, ...; W_l row-major [out][in]), their BF16 copy, Adam
+// moments, gradient, normalisations and the per-batch work buffers.
+struct MlpState {
+  bool on = false;
+  int nin = 0, nh = 0, nl = 0, nj = 0, nout = 0, nz = 0;
+  long np = 0;
+  long woff[8] = {0}, boff[8] = {0};
+  int din[8] = {0}, dout[8] = {0};
+  float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  bf16* wb = nullptr;
+  float* norms = nullptr;   // xmean[nin], xstd[nin], tmean[nout], tstd[nout]
+  double* pinv = nullptr;   // [nj][nz] least-squares node fit
+  double* coefs = nullptr;  // [nz][nj] hat basis
+  long step = 0;
+  int cap = 0;              // rows the work buffers hold
+  bf16* X = nullptr;
+  bf16* act = nullptr;      // [nl][cap][nh]
+  float* Y = nullptr;       // [cap][nout]
+  float* dYf = nullptr;
+  bf16* dYb = nullptr;
+  float* dZf[2] = {nullptr, nullptr};
+  bf16* dZb[2] = {nullptr, nullptr};
+  float* partial = nullptr;
+  float* loss = nullptr;
+  void release() {
+    void* ps[] = {w, g, m1, m2, wb, norms, pinv, coefs, X, act, Y, dYf, dYb, dZf[0], dZf[1], dZb[0], dZb[1],
+                  partial, loss};
+    for (void* q : ps)
+      if (q) (void)hipFree(q);
+    *this = MlpState();
+  }
+  void release_work() {
+    void* ps[] = {X, act, Y, dYf, dYb, dZf[0], dZf[1], dZb[0], dZb[1], partial};
+    for (void* q : ps)
+      if (q) (void)hipFree(q);
+    X = nullptr;
+    act = nullptr;
+    Y = dYf = nullptr;
+    dYb = nullptr;
+    dZf[0] = dZf[1] = nullptr;
+    dZb[0] = dZb[1] = nullptr;
+    partial = nullptr;
+    cap = 0;
+  }
+};
+
 struct fz_ctx {
   int device = 0;
   bool configured = false;
@@ -872,6 +920,7 @@ struct fz_ctx {
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
+  MlpState mlp;
 };
 
 
@@ -945,6 +994,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_stat) (void)hipFree(c->d_stat);
   if (c->d_smid) (void)hipFree(c->d_smid);
   if (c->d_lsf) (void)hipFree(c->d_lsf);
+  c->mlp.release();
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
   delete c;
@@ -1825,6 +1875,387 @@ int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho
   HIPCHK(c, hipMemcpy(st.data(), b_st.p, sizeof(float) * st.size(), hipMemcpyDeviceToHost));
   for (size_t i = 0; i < (size_t)4 * numw * ncol; ++i) syn3d[i] = (double)st[i];
   if (chi2) HIPCHK(c, hipMemcpy(chi2, b_chi.p, sizeof(double) * ncol, hipMemcpyDeviceToHost));
+  return FZ_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- learned initial model
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+int mlp_work(fz_ctx* c, int n) {
+  MlpState& M = c->mlp;
+  if (M.cap >= n) return FZ_OK;
+  M.release_work();
+  const size_t r = (size_t)n;
+  const int nmax = std::max(M.nh, std::max(M.nin, M.nout));
+  bool ok = hipMalloc(&M.X, sizeof(bf16) * r * M.nin) == hipSuccess &&
+            hipMalloc(&M.act, sizeof(bf16) * r * M.nh * M.nl) == hipSuccess &&
+            hipMalloc(&M.Y, sizeof(float) * r * M.nout) == hipSuccess &&
+            hipMalloc(&M.dYf, sizeof(float) * r * M.nout) == hipSuccess &&
+            hipMalloc(&M.dYb, sizeof(bf16) * r * M.nout) == hipSuccess &&
+            hipMalloc(&M.partial, sizeof(float) * ((r * M.nout + 255) / 256 + 1)) == hipSuccess;
+  for (int i = 0; i < 2 && ok; ++i)
+    ok = hipMalloc(&M.dZf[i], sizeof(float) * r * nmax) == hipSuccess &&
+         hipMalloc(&M.dZb[i], sizeof(bf16) * r * nmax) == hipSuccess;
+  if (!ok) {
+    M.release_work();
+    return set_err(c, FZ_ENOMEM, "device allocation failed (network work buffers)");
+  }
+  M.cap = n;
+  return FZ_OK;
+}
+
+void mlp_gemm(const GemmArgs& g, hipStream_t st) {
+  const dim3 grid((unsigned)((g.N + MLP_TN - 1) / MLP_TN), (unsigned)((g.M + MLP_TM - 1) / MLP_TM));
+  hipLaunchKernelGGL(mlp_gemm_kernel, grid, dim3(256), 0, st, g);
+}
+
+GemmArgs gemm0() {
+  GemmArgs g;
+  memset(&g, 0, sizeof(g));
+  return g;
+}
+
+// forward pass of n rows already in M.X; the output (standardised) in M.Y
+void mlp_forward_rows(MlpState& M, int n, hipStream_t st) {
+  for (int l = 0; l <= M.nl; ++l) {
+    GemmArgs g = gemm0();
+    g.M = n;
+    g.N = M.dout[l];
+    g.K = M.din[l];
+    g.A = l == 0 ? M.X : M.act + (size_t)(l - 1) * M.cap * M.nh;
+    g.sam = M.din[l];
+    g.sak = 1;
+    g.B = M.wb + M.woff[l];
+    g.sbn = M.din[l];
+    g.sbk = 1;
+    g.bias = M.w + M.boff[l];
+    if (l < M.nl) {
+      g.relu = 1;
+      g.Cb = M.act + (size_t)l * M.cap * M.nh;
+      g.sbm2 = M.dout[l];
+      g.sbn2 = 1;
+    } else {
+      g.Cf = M.Y;
+      g.scm = M.nout;
+      g.scn = 1;
+    }
+    mlp_gemm(g, st);
+  }
+}
+
+// loss and gradient of the parameters into M.g for n rows (targets t, rows
+// gathered by idx when given); the mean squared error into M.loss[0]
+void mlp_backward_rows(MlpState& M, int n, const float* d_t, const int32_t* d_idx, hipStream_t st) {
+  const float* tmean = M.norms + 2 * M.nin;
+  const float* tstd = tmean + M.nout;
+  const long ne = (long)n * M.nout;
+  const unsigned nb = (unsigned)((ne + 255) / 256);
+  hipLaunchKernelGGL(mlp_loss_kernel, dim3(nb), dim3(256), 0, st, n, M.nout, (const float*)M.Y, d_t, d_idx, tmean,
+                     tstd, M.dYf, M.dYb, M.partial);
+  hipLaunchKernelGGL(mlp_reduce_kernel, dim3(1), dim3(256), 0, st, (int)nb, (const float*)M.partial, (float)ne, M.loss);
+  const float* cf = M.dYf;
+  const bf16* cb = M.dYb;
+  int buf = 0;
+  for (int l = M.nl; l >= 0; --l) {
+    const bf16* ain = l == 0 ? M.X : M.act + (size_t)(l - 1) * M.cap * M.nh;
+    const int di = M.din[l], dO = M.dout[l];
+    GemmArgs g = gemm0();   // dW_l[o][k] = sum_m dZ[m][o] A_in[m][k]
+    g.M = dO;
+    g.N = di;
+    g.K = n;
+    g.A = cb;
+    g.sam = 1;
+    g.sak = dO;
+    g.B = ain;
+    g.sbn = 1;
+    g.sbk = di;
+    g.Cf = M.g + M.woff[l];
+    g.scm = di;
+    g.scn = 1;
+    mlp_gemm(g, st);
+    hipLaunchKernelGGL(mlp_colsum_kernel, dim3((unsigned)((dO + 255) / 256)), dim3(256), 0, st, n, dO, cf,
+                       M.g + M.boff[l]);
+    if (l == 0) break;
+    GemmArgs h = gemm0();   // dA_in[m][k] = sum_o dZ[m][o] W_l[o][k], times ReLU'(A_in)
+    h.M = n;
+    h.N = di;
+    h.K = dO;
+    h.A = cb;
+    h.sam = dO;
+    h.sak = 1;
+    h.B = M.wb + M.woff[l];
+    h.sbn = 1;
+    h.sbk = di;
+    h.mask = ain;
+    h.smm = di;
+    h.smn = 1;
+    h.Cf = M.dZf[buf];
+    h.scm = di;
+    h.scn = 1;
+    h.Cb = M.dZb[buf];
+    h.sbm2 = di;
+    h.sbn2 = 1;
+    mlp_gemm(h, st);
+    cf = M.dZf[buf];
+    cb = M.dZb[buf];
+    buf ^= 1;
+  }
+}
+
+int mlp_input(fz_ctx* c, int n, const float* d_x, const double* d_xd, const int32_t* d_idx, hipStream_t st) {
+  MlpState& M = c->mlp;
+  const long ne = (long)n * M.nin;
+  hipLaunchKernelGGL(mlp_input_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, n, M.nin, d_x, d_xd,
+                     d_idx, (const float*)M.norms, (const float*)(M.norms + M.nin), M.X);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fz_mlp_create(fz_ctx* c, int nin, int nhidden, int nlayers, int nodes, uint64_t seed) {
+  if (!c || !c->configured || nin < 1 || nhidden < 1 || nlayers < 1 || nlayers > 6 || nodes < 1 ||
+      nodes > MLP_MAXNODE || nodes > c->cfg.nz)
+    return set_err(c, FZ_EARG, "network shape outside limits");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->mlp.release();
+  MlpState& M = c->mlp;
+  M.nin = nin;
+  M.nh = nhidden;
+  M.nl = nlayers;
+  M.nj = nodes;
+  M.nout = MLP_NQ * nodes;
+  M.nz = c->cfg.nz;
+  long off = 0;
+  for (int l = 0; l <= nlayers; ++l) {
+    M.din[l] = l == 0 ? nin : nhidden;
+    M.dout[l] = l == nlayers ? M.nout : nhidden;
+    M.woff[l] = off;
+    off += (long)M.din[l] * M.dout[l];
+    M.boff[l] = off;
+    off += M.dout[l];
+  }
+  M.np = off;
+  // He-uniform weights for ReLU layers (bound sqrt(6 / fan_in)), the output
+  // layer scaled by 1/4; zero biases
+  std::vector<float> w(off, 0.f);
+  uint64_t s = seed;
+  for (int l = 0; l <= nlayers; ++l) {
+    const double b = std::sqrt(6.0 / M.din[l]) * (l == nlayers ? 0.25 : 1.0);
+    for (long i = 0; i < (long)M.din[l] * M.dout[l]; ++i)
+      w[M.woff[l] + i] = (float)(b * (2.0 * ((splitmix64(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0));
+  }
+  bool ok = hipMalloc(&M.w, sizeof(float) * off) == hipSuccess && hipMalloc(&M.g, sizeof(float) * off) == hipSuccess &&
+            hipMalloc(&M.m1, sizeof(float) * off) == hipSuccess &&
+            hipMalloc(&M.m2, sizeof(float) * off) == hipSuccess && hipMalloc(&M.wb, sizeof(bf16) * off) == hipSuccess &&
+            hipMalloc(&M.norms, sizeof(float) * 2 * (nin + M.nout)) == hipSuccess &&
+            hipMalloc(&M.pinv, sizeof(double) * nodes * M.nz) == hipSuccess &&
+            hipMalloc(&M.coefs, sizeof(double) * nodes * M.nz) == hipSuccess &&
+            hipMalloc(&M.loss, sizeof(float) * 4) == hipSuccess;
+  if (!ok) {
+    M.release();
+    return set_err(c, FZ_ENOMEM, "device allocation failed (network)");
+  }
+  HIPCHK(c, hipMemcpy(M.w, w.data(), sizeof(float) * off, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemset(M.m1, 0, sizeof(float) * off));
+  HIPCHK(c, hipMemset(M.m2, 0, sizeof(float) * off));
+  HIPCHK(c, hipMemset(M.g, 0, sizeof(float) * off));
+  hipLaunchKernelGGL(mlp_to_bf16_kernel, dim3((unsigned)((off + 255) / 256)), dim3(256), 0, 0, off, (const float*)M.w,
+                     M.wb);
+  HIPCHK(c, hipGetLastError());
+  std::vector<float> nrm(2 * (nin + M.nout));
+  for (int i = 0; i < nin; ++i) {
+    nrm[i] = 0.f;
+    nrm[nin + i] = 1.f;
+  }
+  for (int i = 0; i < M.nout; ++i) {
+    nrm[2 * nin + i] = 0.f;
+    nrm[2 * nin + M.nout + i] = 1.f;
+  }
+  HIPCHK(c, hipMemcpy(M.norms, nrm.data(), sizeof(float) * nrm.size(), hipMemcpyHostToDevice));
+  // hat basis of GET_INT_COEFS and its least-squares pseudo-inverse
+  std::vector<double> cf((size_t)M.nz * nodes), pv((size_t)nodes * M.nz);
+  int r = fz_int_coefs(M.nz, nodes, cf.data());
+  if (r) return set_err(c, r, "node basis");
+  {
+    // normal equations (C^T C) P = C^T, solved by Gauss-Jordan on nodes x nodes
+    const int nj = nodes, nz = M.nz;
+    std::vector<double> a((size_t)nj * nj, 0.0), inv((size_t)nj * nj, 0.0);
+    for (int i = 0; i < nj; ++i)
+      for (int j = 0; j < nj; ++j)
+        for (int k = 0; k < nz; ++k) a[i * nj + j] += cf[(size_t)k * nj + i] * cf[(size_t)k * nj + j];
+    for (int i = 0; i < nj; ++i) inv[i * nj + i] = 1.0;
+    for (int p = 0; p < nj; ++p) {
+      int piv = p;
+      for (int i = p + 1; i < nj; ++i)
+        if (std::fabs(a[i * nj + p]) > std::fabs(a[piv * nj + p])) piv = i;
+      if (a[piv * nj + p] == 0.0) return set_err(c, FZ_ECONFIG, "singular node basis");
+      for (int j = 0; j < nj; ++j) {
+        std::swap(a[p * nj + j], a[piv * nj + j]);
+        std::swap(inv[p * nj + j], inv[piv * nj + j]);
+      }
+      const double d = a[p * nj + p];
+      for (int j = 0; j < nj; ++j) {
+        a[p * nj + j] /= d;
+        inv[p * nj + j] /= d;
+      }
+      for (int i = 0; i < nj; ++i) {
+        if (i == p) continue;
+        const double f = a[i * nj + p];
+        for (int j = 0; j < nj; ++j) {
+          a[i * nj + j] -= f * a[p * nj + j];
+          inv[i * nj + j] -= f * inv[p * nj + j];
+        }
+      }
+    }
+    for (int j = 0; j < nj; ++j)
+      for (int k = 0; k < nz; ++k) {
+        double v = 0.0;
+        for (int i = 0; i < nj; ++i) v += inv[j * nj + i] * cf[(size_t)k * nj + i];
+        pv[(size_t)j * nz + k] = v;
+      }
+  }
+  HIPCHK(c, hipMemcpy(M.coefs, cf.data(), sizeof(double) * cf.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(M.pinv, pv.data(), sizeof(double) * pv.size(), hipMemcpyHostToDevice));
+  M.on = true;
+  return FZ_OK;
+}
+
+int64_t fz_mlp_size(const fz_ctx* c) {
+  if (!c || !c->mlp.on) return -1;
+  return c->mlp.np + 2 * (c->mlp.nin + c->mlp.nout);
+}
+
+int fz_mlp_get_params(fz_ctx* c, float* params) {
+  if (!c || !c->mlp.on || !params) return FZ_EARG;
+  MlpState& M = c->mlp;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy(params, M.w, sizeof(float) * M.np, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(params + M.np, M.norms, sizeof(float) * 2 * (M.nin + M.nout), hipMemcpyDeviceToHost));
+  return FZ_OK;
+}
+
+int fz_mlp_set_params(fz_ctx* c, const float* params) {
+  if (!c || !c->mlp.on || !params) return FZ_EARG;
+  MlpState& M = c->mlp;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (long i = 0; i < M.np + 2 * (M.nin + M.nout); ++i)
+    if (!std::isfinite(params[i])) return set_err(c, FZ_EARG, "non-finite network parameter");
+  HIPCHK(c, hipMemcpy(M.w, params, sizeof(float) * M.np, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(M.norms, params + M.np, sizeof(float) * 2 * (M.nin + M.nout), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(mlp_to_bf16_kernel, dim3((unsigned)((M.np + 255) / 256)), dim3(256), 0, 0, M.np,
+                     (const float*)M.w, M.wb);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemset(M.m1, 0, sizeof(float) * M.np));
+  HIPCHK(c, hipMemset(M.m2, 0, sizeof(float) * M.np));
+  M.step = 0;
+  return FZ_OK;
+}
+
+int fz_mlp_fit_norms_dev(fz_ctx* c, int n, const float* d_x, const float* d_t, void* stream) {
+  if (!c || !c->mlp.on || n < 2 || !d_x || !d_t) return FZ_EARG;
+  MlpState& M = c->mlp;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipLaunchKernelGGL(mlp_stats_kernel, dim3((unsigned)((M.nin + 63) / 64)), dim3(64), 0, st, n, M.nin, d_x, M.norms,
+                     M.norms + M.nin);
+  hipLaunchKernelGGL(mlp_stats_kernel, dim3((unsigned)((M.nout + 63) / 64)), dim3(64), 0, st, n, M.nout, d_t,
+                     M.norms + 2 * M.nin, M.norms + 2 * M.nin + M.nout);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_mlp_targets_dev(fz_ctx* c, int ncol, const float* d_atm, float* d_t, void* stream) {
+  if (!c || !c->mlp.on || ncol < 1 || !d_atm || !d_t) return FZ_EARG;
+  if (c->cfg.nz != c->mlp.nz) return set_err(c, FZ_ECONFIG, "height grid changed since fz_mlp_create");
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipLaunchKernelGGL(mlp_targets_kernel, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, st, ncol, c->mlp.nz,
+                     c->mlp.nj, d_atm, (const double*)c->mlp.pinv, d_t);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_mlp_forward_dev(fz_ctx* c, int n, const float* d_x, const double* d_xd, float* d_y, void* stream) {
+  if (!c || !c->mlp.on || n < 1 || (!d_x && !d_xd) || !d_y) return FZ_EARG;
+  MlpState& M = c->mlp;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  const int chunk = 65536;
+  int r = mlp_work(c, std::min(n, chunk));
+  if (r) return r;
+  for (int r0 = 0; r0 < n; r0 += M.cap) {
+    const int nr = std::min(M.cap, n - r0);
+    r = mlp_input(c, nr, d_x ? d_x + (size_t)r0 * M.nin : nullptr, d_xd ? d_xd + (size_t)r0 * M.nin : nullptr,
+                  nullptr, st);
+    if (r) return r;
+    mlp_forward_rows(M, nr, st);
+    HIPCHK(c, hipMemcpyAsync(d_y + (size_t)r0 * M.nout, M.Y, sizeof(float) * nr * M.nout, hipMemcpyDeviceToDevice,
+                             st));
+  }
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_mlp_gradient_dev(fz_ctx* c, int n, const float* d_x, const float* d_t, const int32_t* d_idx, float* d_loss,
+                        float* d_grad, void* stream) {
+  if (!c || !c->mlp.on || n < 1 || !d_x || !d_t) return FZ_EARG;
+  MlpState& M = c->mlp;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = mlp_work(c, n);
+  if (r) return r;
+  r = mlp_input(c, n, d_x, nullptr, d_idx, st);
+  if (r) return r;
+  mlp_forward_rows(M, n, st);
+  mlp_backward_rows(M, n, d_t, d_idx, st);
+  if (d_loss) HIPCHK(c, hipMemcpyAsync(d_loss, M.loss, sizeof(float), hipMemcpyDeviceToDevice, st));
+  if (d_grad) HIPCHK(c, hipMemcpyAsync(d_grad, M.g, sizeof(float) * M.np, hipMemcpyDeviceToDevice, st));
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_mlp_train_step_dev(fz_ctx* c, int n, const float* d_x, const float* d_t, const int32_t* d_idx, float lr,
+                          float* d_loss, void* stream) {
+  int r = fz_mlp_gradient_dev(c, n, d_x, d_t, d_idx, d_loss, nullptr, stream);
+  if (r) return r;
+  MlpState& M = c->mlp;
+  const float b1 = 0.9f, b2 = 0.999f;
+  ++M.step;
+  const float c1 = (float)(1.0 - std::pow((double)b1, (double)M.step));
+  const float c2 = (float)(1.0 - std::pow((double)b2, (double)M.step));
+  hipLaunchKernelGGL(mlp_adam_kernel, dim3((unsigned)((M.np + 255) / 256)), dim3(256), 0, (hipStream_t)stream, M.np,
+                     M.w, (const float*)M.g, M.m1, M.m2, M.wb, lr, b1, b2, 1e-8f, c1, c2);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+int fz_learned_init_dev(fz_ctx* c, int ncol, const double* d_obs, float* d_atm, void* stream) {
+  if (!c || !c->mlp.on || ncol < 1 || !d_obs || !d_atm) return FZ_EARG;
+  if (c->cfg.nz != c->mlp.nz) return set_err(c, FZ_ECONFIG, "height grid changed since fz_mlp_create");
+  MlpState& M = c->mlp;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  DevBuf y;
+  HIPCHK(c, hipMalloc(&y.p, sizeof(float) * (size_t)ncol * M.nout));
+  int r = fz_mlp_forward_dev(c, ncol, nullptr, d_obs, (float*)y.p, stream);
+  if (r) return r;
+  hipLaunchKernelGGL(mlp_decode_kernel, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, st, ncol, M.nz, M.nj,
+                     (const float*)y.p, (const float*)(M.norms + 2 * M.nin), (const float*)(M.norms + 2 * M.nin + M.nout),
+                     (const double*)M.coefs, d_atm);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
   return FZ_OK;
 }
 

@@ -101,6 +101,17 @@ def library():
                                   ctypes.POINTER(ctypes.c_int), vp]
     L.fz_invert3d.argtypes = [vp, ip, ip] + [fp] * 12 + [dp, ctypes.POINTER(CInvertControl), dp, dp,
                                                        ctypes.POINTER(ctypes.c_int)]
+    L.fz_mlp_create.argtypes = [vp, ip, ip, ip, ip, ctypes.c_uint64]
+    L.fz_mlp_size.argtypes = [vp]
+    L.fz_mlp_size.restype = ctypes.c_int64
+    L.fz_mlp_get_params.argtypes = [vp, vp]
+    L.fz_mlp_set_params.argtypes = [vp, vp]
+    L.fz_mlp_fit_norms_dev.argtypes = [vp, ip, vp, vp, vp]
+    L.fz_mlp_targets_dev.argtypes = [vp, ip, vp, vp, vp]
+    L.fz_mlp_forward_dev.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.fz_mlp_gradient_dev.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp]
+    L.fz_mlp_train_step_dev.argtypes = [vp, ip, vp, vp, vp, ctypes.c_float, vp, vp]
+    L.fz_learned_init_dev.argtypes = [vp, ip, vp, vp, vp]
     _lib = L
     return L
 
@@ -112,7 +123,10 @@ def exported_symbols():
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
             "fz_set_hydrostatic", "fz_set_lsf",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
-            "fz_invert3d"]
+            "fz_invert3d",
+            "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
+            "fz_mlp_targets_dev", "fz_mlp_forward_dev", "fz_mlp_gradient_dev", "fz_mlp_train_step_dev",
+            "fz_learned_init_dev"]
 
 
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
@@ -337,6 +351,75 @@ class Context:
                                             done.data_ptr() if done is not None else None,
                                             delta.data_ptr() if delta is not None else None, stream),
                     "fz_lm_step_dev")
+
+    # ---------------------------------------------------------------- learned initial model
+    def mlp_create(self, nin: int, nhidden: int = 512, nlayers: int = 3, nodes: int = 4, seed: int = 1):
+        """Network nin -> nhidden (x nlayers, ReLU) -> 5*nodes, BF16 weights."""
+        self._check(self.lib.fz_mlp_create(self.h, nin, nhidden, nlayers, nodes, seed), "fz_mlp_create")
+        self.mlp_shape = (nin, nhidden, nlayers, 5 * nodes)
+
+    def mlp_params(self) -> np.ndarray:
+        n = self.lib.fz_mlp_size(self.h)
+        if n < 0:
+            raise FzError("no network: call mlp_create first")
+        out = np.empty(n, np.float32)
+        self._check(self.lib.fz_mlp_get_params(self.h, out.ctypes.data), "fz_mlp_get_params")
+        return out
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != self.lib.fz_mlp_size(self.h):
+            raise ValueError("parameter vector of %d, network has %d" % (p.size, self.lib.fz_mlp_size(self.h)))
+        self._check(self.lib.fz_mlp_set_params(self.h, p.ctypes.data), "fz_mlp_set_params")
+
+    @staticmethod
+    def _stream(t):
+        import torch
+        return torch.cuda.current_stream(t.device).cuda_stream
+
+    def mlp_fit_norms_torch(self, x, t):
+        self._check(self.lib.fz_mlp_fit_norms_dev(self.h, x.shape[0], x.data_ptr(), t.data_ptr(), self._stream(x)),
+                    "fz_mlp_fit_norms_dev")
+
+    def mlp_targets_torch(self, atm):
+        import torch
+        ncol = atm.shape[2]
+        t = torch.empty((ncol, self.mlp_shape[3]), dtype=torch.float32, device=atm.device)
+        self._check(self.lib.fz_mlp_targets_dev(self.h, ncol, atm.data_ptr(), t.data_ptr(), self._stream(atm)),
+                    "fz_mlp_targets_dev")
+        return t
+
+    def mlp_forward_torch(self, x):
+        """Standardised network output for inputs x [n][nin] (float32 or float64)."""
+        import torch
+        y = torch.empty((x.shape[0], self.mlp_shape[3]), dtype=torch.float32, device=x.device)
+        xf = x.data_ptr() if x.dtype == torch.float32 else None
+        xd = x.data_ptr() if x.dtype == torch.float64 else None
+        self._check(self.lib.fz_mlp_forward_dev(self.h, x.shape[0], xf, xd, y.data_ptr(), self._stream(x)),
+                    "fz_mlp_forward_dev")
+        return y
+
+    def mlp_gradient_torch(self, x, t, idx=None):
+        """(loss, gradient of the parameter vector) on rows x, targets t."""
+        import torch
+        n = x.shape[0] if idx is None else idx.shape[0]
+        loss = torch.empty(1, dtype=torch.float32, device=x.device)
+        g = torch.empty(self.lib.fz_mlp_size(self.h), dtype=torch.float32, device=x.device)
+        self._check(self.lib.fz_mlp_gradient_dev(self.h, n, x.data_ptr(), t.data_ptr(),
+                                                 None if idx is None else idx.data_ptr(), loss.data_ptr(),
+                                                 g.data_ptr(), self._stream(x)), "fz_mlp_gradient_dev")
+        return loss, g
+
+    def mlp_train_step_torch(self, x, t, idx, lr, loss):
+        self._check(self.lib.fz_mlp_train_step_dev(self.h, idx.shape[0], x.data_ptr(), t.data_ptr(), idx.data_ptr(),
+                                                   float(lr), loss.data_ptr(), self._stream(x)),
+                    "fz_mlp_train_step_dev")
+
+    def learned_init_torch(self, obs, atm):
+        """Write the network's T, Bx, By, Bz, vz for observations obs
+        [ncol][4*numw] (float64) into atm [12][nz][ncol]."""
+        self._check(self.lib.fz_learned_init_dev(self.h, obs.shape[0], obs.data_ptr(), atm.data_ptr(),
+                                                 self._stream(atm)), "fz_learned_init_dev")
 
     def invert3d_torch(self, atm, obs, control: CInvertControl, nx: int, ny: int, stokes=None, chi2=None):
         """INVERT3D on device tensors: atm [12, nz, nx*ny] float32 (initial

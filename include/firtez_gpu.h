@@ -238,6 +238,46 @@ int fz_invert3d(fz_ctx* ctx, int nx, int ny, float* tem3d, float* pg3d, float* r
                 float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
                 int* iterations);
 
+/* ---- Learned initial model.  No counterpart in the reference, which starts
+ * every inversion from the input model file (invert3d.f90:57-178); these
+ * entry points propose that initial model from the observed profiles and
+ * leave the inversion unchanged.
+ *
+ * Network: nin -> nhidden (nlayers times, ReLU) -> 5*nodes outputs, the node
+ * values (hat basis of GET_INT_COEFS, get_dmodel.f90:563-644, on the
+ * configured heights) of T, Bz, B_perp cos 2phi, B_perp sin 2phi and v_z.
+ * Weights and activations BF16, FP32 accumulation (MFMA), FP32 master
+ * weights and Adam moments.  One network per context. */
+int fz_mlp_create(fz_ctx* ctx, int nin, int nhidden, int nlayers, int nodes, uint64_t seed);
+/* Length of the parameter vector: weights and biases (W_l [out][in] row-major,
+ * then b_l, layer by layer) followed by xmean[nin], xstd[nin], tmean[nout],
+ * tstd[nout]; -1 without a network. */
+int64_t fz_mlp_size(const fz_ctx* ctx);
+int fz_mlp_get_params(fz_ctx* ctx, float* params);
+int fz_mlp_set_params(fz_ctx* ctx, const float* params);
+/* Input and target standardisation from a training set on the device:
+ * d_x [n][nin], d_t [n][5*nodes] float. */
+int fz_mlp_fit_norms_dev(fz_ctx* ctx, int n, const float* d_x, const float* d_t, void* stream);
+/* Training targets of a model cube d_atm [12][nz][ncol] (least-squares node
+ * values): d_t [ncol][5*nodes]. */
+int fz_mlp_targets_dev(fz_ctx* ctx, int ncol, const float* d_atm, float* d_t, void* stream);
+/* Standardised outputs d_y [n][5*nodes] for inputs d_x (float) or d_xd
+ * (double) [n][nin]; the other pointer NULL. */
+int fz_mlp_forward_dev(fz_ctx* ctx, int n, const float* d_x, const double* d_xd, float* d_y, void* stream);
+/* Mean squared error on standardised targets (d_loss[0]) and its gradient
+ * (d_grad, fz_mlp_size floats of which the normalisation entries are left
+ * unwritten) over the rows d_idx[0..n) of d_x/d_t (d_idx NULL: rows 0..n). */
+int fz_mlp_gradient_dev(fz_ctx* ctx, int n, const float* d_x, const float* d_t, const int32_t* d_idx, float* d_loss,
+                        float* d_grad, void* stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8, learning rate lr) on the
+ * gradient of the rows d_idx[0..n). */
+int fz_mlp_train_step_dev(fz_ctx* ctx, int n, const float* d_x, const float* d_t, const int32_t* d_idx, float lr,
+                          float* d_loss, void* stream);
+/* Initial model from observations d_obs [ncol][nin] (double, as fz_invert3d_dev
+ * takes them): overwrites T (kept in 2500-18000 K), Bx, By, Bz and v_z of
+ * d_atm [12][nz][ncol]; the other rows stay as given.  Synchronises. */
+int fz_learned_init_dev(fz_ctx* ctx, int ncol, const double* d_obs, float* d_atm, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
