@@ -110,6 +110,7 @@ __global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
   const double lam = g_c.csaha1 * pow(T, -1.5);
   const double dlam = -3.0 / (2.0 * T);
   double* o = a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct;
+#pragma unroll 1
   for (int i = 0; i < FZ_NELEM; ++i) {
     double a1, a2, da1, da2;
     saha_element(i, T, lam, dlam, a.der != 0, a1, a2, da1, da2);
@@ -226,6 +227,7 @@ __global__ void __launch_bounds__(256) saha_mid_kernel(MicroTile a, double* __re
   const double T = he_tmid(k, tm0, tm1);
   const double lam = g_c.csaha1 * pow(T, -1.5);
   double* o = smid + (long)k * FZ_NELEM * 4 * a.ntile + ct;
+#pragma unroll 1
   for (int i = 0; i < FZ_NELEM; ++i) {
     double a1, a2, da1, da2;
     saha_element(i, T, lam, 0.0, false, a1, a2, da1, da2);
