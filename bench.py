@@ -43,6 +43,22 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
+def cpu_baseline_reference(nz, seconds=10.0, procs=16):
+    """The reference's own Fortran column path (oracle/_ref, compiled from its
+    sources) on the host cores, one process per core, in a child process that
+    never touches the GPU; None when the reference build is absent."""
+    import subprocess
+    lib = os.path.join(ROOT, "oracle", "_ref", "libfzref.so")
+    if not os.path.exists(lib):
+        return None
+    r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--nz", str(nz), "--seconds", str(seconds),
+                        "--procs", str(procs)], cwd=ROOT, capture_output=True, text=True, timeout=seconds * 4 + 120)
+    if r.returncode != 0:
+        print("[bench] reference CPU baseline failed: " + r.stderr[-400:], file=sys.stderr, flush=True)
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def cpu_baseline(ls, nz, gain, seconds=12.0, threads=16):
     """The oracle (FP64 C restatement, OpenMP over columns) on a bounded sample
     of the same workload: columns of the seeded cube, full Jacobians."""
@@ -82,7 +98,9 @@ def main():
     ap.add_argument("--nz", type=int, default=96)
     ap.add_argument("--tile", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
+                    help="CPU baseline: the reference's Fortran (oracle/_ref) or the C restatement")
     ap.add_argument("--lm", action="store_true", help="measure LM-converged pixels/s instead")
     ap.add_argument("--learned", action="store_true", help="with --lm: learned initial model vs plain")
     ap.add_argument("--train-cubes", type=int, default=8)
@@ -160,8 +178,14 @@ def main():
                      "flop_per_column_lambda": fl},
     }
     if not args.no_cpu:
-        print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds)
+        base = None
+        if args.cpu_kind == "reference":
+            print("[bench] CPU baseline (reference Fortran, 16 processes) ...", file=sys.stderr, flush=True)
+            base = cpu_baseline_reference(args.nz, seconds=args.cpu_seconds)
+        if base is None:
+            print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
+            base = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds)
+        line["cpu_baseline"] = base
     print(json.dumps(line), flush=True)
 
 

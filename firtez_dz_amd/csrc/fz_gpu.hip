@@ -97,6 +97,7 @@ struct MicroTile {
   float* micro;
   double* sbuf;    // [nz][92][4][ntile]
   double* eos;     // [2][nz][ntile]: n_e, N_H
+  double* deriv;   // [8][nz][ntile]: the DERIVVAR scalars (with response functions)
   int32_t* status; // [ncol]
 };
 
@@ -143,6 +144,33 @@ __global__ void __launch_bounds__(64) rho_kernel(MicroTile a) {
   a.status[col] = err;
 }
 
+// The implicit EOS derivatives (GET_DNHYDDTEMP and its four solves,
+// chemical_equilibrium.f90:2008-2052) per (height, column), kept apart from
+// the opacities so that neither kernel carries the other's registers.
+__global__ void __launch_bounds__(256) deriv_kernel(MicroTile a) {
+  const int nz = c_cfg.nz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nz * a.ntile) return;
+  const int k = (int)(t / a.ntile), ct = (int)(t - (long)k * a.ntile), col = a.col0 + ct;
+  if (a.status[col] != 0) return;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
+  const float rhof = a.atm[A_RHO * ps + k * cs + col];
+  const double ne = a.eos[(long)k * a.ntile + ct];
+  const SahaView s(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile);
+  Deriv d = {};
+  eos_derivatives(s, T, P, (double)rhof, ne, d);
+  const long ds = (long)nz * a.ntile, o = (long)k * a.ntile + ct;
+  a.deriv[0 * ds + o] = d.nh_T_pg;
+  a.deriv[1 * ds + o] = d.nh_T_rho;
+  a.deriv[2 * ds + o] = d.nh_pg_T;
+  a.deriv[3 * ds + o] = d.nh_rho_T;
+  a.deriv[4 * ds + o] = d.ne_pg_T;
+  a.deriv[5 * ds + o] = d.ne_T_rho;
+  a.deriv[6 * ds + o] = d.ne_T_pg;
+  a.deriv[7 * ds + o] = d.ne_rho_T;
+}
+
 __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
   const int nz = c_cfg.nz;
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,7 +184,19 @@ __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
   const double ne = a.eos[(long)k * a.ntile + ct], nh = a.eos[((long)nz + k) * a.ntile + ct];
   const SahaView s(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile);
   Deriv d = {};
-  if (a.der) eos_derivatives(s, T, P, (double)rhof, ne, d);
+  if (a.der) {
+    const long ds = (long)nz * a.ntile, o = (long)k * a.ntile + ct;
+    d.nh_T_pg = a.deriv[0 * ds + o];
+    d.nh_T_rho = a.deriv[1 * ds + o];
+    d.nh_pg_T = a.deriv[2 * ds + o];
+    d.nh_rho_T = a.deriv[3 * ds + o];
+    d.ne_pg_T = a.deriv[4 * ds + o];
+    d.ne_T_rho = a.deriv[5 * ds + o];
+    d.ne_T_pg = a.deriv[6 * ds + o];
+    d.ne_rho_T = a.deriv[7 * ds + o];
+  }
+  (void)P;
+  (void)rhof;
   double kc[4];
   int err = opac_cont(T, 5000.0, ne, nh, d, false, kc);
   if (!err) {
@@ -919,6 +959,8 @@ struct fz_ctx {
   size_t stat_cap = 0;
   float* d_smid = nullptr;    // Saha factors at the RK4 mid-step temperatures (double)
   size_t smid_cap = 0;
+  float* d_derb = nullptr;    // EOS derivatives [8][nz][mtile] (double)
+  size_t derb_cap = 0;
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
@@ -995,6 +1037,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_eosb) (void)hipFree(c->d_eosb);
   if (c->d_stat) (void)hipFree(c->d_stat);
   if (c->d_smid) (void)hipFree(c->d_smid);
+  if (c->d_derb) (void)hipFree(c->d_derb);
   if (c->d_lsf) (void)hipFree(c->d_lsf);
   c->mlp.release();
   for (int i = 0; i < 3; ++i)
@@ -1235,6 +1278,10 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
   if (r) return r;
   r = ensure(c, &c->d_eosb, &c->eosb_cap, sizeof(double) * (size_t)2 * nz * mt);
   if (r) return r;
+  if (der && physics) {
+    r = ensure(c, &c->d_derb, &c->derb_cap, sizeof(double) * (size_t)8 * nz * mt);
+    if (r) return r;
+  }
   if (hydro) {
     r = ensure(c, &c->d_smid, &c->smid_cap, sizeof(double) * (size_t)(nz - 1) * FZ_NELEM * 4 * mt);
     if (r) return r;
@@ -1249,6 +1296,7 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
     a.micro = c->d_micro;
     a.sbuf = (double*)c->d_sbuf;
     a.eos = physics ? (double*)c->d_eosb : nullptr;
+    a.deriv = (der && physics) ? (double*)c->d_derb : nullptr;
     a.status = stv;
     const long nkt = (long)nz * a.ntile;
     hipLaunchKernelGGL(saha_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
@@ -1261,6 +1309,7 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
       hipLaunchKernelGGL(rho_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
     }
     if (physics) {
+      if (a.der) hipLaunchKernelGGL(deriv_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
       hipLaunchKernelGGL(opac_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
       hipLaunchKernelGGL(tau_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
     }
