@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--nz", type=int, default=96)
     ap.add_argument("--tile", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
                     help="CPU baseline: the reference's Fortran (oracle/_ref) or the C restatement")
@@ -136,6 +137,12 @@ def main():
     status = torch.zeros(ncol, dtype=torch.int32, device="cuda")
     print("[bench] buffers: jacobian %.1f GB" % (jac.numel() * 4 / 1e9), file=sys.stderr, flush=True)
     ctx.set_profiling(True)
+    tiles = None
+    if not args.no_autotune:
+        tiles = ctx.autotune_torch(d_atm, gpu.RF_SPHERICAL, stokes, jac, status)
+        args.tile = tiles[0]
+        print("[bench] autotuned tiles: radiative transfer %d columns, microphysics %d columns" % tiles,
+              file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         ctx.forward_torch(d_atm, gpu.RF_SPHERICAL, stokes, jac, status)
     torch.cuda.synchronize()
@@ -169,7 +176,9 @@ def main():
         "config": {"workload": "forward+Jacobian %dx%d columns x nz=%d, Fe I 6301.5+6302.5, %d wavelengths, "
                                "d/d(T,|B|,incl,azim,v_los) per height" % (args.nx, args.ny, args.nz, numw),
                    "columns": ncol, "nz": args.nz, "wavelengths": numw, "parameters": npar,
-                   "kernel_ms": {"micro": float(np.mean(micro_ms)), "rt": rt_avg_ms}},
+                   "kernel_ms": {"micro": float(np.mean(micro_ms)), "rt": rt_avg_ms},
+                   "tiles": {"rt_columns": args.tile, "micro_columns": tiles[1] if tiles else None,
+                             "autotuned": tiles is not None}},
         # compute-bound: FP32 arithmetic on the vector ALUs (no matrix work in
         # this kernel); priced against MI355X's 157.3 TFLOP/s FP32 peak
         "roofline": {"bound": "mfma", "pipe": "fp32 valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,

@@ -1402,6 +1402,72 @@ int fz_last_timing(fz_ctx* c, float* micro_ms, float* rt_ms, int* launches) {
   return FZ_OK;
 }
 
+int fz_autotune(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac, int32_t* d_status,
+                void* stream, int* tile_out, int* mtile_out) {
+  if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool prof = c->prof;
+  int r = fz_set_profiling(c, 1);
+  if (r) return r;
+  const int nz = c->cfg.nz, numw = c->cfg.numw;
+  size_t freeb = 0, totb = 0;
+  HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+  const size_t budget = freeb / 3;
+  // microphysics tile: Saha factors, EOS and derivative buffers per column
+  const size_t per_mcol = sizeof(double) * (size_t)nz * (FZ_NELEM * 4 * (c->hydro ? 2 : 1) + 2 + 8);
+  // radiative-transfer tile: the Stokes scratch per column
+  const size_t per_rcol = sizeof(float) * (size_t)nz * (c->hydro ? 8 : 4) * numw;
+  const int mcand[] = {4096, 8192, 16384, 32768, 65536, 131072, 262144};
+  const int rcand[] = {2048, 4096, 8192, 16384, 32768};
+  float best = 1e30f;
+  int best_m = c->mtile, best_t = c->tile;
+  for (int m : mcand) {
+    if (m > 2 * ncol && m != mcand[0]) break;
+    if ((size_t)std::min(m, ncol) * per_mcol > budget) break;
+    c->mtile = m;
+    r = fz_forward_dev(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_status, stream);
+    if (r) return r;
+    float ms = 0.f;
+    r = fz_last_timing(c, &ms, nullptr, nullptr);
+    if (r) return r;
+    if (ms < best) {
+      best = ms;
+      best_m = m;
+    }
+  }
+  c->mtile = best_m;
+  best = 1e30f;
+  if (rf_mode != FZ_RF_NONE) {
+    for (int t : rcand) {
+      if (t > 2 * ncol && t != rcand[0]) break;
+      if ((size_t)std::min(t, ncol) * per_rcol > budget) break;
+      c->tile = t;
+      r = fz_forward_dev(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_status, stream);
+      if (r) return r;
+      float ms = 0.f;
+      r = fz_last_timing(c, nullptr, &ms, nullptr);
+      if (r) return r;
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+      }
+    }
+  }
+  c->tile = best_t;
+  // release what the larger trials left behind; the next call allocates its size
+  if (c->d_sbuf) (void)hipFree(c->d_sbuf);
+  if (c->d_eosb) (void)hipFree(c->d_eosb);
+  if (c->d_derb) (void)hipFree(c->d_derb);
+  if (c->d_smid) (void)hipFree(c->d_smid);
+  if (c->d_scratch) (void)hipFree(c->d_scratch);
+  c->d_sbuf = c->d_eosb = c->d_derb = c->d_smid = c->d_scratch = nullptr;
+  c->sbuf_cap = c->eosb_cap = c->derb_cap = c->smid_cap = c->scratch_cap = 0;
+  c->prof = prof;
+  if (tile_out) *tile_out = c->tile;
+  if (mtile_out) *mtile_out = c->mtile;
+  return FZ_OK;
+}
+
 int fz_hydrostatic_dev(fz_ctx* c, int ncol, float* d_atm, int32_t* d_status, void* stream) {
   if (!c || !c->configured || ncol < 1 || !d_atm) return FZ_EARG;
   HIPCHK(c, hipSetDevice(c->device));

@@ -101,6 +101,7 @@ def library():
                                   ctypes.POINTER(ctypes.c_int), vp]
     L.fz_invert3d.argtypes = [vp, ip, ip] + [fp] * 12 + [dp, ctypes.POINTER(CInvertControl), dp, dp,
                                                        ctypes.POINTER(ctypes.c_int)]
+    L.fz_autotune.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.fz_mlp_create.argtypes = [vp, ip, ip, ip, ip, ctypes.c_uint64]
     L.fz_mlp_size.argtypes = [vp]
     L.fz_mlp_size.restype = ctypes.c_int64
@@ -124,7 +125,7 @@ def exported_symbols():
             "fz_set_hydrostatic", "fz_set_lsf",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d",
-            "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
+            "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
             "fz_mlp_targets_dev", "fz_mlp_forward_dev", "fz_mlp_gradient_dev", "fz_mlp_train_step_dev",
             "fz_learned_init_dev"]
 
@@ -246,6 +247,17 @@ class Context:
         v = None if valid is None else np.ascontiguousarray(valid, dtype=np.int32)
         self._check(self.lib.fz_set_lsf(self.h, k.ctypes.data, None if v is None else v.ctypes.data), "fz_set_lsf")
         self._lsf = (k, v)
+
+    def autotune_torch(self, atm, rf_mode, stokes, jac, status):
+        """Pick the microphysics and radiative-transfer tiles by timing forwards
+        on these buffers; returns (rt tile, microphysics tile)."""
+        import torch
+        t, m = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.fz_autotune(self.h, atm.shape[2], atm.data_ptr(), rf_mode, stokes.data_ptr(),
+                                         None if jac is None else jac.data_ptr(), status.data_ptr(),
+                                         torch.cuda.current_stream(atm.device).cuda_stream, ctypes.byref(t),
+                                         ctypes.byref(m)), "fz_autotune")
+        return t.value, m.value
 
     def set_tile(self, columns: int, block: int = 256):
         self._check(self.lib.fz_set_tile(self.h, columns, block), "fz_set_tile")

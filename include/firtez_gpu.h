@@ -136,6 +136,14 @@ int fz_set_hydrostatic(fz_ctx* ctx, int on);
  * tile); the autotuner sets it.  Default 8192. */
 int fz_set_tile(fz_ctx* ctx, int columns_per_launch, int block_threads);
 
+/* Autotuner (no counterpart in the reference): times fz_forward_dev on the
+ * caller's data with microphysics tiles of 4096 .. 262144 columns and
+ * radiative-transfer launches of 2048 .. 32768 columns (within a third of the
+ * free device memory), keeps the fastest of each for this context and
+ * returns them.  The outputs of the last trial are a valid forward. */
+int fz_autotune(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
+                int32_t* d_status, void* stream, int* tile, int* mtile);
+
 
 /* ------------------------------------------------------------------ inversion */
 
