@@ -950,7 +950,7 @@ struct fz_ctx {
   float* d_jtile = nullptr;   // per-height response functions of one tile
   size_t jtile_cap = 0;
   // microphysics tile buffers
-  int mtile = 16384;
+  int mtile = 0;               // microphysics columns per tile; 0: from the free memory (auto_mtile)
   float* d_sbuf = nullptr;    // Saha factors [nz][92][4][mtile] (double)
   size_t sbuf_cap = 0;
   float* d_eosb = nullptr;    // n_e, N_H [2][nz][mtile] (double)
@@ -1263,6 +1263,21 @@ static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
 // hydro, Pg is first re-integrated from the top (forward.f90:891-906) and the
 // EOS takes N_H and n_e from that integration; with physics == false only the
 // hydrostatic integration runs (fz_hydrostatic_dev).
+// Microphysics tile when none is set: the electron-density fixed point is
+// sequential in height, one thread per column, so a tile should hold as many
+// columns as memory allows (up to a quarter of the free memory, at most
+// 64 GiB, for the Saha factors and the EOS buffers of every column).
+static int auto_mtile(fz_ctx* c, bool hydro) {
+  size_t freeb = 0, totb = 0;
+  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return 16384;
+  const size_t per_col = sizeof(double) * (size_t)c->cfg.nz * (FZ_NELEM * 4 * (hydro ? 2 : 1) + 2 + 8);
+  // what this context already holds for the tile counts as available
+  const size_t held = c->sbuf_cap + c->eosb_cap + c->derb_cap + c->smid_cap;
+  const size_t budget = std::min((freeb + held) / 4, (size_t)64 << 30);
+  const size_t m = budget / per_col;
+  return (int)std::max<size_t>(4096, std::min<size_t>(m, 1 << 20));
+}
+
 static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool der, bool hydro, bool physics,
                      hipStream_t st) {
   const int nz = c->cfg.nz;
@@ -1273,7 +1288,7 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
     stv = (int32_t*)c->d_stat;
     *status = stv;
   }
-  const int mt = std::min(c->mtile, ncol);
+  const int mt = std::min(c->mtile > 0 ? c->mtile : auto_mtile(c, hydro), ncol);
   int r = ensure(c, &c->d_sbuf, &c->sbuf_cap, sizeof(double) * (size_t)nz * FZ_NELEM * 4 * mt);
   if (r) return r;
   r = ensure(c, &c->d_eosb, &c->eosb_cap, sizeof(double) * (size_t)2 * nz * mt);
@@ -2201,10 +2216,16 @@ int fz_mlp_create(fz_ctx* c, int nin, int nhidden, int nlayers, int nodes, uint6
     nrm[2 * nin + M.nout + i] = 1.f;
   }
   HIPCHK(c, hipMemcpy(M.norms, nrm.data(), sizeof(float) * nrm.size(), hipMemcpyHostToDevice));
-  // hat basis of GET_INT_COEFS and its least-squares pseudo-inverse
+  // hat basis of GET_INT_COEFS (fz_int_coefs: node-major [nj][nz]) as
+  // cf[k * nj + j], and its least-squares pseudo-inverse pv[j * nz + k]
   std::vector<double> cf((size_t)M.nz * nodes), pv((size_t)nodes * M.nz);
-  int r = fz_int_coefs(M.nz, nodes, cf.data());
-  if (r) return set_err(c, r, "node basis");
+  {
+    std::vector<double> cj((size_t)M.nz * nodes);
+    const int r0 = fz_int_coefs(M.nz, nodes, cj.data());
+    if (r0) return set_err(c, r0, "node basis");
+    for (int j = 0; j < nodes; ++j)
+      for (int k = 0; k < M.nz; ++k) cf[(size_t)k * nodes + j] = cj[(size_t)j * M.nz + k];
+  }
   {
     // normal equations (C^T C) P = C^T, solved by Gauss-Jordan on nodes x nodes
     const int nj = nodes, nz = M.nz;

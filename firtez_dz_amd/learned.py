@@ -68,6 +68,7 @@ class TrainReport:
     seconds: float
     train_loss: List[float] = field(default_factory=list)
     valid_loss: List[float] = field(default_factory=list)
+    valid_loss_initial: float = float("nan")
 
 
 def train(ctx, X, T, epochs: int = 8, batch: int = 1024, lr: float = 1e-3, lr_min: float = 2e-5,
@@ -87,6 +88,7 @@ def train(ctx, X, T, epochs: int = 8, batch: int = 1024, lr: float = 1e-3, lr_mi
     total = steps_per_epoch * epochs
     rep = TrainReport(epochs, total, 0.0)
     vidx = torch.arange(nt, n, dtype=torch.int32, device="cuda")
+    rep.valid_loss_initial = _valid_loss(ctx, X, T, vidx)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     step = 0
