@@ -1795,7 +1795,33 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
   a.atm_best = d_atm_best;
   a.done = d_done;
   a.delta = d_delta;
+  a.pix_of = nullptr;
   hipLaunchKernelGGL(lm_kernel, dim3(ncol), dim3(LM_THREADS), c->lm_lds, (hipStream_t)stream, a);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+// fz_lm_step_dev over the na active pixels pix_of[0..na): stokes and eqv are
+// the compacted rows of those pixels, everything else is in pixel space.
+static int lm_step_active(fz_ctx* c, int ncol, int na, const int32_t* pix_of, const double* d_obs,
+                          const float* d_stokes_c, double* d_best_syn, double* d_sla, const float* d_eqv_c,
+                          float* d_eqv_best, const double* d_isigmap, float* d_atm, float* d_atm_best, int32_t* d_done,
+                          hipStream_t st) {
+  LmArgs a = c->lm;
+  a.ncol = ncol;
+  a.obs = d_obs;
+  a.stokes = d_stokes_c;
+  a.best_syn = d_best_syn;
+  a.sla = d_sla;
+  a.eqv = d_eqv_c;
+  a.eqv_best = d_eqv_best;
+  a.isigmap = d_isigmap;
+  a.atm = d_atm;
+  a.atm_best = d_atm_best;
+  a.done = d_done;
+  a.delta = nullptr;
+  a.pix_of = pix_of;
+  hipLaunchKernelGGL(lm_kernel, dim3(na), dim3(LM_THREADS), c->lm_lds, st, a);
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
 }
@@ -1870,7 +1896,7 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   for (int p = 0; p < 8; ++p)
     if (ctl->atmpar[p]) npmax += ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
   if (npmax > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters in the last cycle");
-  DevBuf b_isig, b_best, b_eqv, b_eqvb, b_bsyn, b_sla, b_done, b_stat, b_t0, b_t1;
+  DevBuf b_isig, b_best, b_eqv, b_eqvb, b_bsyn, b_sla, b_done, b_stat, b_t0, b_t1, b_atmc, b_idx, b_cnt;
   const size_t natm = (size_t)12 * nz * ncol;
   HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));
   HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * natm));
@@ -1882,6 +1908,11 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   HIPCHK(c, hipMalloc(&b_stat.p, sizeof(double) * 2));
   HIPCHK(c, hipMalloc(&b_t0.p, sizeof(double) * (size_t)nz * ncol));
   HIPCHK(c, hipMalloc(&b_t1.p, sizeof(double) * (size_t)nz * ncol));
+  // active pixels: FORWARD3D models only the columns with AM_I_DONE == 0
+  // (forward.f90:43-306); they are gathered into a compact cube
+  HIPCHK(c, hipMalloc(&b_atmc.p, sizeof(float) * natm));
+  HIPCHK(c, hipMalloc(&b_idx.p, sizeof(int32_t) * ncol));
+  HIPCHK(c, hipMalloc(&b_cnt.p, sizeof(int32_t)));
   HIPCHK(c, hipMemcpyAsync(b_isig.p, isig.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, st));
   float* best = (float*)b_best.p;
   double* sla = (double*)b_sla.p;
@@ -1935,13 +1966,29 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
     }
     HIPCHK(c, hipGetLastError());
     const double need = (double)std::max(1, maxsteps - (maxiter - cyc));
+    int na = ncol;   // active pixels (every pixel at a cycle start)
     for (int it = 0; it < itcap; ++it) {
       // MODELLING -> SOLVE_DMODEL -> MAKE_DECISION
-      r = fz_forward_nodes_dev(c, ncol, d_atm, d_stokes, (float*)b_eqv.p, nullptr, stream);
-      if (r) return r;
-      r = fz_lm_step_dev(c, ncol, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p, (float*)b_eqvb.p,
-                         (const double*)b_isig.p, d_atm, best, done, nullptr, stream);
-      if (r) return r;
+      if (na == ncol) {
+        r = fz_forward_nodes_dev(c, ncol, d_atm, d_stokes, (float*)b_eqv.p, nullptr, stream);
+        if (r) return r;
+        r = lm_step_active(c, ncol, ncol, nullptr, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p,
+                           (float*)b_eqvb.p, (const double*)b_isig.p, d_atm, best, done, st);
+        if (r) return r;
+      } else {
+        const int32_t* idx = (const int32_t*)b_idx.p;
+        float* atmc = (float*)b_atmc.p;
+        const unsigned gc = (unsigned)(((long)12 * nz * na + 255) / 256);
+        hipLaunchKernelGGL(gather_cols_kernel, dim3(gc), dim3(256), 0, st, (const float*)d_atm, atmc, idx, na, ncol,
+                           12 * nz);
+        r = fz_forward_nodes_dev(c, na, atmc, d_stokes, (float*)b_eqv.p, nullptr, stream);
+        if (r) return r;
+        hipLaunchKernelGGL(scatter_cols_kernel, dim3(gc), dim3(256), 0, st, (const float*)atmc, d_atm, idx, na,
+                           ncol, 12 * nz);
+        r = lm_step_active(c, ncol, na, idx, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p,
+                           (float*)b_eqvb.p, (const double*)b_isig.p, d_atm, best, done, st);
+        if (r) return r;
+      }
       hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, st, ncol, sla, done, (double*)b_stat.p);
       double hs[2];
       HIPCHK(c, hipMemcpyAsync(hs, b_stat.p, sizeof(hs), hipMemcpyDeviceToHost, st));
@@ -1949,6 +1996,13 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
       ++total_it;
       if (ncol - (int)hs[1] <= 0) break;
       if (hs[0] / (double)ncol >= need) break;
+      const int nact = ncol - (int)hs[1];
+      if (nact < ncol) {
+        hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, st, ncol, (const int32_t*)done, (int32_t*)b_idx.p,
+                           (int32_t*)b_cnt.p);
+        HIPCHK(c, hipGetLastError());
+      }
+      na = nact;
     }
     // END_CYCLE (:328-403): current := best, synthesis without response functions
     hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, best, d_atm, (long)natm);

@@ -82,6 +82,48 @@ __global__ void __launch_bounds__(256) stats_kernel(int ncol, const double* __re
   }
 }
 
+// Active pixels (AM_I_DONE == 0) in ascending order: idx[0..count), one
+// workgroup of 1024 threads, each scanning a contiguous chunk of the flags.
+__global__ void __launch_bounds__(1024) compact_kernel(int ncol, const int32_t* __restrict__ done,
+                                                       int32_t* __restrict__ idx, int32_t* __restrict__ count) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (ncol + 1023) / 1024;
+  const int b = t * chunk, e = min(ncol, b + chunk);
+  int n = 0;
+  for (int i = b; i < e; ++i) n += done[i] == 0;
+  part[t] = n;
+  __syncthreads();
+  // inclusive scan (Hillis-Steele) of the per-thread counts
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int o = part[t] - n;
+  for (int i = b; i < e; ++i)
+    if (done[i] == 0) idx[o++] = i;
+  if (t == 1023) count[0] = part[1023];
+}
+
+// rows x [nrow][ncol] <-> compacted xc [nrow][na] through idx
+__global__ void gather_cols_kernel(const float* __restrict__ x, float* __restrict__ xc,
+                                   const int32_t* __restrict__ idx, int na, int ncol, int nrow) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)na * nrow) return;
+  const int r = (int)(i / na), j = (int)(i - (long)r * na);
+  xc[i] = x[(long)r * ncol + idx[j]];
+}
+
+__global__ void scatter_cols_kernel(const float* __restrict__ xc, float* __restrict__ x,
+                                    const int32_t* __restrict__ idx, int na, int ncol, int nrow) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)na * nrow) return;
+  const int r = (int)(i / na), j = (int)(i - (long)r * na);
+  x[(long)r * ncol + idx[j]] = xc[i];
+}
+
 __global__ void copy_f32_kernel(const float* __restrict__ src, float* __restrict__ dst, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];

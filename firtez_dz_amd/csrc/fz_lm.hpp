@@ -53,6 +53,9 @@ struct LmArgs {
   float* atm_best;             // [12][nz][ncol]
   int32_t* done;               // [ncol] AM_I_DONE
   double* delta;               // [ncol][np] applied node perturbation, or null
+  // active-pixel launch: workgroup b handles pixel pix_of[b], and stokes/eqv
+  // hold the compacted rows (row b); null: workgroup b is pixel b
+  const int32_t* pix_of;
 };
 
 // ---------------------------------------------------------------- node contraction
@@ -155,7 +158,8 @@ __device__ void lm_penalties(const LmArgs& a, const float* mod, double* pen_res,
 __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   extern __shared__ double smem[];
   const int t = threadIdx.x;
-  const int pix = blockIdx.x;
+  const int row = blockIdx.x;
+  const int pix = a.pix_of ? a.pix_of[row] : row;
   if (a.done && a.done[pix] != 0) return;
   const int np = a.np, nz = a.nz, nf = a.nfreq, numw = a.numw;
   const int ld = np + 1;                       // LDS row stride of A and V
@@ -186,7 +190,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm[(long)i * ncol + pix];
   if (t < 9) s_sla[t] = a.sla[(long)pix * 9 + t];
   // columns whose absolute sum overflows REAL(4) are zeroed (forward.f90:672-678)
-  const float* ecur = a.eqv + (long)pix * np * nf;
+  const float* ecur = a.eqv + (long)row * np * nf;
   for (int j = t; j < np; j += LM_THREADS) {
     float s = 0.f;
     for (int f = 0; f < nf; ++f) s += fabsf(ecur[(long)j * nf + f]);
@@ -196,7 +200,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
 
   // ---- NEW_CALCULATE_CHI2 (get_dmodel.f90:960-1032)
   const double* obs = a.obs + (long)pix * nf;
-  const float* stk = a.stokes + (long)pix * 4 * numw;
+  const float* stk = a.stokes + (long)row * 4 * numw;
   double part = 0.0;
   for (int f = t; f < nf; f += LM_THREADS) {
     const int b = f / numw, l = f - b * numw;
