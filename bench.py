@@ -107,7 +107,7 @@ def main():
     ap.add_argument("--train-cubes", type=int, default=8)
     ap.add_argument("--train-nx", type=int, default=512)
     ap.add_argument("--epochs", type=int, default=6)
-    ap.add_argument("--max-passes", type=int, default=4)
+    ap.add_argument("--max-passes", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (columns are independent; "
@@ -224,7 +224,9 @@ def lm_problem(args):
     obs = (st.double().reshape(ncol, 4 * numw) +
            1e-3 * torch.randn((ncol, 4 * numw), generator=gen, device="cuda", dtype=torch.float64)).contiguous()
     init = plain_init(ctx, truth)
-    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3)
+    # no SMOOTHING_B: B and v of the seeded truth are independent from pixel
+    # to pixel, so the 1-2-1 filter would mix unrelated pixels at every cycle
+    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3, smoothing=False)
     return ctx, truth, obs, init, ctl, nx, ny, nz, numw
 
 
@@ -289,8 +291,10 @@ def bench_lm(args):
     rep = learned.train(ctx, X, T, epochs=args.epochs, batch=1024, lr=1e-3,
                         log=lambda m: print("[bench-lm] " + m, file=sys.stderr, flush=True))
     del X, T
-    # the fallback threshold: twice the chi^2 level plain LM reaches on a
-    # held-out 64x64 cube of the same statistics
+    # the fallback threshold: twice the median chi^2 the learned pipeline
+    # reaches on a held-out 64x64 cube of the same statistics (the level the
+    # 4-node model reaches with this noise); pixels above it are the
+    # network's failures and go to plain LM
     from firtez_dz_amd import atmos
     ca, _ = atmos.seeded(64, 64, nz, dz_km=15.0, seed=4242)
     ctruth = torch.from_numpy(ca).cuda()
@@ -300,7 +304,7 @@ def bench_lm(args):
     g2.manual_seed(12)
     cobs = (cst.double().reshape(64 * 64, -1) + 1e-3 * torch.randn(cst.reshape(64 * 64, -1).shape, generator=g2,
                                                                    device="cuda", dtype=torch.float64)).contiguous()
-    _, _, cchi, _ = learned.invert(ctx, plain_init(ctx, ctruth), cobs, ctl, 64, 64, None, learned=False)
+    _, _, cchi, _ = learned.invert(ctx, plain_init(ctx, ctruth), cobs, ctl, 64, 64, None, learned=True)
     noise_chi2 = float(cchi.median())
     threshold = 2.0 * noise_chi2
     print("[bench-lm] network trained in %.1f s (data %.1f s); calibration chi2 %.3g -> threshold %.3g"
@@ -338,8 +342,13 @@ def bench_lm(args):
             "data": "synthetic (seeded truth + 1e-3 noise; network trained on %d other seeded pairs)"
                     % (tc * args.train_nx ** 2),
             "config": {"workload": workload, "learned": lrn, "plain": plain, "plain_to_learned_chi2": to_target,
-                       "speedup_vs_plain_at_equal_median_chi2": speed if to_target["reached"] else None,
-                       "speedup_lower_bound": None if to_target["reached"] else speed,
+                       # the learned pipeline ends at a median chi^2 no higher than plain LM's, so it
+                       # reaches plain's level no later than its own end: a conservative speed-up
+                       "speedup_at_plain_median_chi2": (plain["seconds"] / dtl
+                                                        if lrn["median_chi2"] <= plain["median_chi2"] else None),
+                       # plain LM restarted from its own result towards the learned median chi^2
+                       "speedup_at_learned_median_chi2": speed if to_target["reached"] else None,
+                       "speedup_at_learned_median_chi2_lower_bound": None if to_target["reached"] else speed,
                        "network": {"shape": "%d-512-512-512-20" % (4 * numw), "pairs": tc * args.train_nx ** 2,
                                    "epochs": args.epochs, "train_seconds": rep.seconds, "data_seconds": t_data,
                                    "train_loss": rep.train_loss, "valid_loss": rep.valid_loss,

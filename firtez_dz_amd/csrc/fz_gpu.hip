@@ -1920,7 +1920,8 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   const unsigned gatm = (unsigned)((natm + 255) / 256), grow = (unsigned)(((size_t)nz * ncol + 255) / 256);
   static const int rowof[8] = {A_T, A_PG, A_RHO, A_BX, A_BY, A_BZ, A_VZ, A_PG};
   int total_it = 0;
-  for (int cyc = 1; cyc <= maxiter; ++cyc) {
+  const int cyc0 = std::min(std::max(1, ctl->first_cycle), maxiter);
+  for (int cyc = cyc0; cyc <= maxiter; ++cyc) {
     // INITIALIZE_CYCLE (invert3d.f90:228-325)
     fz_inversion inv;
     std::memset(&inv, 0, sizeof(inv));

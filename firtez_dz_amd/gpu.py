@@ -51,7 +51,8 @@ class CInvertControl(ctypes.Structure):
                 ("wstk", ctypes.c_float * 4), ("noise", ctypes.c_double), ("cycpow", ctypes.c_int32),
                 ("maxsteps", ctypes.c_int32), ("solver", ctypes.c_int32), ("node_expansion", ctypes.c_int32),
                 ("regul", ctypes.c_int32), ("pen_typ", ctypes.c_int32 * 8), ("pen_alp", ctypes.c_double * 8),
-                ("svdtol", ctypes.c_double), ("max_iterations", ctypes.c_int32), ("smoothing", ctypes.c_int32)]
+                ("svdtol", ctypes.c_double), ("max_iterations", ctypes.c_int32), ("smoothing", ctypes.c_int32),
+                ("first_cycle", ctypes.c_int32)]
 
 
 # inverted physical parameters, in the reference's INV_ATMPAR order
@@ -132,7 +133,8 @@ def exported_symbols():
 
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
                    maxsteps=5, solver=1, node_expansion=1, svdtol=0.0, max_iterations=0, smoothing=True,
-                   regul=False, pen_typ: Optional[dict] = None, pen_alp: Optional[dict] = None) -> CInvertControl:
+                   regul=False, pen_typ: Optional[dict] = None, pen_alp: Optional[dict] = None,
+                   first_cycle=1) -> CInvertControl:
     """fz_invert_control from {parameter name: maximum nodes}, e.g. {"T": 4, "vz": 4}."""
     c = CInvertControl()
     for name, n in max_nodes.items():
@@ -143,6 +145,7 @@ def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.
     c.wstk[:] = [float(x) for x in wstk]
     c.noise, c.cycpow, c.maxsteps, c.solver, c.node_expansion = noise, cycpow, maxsteps, solver, node_expansion
     c.regul, c.svdtol, c.max_iterations, c.smoothing = int(regul), svdtol, max_iterations, int(smoothing)
+    c.first_cycle = int(first_cycle)
     for name, v in (pen_typ or {}).items():
         c.pen_typ[INV_PARAMS.index(name)] = int(v)
     for name, v in (pen_alp or {}).items():

@@ -130,17 +130,25 @@ class InvertReport:
     seconds_fallback: float
 
 
-def invert(ctx, init, obs, control, nx: int, ny: int, threshold: Optional[float], learned: bool = True):
+def invert(ctx, init, obs, control, nx: int, ny: int, threshold: Optional[float], learned: bool = True,
+           final_cycle_only: bool = True):
     """INVERT3D from the learned initial model (learned=True) or from init.
 
     init: [12][nz][ncol] default initial model (not modified); obs [ncol][NFREQ]
-    float64.  Pixels with a final chi^2 above threshold are inverted again from
-    init without the spatial smoothing (they are no longer neighbours) and keep
-    the lower chi^2.  Returns (model, stokes, chi2, InvertReport)."""
+    float64.  The proposal is already on the final node basis, so by default
+    its inversion runs only the last cycle (the coarse cycles would first
+    contract it to fewer nodes).  Pixels with a final chi^2 above threshold
+    are inverted again from init with the full schedule, without the spatial
+    smoothing (they are no longer neighbours), and keep the lower chi^2.
+    Returns (model, stokes, chi2, InvertReport)."""
     import torch
     ncol = nx * ny
     t0 = time.perf_counter()
     atm = init.clone()
+    ctl = control
+    if learned and final_cycle_only:
+        ctl = _gpu.CInvertControl.from_buffer_copy(control)
+        ctl.first_cycle = 1 << 20
     if learned:
         ctx.learned_init_torch(obs, atm)
         st = ctx.hydrostatic_torch(atm)
@@ -150,7 +158,7 @@ def invert(ctx, init, obs, control, nx: int, ny: int, threshold: Optional[float]
             ctx.hydrostatic_torch(atm)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    stokes, chi2, its = ctx.invert3d_torch(atm, obs, control, nx, ny)
+    stokes, chi2, its = ctx.invert3d_torch(atm, obs, ctl, nx, ny)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     nfb, its_fb = 0, 0

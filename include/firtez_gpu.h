@@ -218,6 +218,8 @@ typedef struct {
   double svdtol;         /* 0 -> 1e-4 */
   int32_t max_iterations;/* cap on LM iterations per cycle, 0 -> 1000 */
   int32_t smoothing;     /* SMOOTHING_B at every cycle start (the reference always applies it) */
+  int32_t first_cycle;   /* first cycle to run, 0 -> 1 (a start model already on the final node
+                            basis can skip the coarse cycles); no counterpart in the reference */
 } fz_invert_control;
 
 /* INVERT3D (invert3d.f90:57-178) on device data: the cycle schedule (nodes

@@ -72,7 +72,8 @@ def smooth(a, nx, ny):
 
 
 def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 1, 1, 1), noise=4e-3, cycpow=2,
-             maxsteps=5, solver=1, node_expansion=1, svdtol=1e-4, smoothing=True, max_iterations=1000):
+             maxsteps=5, solver=1, node_expansion=1, svdtol=1e-4, smoothing=True, max_iterations=1000,
+             first_cycle=1):
     """cpu: oracle.Cpu set up with resp=True and all Cartesian slots.
     max_nodes: {physical index 0..6: NSLB_MAX}.  Returns (model, stokes
     [ncol,4,numw], chi2 [ncol], iterations)."""
@@ -103,7 +104,7 @@ def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 
 
     sla = None
     syn = None
-    for cyc in range(1, maxiter + 1):
+    for cyc in range(min(max(1, first_cycle), maxiter), maxiter + 1):
         nodes = [0] * 8
         for p in vmax:
             nodes[p] = min(njevals(p + 1, cyc, maxiter, cycpow, nz), vmax[p])

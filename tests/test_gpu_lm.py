@@ -266,7 +266,7 @@ def _invert_problem(cpu, nx, ny, seed, numw_setup):
     return atm, z, rng
 
 
-@pytest.mark.parametrize("case", ["one-node-no-smoothing", "two-cycles-smoothing"])
+@pytest.mark.parametrize("case", ["one-node-no-smoothing", "two-cycles-smoothing", "final-cycle-only"])
 def test_invert3d_vs_oracle(gpu_ctx, cpu, case):
     """INVERT3D on the device (fz_invert3d_dev) against the oracle's
     restatement of the same schedule on a small cube."""
@@ -294,8 +294,9 @@ def test_invert3d_vs_oracle(gpu_ctx, cpu, case):
         smoothing = False
     else:
         nodes = {"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}
-        smoothing = True
-    ctl = gpu.invert_control(nodes, noise=NOISE, smoothing=smoothing)
+        smoothing = case.startswith("two")
+    first = 2 if case == "final-cycle-only" else 1
+    ctl = gpu.invert_control(nodes, noise=NOISE, smoothing=smoothing, first_cycle=first)
     d_atm = torch.from_numpy(guess.copy()).cuda()
     d_obs = torch.from_numpy(obs.copy()).cuda()
     st, chi2, its = gpu_ctx.invert3d_torch(d_atm, d_obs, ctl, nx, ny)
@@ -303,7 +304,7 @@ def test_invert3d_vs_oracle(gpu_ctx, cpu, case):
     g_model, g_chi2, g_st = d_atm.cpu().numpy(), chi2.cpu().numpy(), st.cpu().numpy()
     idx = {gpu.INV_PARAMS.index(k): v for k, v in nodes.items()}
     o_model, o_syn, o_chi2, o_its = invert.invert3d(cpu, guess, z, obs, nx, ny, idx, noise=NOISE,
-                                                   smoothing=smoothing)
+                                                   smoothing=smoothing, first_cycle=first)
     print("[invert] %s: iterations device %d oracle %d; median chi2 device %.4g oracle %.4g"
           % (case, its, o_its, np.median(g_chi2), np.median(o_chi2)), flush=True)
     assert abs(its - o_its) <= 2
