@@ -12,6 +12,8 @@ and data records of at most 268435446 values (misc.f90:15-17):
   * response functions 18060904 (misc.f90:1862-1999): header [.., 18060904,
     nrec, 6, nx, ny, nz, nw, 4, npar]; records z, line index, wavelength,
     parameter mask (8), then DSTOKES(p, s, w, k, y, x) for the stored p.
+  * line-spread function 3330904 (READ_LSF_FILE, misc.f90:2422-2555):
+    header [.., 3330904, nrec, 1, nw]; then the nw samples of one region.
 """
 from __future__ import annotations
 
@@ -19,6 +21,7 @@ import numpy as np
 
 MODEL_MAGIC = 130904
 PROFILE_MAGIC = 160904
+LSF_MAGIC = 3330904
 RF_MAGIC = 18060904
 VERSION = (3000, 3)      # SS, VV (format 3000, version 3) written as SS+VV, SS-VV (misc.f90:944-953)
 NMAXREC = 1073741784 // 4
@@ -168,3 +171,32 @@ def read_rfs(path):
     out = np.zeros((npar, ns, nw, nz, ny, nx), np.float32)
     out[logv] = keep
     return out, z, wave, logv
+
+
+def write_lsf(path, samples):
+    """Line-spread-function file of one spectral region (the layout
+    READ_LSF_FILE reads; the record count in the header is the data records
+    plus one, as the reference's own writer sets it)."""
+    data = np.asarray(samples, np.float32).reshape(-1)
+    nw = data.size
+    header = np.array([VERSION[0] + VERSION[1], VERSION[0] - VERSION[1], LSF_MAGIC, _nrecs(nw) + 1, 1, nw],
+                      np.float32)
+    _write_records(path, header, [data])
+
+
+def read_lsf(path, nwave=None):
+    """Samples (float64) of a line-spread-function file; nwave, when given,
+    must equal the file's size, as READ_LSF_FILE requires of the region."""
+    recs = _records(path)
+    hdr = recs[0]
+    if abs(int(round(float(hdr[2]))) - LSF_MAGIC) > 0:
+        raise ValueError("%s is not a line spread function file" % path)
+    if int(hdr[4]) != 1:
+        raise ValueError("%s: a line spread function file has one dimension" % path)
+    nw = int(hdr[5])
+    if nwave is not None and nw != nwave:
+        raise ValueError("%s holds %d wavelengths, the region has %d" % (path, nw, nwave))
+    data = np.concatenate(recs[1:]) if len(recs) > 1 else np.zeros(0, np.float32)
+    if data.size != nw:
+        raise ValueError("%s: %d samples, header says %d" % (path, data.size, nw))
+    return data.astype(np.float64)

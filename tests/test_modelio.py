@@ -1,9 +1,10 @@
 """Binary file formats of the reference (model 130904, profiles 160904,
-response functions 18060904): the model writer reproduces the reference's
+response functions 18060904, LSF 3330904): the model writer reproduces the reference's
 own model file byte for byte, and profiles / RF files round-trip."""
 import os
 
 import numpy as np
+import pytest
 
 from firtez_dz_amd import modelio
 
@@ -41,3 +42,30 @@ def test_rf_round_trip(tmp_path):
     assert np.array_equal(lv2, logv)
     assert np.array_equal(d2[logv], d[logv]) and np.all(d2[~logv] == 0)
     assert np.allclose(z2, z)
+
+
+def test_lsf_file_round_trip_and_layout(tmp_path):
+    """LSF file (READ_LSF_FILE, misc.f90:2422-2555): header [SS+VV, SS-VV,
+    3330904, nrec, 1, nw] then the samples; the bytes are checked against a
+    file assembled by hand from that layout, and lsf_kernel takes the samples
+    of a region that has no Gaussian width."""
+    import struct
+
+    from firtez_dz_amd import lines
+    y = np.exp(-0.5 * ((np.arange(61) - 30) / 3.0) ** 2)
+    y = np.roll(y / y.sum(), -30)
+    p = tmp_path / "lsf.bin"
+    modelio.write_lsf(str(p), y)
+    hdr = np.array([3003, 2997, 3330904, 2, 1, 61], np.float32)
+    body = y.astype(np.float32)
+    ref = (struct.pack("<i", 24) + hdr.tobytes() + struct.pack("<i", 24) +
+           struct.pack("<i", 244) + body.tobytes() + struct.pack("<i", 244))
+    assert p.read_bytes() == ref
+    back = modelio.read_lsf(str(p), nwave=61)
+    assert np.array_equal(back, body.astype(np.float64))
+    with pytest.raises(ValueError):
+        modelio.read_lsf(str(p), nwave=60)
+    ls = lines.setup([((4,), 61, -600.0, 20.0), ((5,), 52, -520.0, 20.0)])
+    k, v = lines.lsf_kernel(ls, [-1, 30.0], [-1, 0.0], taps={0: back})
+    assert list(v) == [1, 1]
+    assert np.allclose(k[:61] * 61, back / back.sum(), rtol=1e-12)
