@@ -169,6 +169,14 @@ def main():
     fl = flops.per_column_lambda(args.nz, 2, ncomp, npar)
     # per launch: the tile's columns x wavelengths x flop, over the average launch time
     launch_ms = rt_avg_ms / nlaunch
+    # HBM bytes of rt_kernel<1> per column, from the FETCH_SIZE and WRITE_SIZE
+    # counters of one launch of 9216 columns of this workload (nz = 96, 112
+    # wavelengths, spherical Jacobian), each scaled by the counter's reading for
+    # a 256 MiB device copy (FETCH_SIZE reads half the copied bytes: x2;
+    # WRITE_SIZE exact: x1): 3.53 GB read + 9.53 GB written
+    traffic = None
+    if args.nz == 96 and numw == 112:
+        traffic = 1.4178e6 * ncol / nlaunch
     achieved = fl * ncol * numw / (rt_avg_ms * 1e-3) / 1e12
     print("[bench] step %.1f ms; micro %.1f ms; rt %.1f ms (%d launches); %.0f kflop/column-lambda"
           % (ms_step, np.mean(micro_ms), rt_avg_ms, nlaunch, fl / 1e3), file=sys.stderr, flush=True)
@@ -185,7 +193,7 @@ def main():
         # compute-bound: FP32 arithmetic on the vector ALUs (no matrix work in
         # this kernel); priced against MI355X's 157.3 TFLOP/s FP32 peak
         "roofline": {"bound": "mfma", "pipe": "fp32 valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "kernel": "rt_kernel<1>", "launch_ms": launch_ms, "launches_per_step": nlaunch,
                      "flop_per_column_lambda": fl},
     }

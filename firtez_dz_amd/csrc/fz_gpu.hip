@@ -122,6 +122,104 @@ __global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
   }
 }
 
+// GET_RHO for one column by a group of RHO_G lanes: lane q holds the Saha
+// factors of elements q, q + RHO_G, ... of the current height in registers
+// (loaded once per height, not once per iteration), evaluates their stage
+// populations, and every lane of the group folds the 91 terms in the
+// reference's order j = 1 .. 91 through cross-lane reads, so the fixed point
+// and its iteration path are those of get_rho bit for bit.  Eight lanes per
+// column give eight times the waves of one lane per column for the same
+// cube: the height recurrence is sequential, so the column count alone
+// cannot fill the SIMDs.
+constexpr int RHO_G = 8;
+constexpr int RHO_NS = (FZ_NELEM + RHO_G - 1) / RHO_G;
+
+__device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a2)[RHO_NS], int q, int gbase,
+                                    double T, double PG, double& ne, double& nh, double& pelec, double& dens,
+                                    double& mu) {
+  const double kT = KBOL * T;
+  double n = ne;
+  if (n != n) n = 0.0;
+  if (n < 1.0) n = PG / (101.0 * kT);
+  pelec = n * kT;
+  if (pelec > PG) n = (PG * 0.5) / kT;
+  double err = 1.0, nnew = n, h = 0.0;
+  int iter = 1;
+  while (err > 1.0e-4 && iter <= 100 && h >= 0.0) {
+    const double nold = nnew;
+    h = (PG - nnew * kT) / (g_c.deno_sa * kT);
+    double s2 = 0.0, s3 = 0.0, ni0 = 0.0, niii0 = 0.0;
+#pragma unroll
+    for (int m = 0; m < RHO_NS; ++m) {
+      const int j = q + RHO_G * m;
+      double ni, nii, niii;
+      stages(h * g_c.abfac[j < FZ_NELEM ? j : 0], nnew, a1[m], a2[m], ni, nii, niii);
+      if (m == 0) {
+        ni0 = __shfl(ni, gbase);
+        niii0 = __shfl(niii, gbase);
+      }
+#pragma unroll
+      for (int r = 0; r < RHO_G; ++r) {
+        const int jj = RHO_G * m + r;
+        if (jj == 0 || jj >= FZ_NELEM) continue;
+        s2 += __shfl(nii, gbase + r);
+        s3 += __shfl(niii, gbase + r);
+      }
+    }
+    double est = niii0 - ni0;
+    est = est + s2 + s3;
+    nnew = (fmax(est, 0.0) + nold) / 2.0;
+    err = fabs((nnew - nold) / nnew);
+    ++iter;
+  }
+  if (iter >= 100 || h < 0.0) return ERR_EOS;
+  n = nnew;
+  h = (PG - n * kT) / (g_c.deno_sa * kT);
+  dens = h * g_c.deno_sam + n * MELE;
+  mu = ((dens * kT) / MAMU) * (1.0 / PG);
+  if (h != h || n != n) return ERR_NAN;
+  if (n < 0.0 || h <= 0.0) return ERR_EOS;
+  ne = n;
+  nh = h;
+  return ERR_OK;
+}
+
+__global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane & (RHO_G - 1), gbase = lane & ~(RHO_G - 1);
+  const int ct = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / RHO_G);
+  const bool active = ct < a.ntile;
+  // a group past the tile repeats the last column without writing, so that
+  // every group of a wave runs the same code
+  const int cc = active ? ct : a.ntile - 1;
+  const int nz = c_cfg.nz, col = a.col0 + cc;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  double ne = 0.0;
+  int err = ERR_OK;
+  for (int k = 0; k < nz; ++k) {
+    const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
+    const double* sb = a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + cc;
+    double a1[RHO_NS], a2[RHO_NS];
+#pragma unroll
+    for (int m = 0; m < RHO_NS; ++m) {
+      const int j = q + RHO_G * m;
+      a1[m] = j < FZ_NELEM ? sb[(long)(j * 4 + 0) * a.ntile] : 1.0;
+      a2[m] = j < FZ_NELEM ? sb[(long)(j * 4 + 1) * a.ntile] : 1.0;
+    }
+    double nh, pel, dens, mu;
+    err = get_rho_group(a1, a2, q, gbase, T, P, ne, nh, pel, dens, mu);
+    if (err) break;
+    if (active && q == 0) {
+      a.atm[A_RHO * ps + k * cs + col] = (float)dens;
+      a.atm[A_MU * ps + k * cs + col] = (float)mu;
+      a.atm[A_PEL * ps + k * cs + col] = (float)pel;
+      a.eos[(long)k * a.ntile + cc] = ne;
+      a.eos[((long)nz + k) * a.ntile + cc] = nh;
+    }
+  }
+  if (active && q == 0) a.status[col] = err;
+}
+
 __global__ void __launch_bounds__(64) rho_kernel(MicroTile a) {
   const int ct = blockIdx.x * blockDim.x + threadIdx.x;
   if (ct >= a.ntile) return;
@@ -1321,7 +1419,8 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
                          (double*)c->d_smid);
       hipLaunchKernelGGL(he_rk4_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a, (const double*)c->d_smid);
     } else {
-      hipLaunchKernelGGL(rho_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a);
+      hipLaunchKernelGGL(rho_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + 255) / 256)), dim3(256), 0, st,
+                         a);
     }
     if (physics) {
       if (a.der) hipLaunchKernelGGL(deriv_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
