@@ -307,3 +307,28 @@ def test_lsf_forward_vs_oracle(gpu_ctx, cpu):
     finally:
         gpu_ctx.set_lsf(None)
         cpu.set_lsf(None)
+
+
+def test_eos_outputs_vs_oracle(gpu_ctx, cpu):
+    """The forward's equation-of-state outputs written back to the model (rho,
+    Pe, mu: the electron-density fixed point with its warm start from the
+    height below) against the oracle on seeded columns.  The fixed point runs
+    in the reference's operation order, so agreement is at FP64 rounding."""
+    import torch
+    ls = lines.fe6301_6302()
+    atm, z = atmos.seeded(6, 5, 64, seed=17)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t).cpu().numpy() == 0)
+    atm_in = t.cpu().numpy()
+    atm_out, _, _, status = run_gpu(gpu_ctx, atm_in, 0)
+    assert np.all(status == 0)
+    cpu.setup(ls, resp=False, atmpar=CART)
+    cpu.set_gain(gain)
+    worst = 0.0
+    for c in range(atm.shape[2]):
+        mo, _, _, _ = cpu.forward1d(atmos.column_model(atm_in, z, c))
+        for row in (2, 3, 4):
+            worst = max(worst, float(np.max(np.abs(atm_out[row, :, c] - mo[row]) / np.abs(mo[row]))))
+    report("EOS rho/Pe/mu vs oracle (relative, 30 columns)", worst, 1e-12)
