@@ -184,6 +184,29 @@ __device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a
   return ERR_OK;
 }
 
+// get_rho_group on the factors of one height (base: [element][4][column]),
+// out of line so that a caller with several fixed points (the RK4 steps)
+// does not carry one unrolled copy of the element loop per call
+struct RhoOut {
+  double ne, nh, pel, dens, mu;
+  int err;
+};
+
+__device__ __noinline__ RhoOut rho_group_at(const double* __restrict__ base, long stride, int q, int gbase, double T,
+                                            double PG, double ne) {
+  double a1[RHO_NS], a2[RHO_NS];
+#pragma unroll
+  for (int m = 0; m < RHO_NS; ++m) {
+    const int j = q + RHO_G * m;
+    a1[m] = j < FZ_NELEM ? base[(long)(j * 4 + 0) * stride] : 1.0;
+    a2[m] = j < FZ_NELEM ? base[(long)(j * 4 + 1) * stride] : 1.0;
+  }
+  RhoOut o;
+  o.ne = ne;
+  o.err = get_rho_group(a1, a2, q, gbase, T, PG, o.ne, o.nh, o.pel, o.dens, o.mu);
+  return o;
+}
+
 __global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
   const int lane = threadIdx.x & 63;
   const int q = lane & (RHO_G - 1), gbase = lane & ~(RHO_G - 1);
@@ -380,6 +403,73 @@ __device__ __forceinline__ double he_step_rhs(const S& s, double lnp, double tem
   const int e = get_rho(s, tem, exp(lnp), nel, nh, pel, dens, mw);
   if (e) err = e;
   return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
+}
+
+__device__ __forceinline__ double he_step_rhs_group(const double* base, long stride, int q, int gbase, double lnp,
+                                                    double tem, double& nel, int& err) {
+  const RhoOut o = rho_group_at(base, stride, q, gbase, tem, exp(lnp), nel);
+  nel = o.ne;
+  if (o.err) err = o.err;
+  return -o.mu * MAMU * GRAV / (KBOL * tem);        // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
+}
+
+// RK4_INTEGRATION_LOG by groups of RHO_G lanes per column, each fixed point as
+// get_rho_group (same values as he_rk4_kernel's get_rho, bit for bit); the
+// integration scalars are identical in the lanes of a group.
+__global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const double* __restrict__ smid) {
+  const int lane = threadIdx.x & 63;
+  const int q = lane & (RHO_G - 1), gbase = lane & ~(RHO_G - 1);
+  const int ct = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / RHO_G);
+  const bool active = ct < a.ntile;
+  const int cc = active ? ct : a.ntile - 1;
+  const bool wr = active && q == 0;
+  const int nz = c_cfg.nz, col = a.col0 + cc;
+  const long cs = a.ncol, ps = (long)nz * a.ncol;
+  const long hs = (long)FZ_NELEM * 4 * a.ntile;
+  const double* SH = a.sbuf + cc;
+  const double* SM = smid + cc;
+  auto A = [&](int p, int k) -> float& { return a.atm[p * ps + k * cs + col]; };
+  int err = 0;
+  const int top = nz - 1;
+  double u0 = log((double)A(A_PG, top));
+  double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
+  double nh, pel, dens, mw;
+  {
+    const RhoOut o = rho_group_at(SH + top * hs, a.ntile, q, gbase, (double)A(A_T, top), exp(u0), nel);
+    nel = o.ne, nh = o.nh, pel = o.pel, dens = o.dens, mw = o.mu, err = o.err;
+  }
+  if (wr) {
+    A(A_PG, top) = (float)exp(u0);
+    A(A_RHO, top) = (float)dens;
+    A(A_PEL, top) = (float)pel;
+    A(A_MU, top) = (float)mw;
+    if (a.eos) a.eos[((long)nz + top) * a.ntile + cc] = nh;
+  }
+  for (int k = nz - 2; k >= 0 && !err; --k) {
+    const double dz = c_cfg.he_dz[k];
+    const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
+    const double k1 = he_step_rhs_group(SH + (k + 1) * hs, a.ntile, q, gbase, u0, tm0, nel, err) * dz;
+    const double tmid = he_tmid(k, tm0, tm1);
+    const double k2 = he_step_rhs_group(SM + k * hs, a.ntile, q, gbase, u0 + k1 / 2.0, tmid, nel, err) * dz;
+    const double k3 = he_step_rhs_group(SM + k * hs, a.ntile, q, gbase, u0 + k2 / 2.0, tmid, nel, err) * dz;
+    const double k4 = he_step_rhs_group(SH + k * hs, a.ntile, q, gbase, u0 + k3, tm1, nel, err) * dz;
+    const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
+    const RhoOut o = rho_group_at(SH + k * hs, a.ntile, q, gbase, tm1, exp(u1), nel);
+    nel = o.ne, nh = o.nh, pel = o.pel, dens = o.dens, mw = o.mu;
+    if (o.err) err = o.err;
+    if (wr) {
+      A(A_PG, k) = (float)exp(u1);
+      A(A_RHO, k) = (float)dens;
+      A(A_PEL, k) = (float)pel;
+      A(A_MU, k) = (float)mw;
+      if (a.eos) a.eos[((long)nz + k) * a.ntile + cc] = nh;
+    }
+    u0 = u1;
+  }
+  if (a.eos && wr)
+    for (int k = 0; k < nz; ++k)
+      a.eos[(long)k * a.ntile + cc] = (double)A(A_PEL, k) / KBOL / (double)A(A_T, k);
+  if (wr) a.status[col] = err;
 }
 
 // writes Pg, rho, Pe, mu of every height and, for the forward, (n_e, N_H)
@@ -1417,7 +1507,8 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
       const long nmt = (long)(nz - 1) * a.ntile;
       hipLaunchKernelGGL(saha_mid_kernel, dim3((unsigned)((nmt + 255) / 256)), dim3(256), 0, st, a,
                          (double*)c->d_smid);
-      hipLaunchKernelGGL(he_rk4_kernel, dim3((a.ntile + 63) / 64), dim3(64), 0, st, a, (const double*)c->d_smid);
+      hipLaunchKernelGGL(he_rk4_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + 255) / 256)), dim3(256), 0,
+                         st, a, (const double*)c->d_smid);
     } else {
       hipLaunchKernelGGL(rho_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + 255) / 256)), dim3(256), 0, st,
                          a);
