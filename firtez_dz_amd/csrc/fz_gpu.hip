@@ -748,7 +748,9 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   a.jac = jac;
   a.jcol0 = jcol0;
   a.scratch = scratch;
-  constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : 0);
+  // parameters: 5 spherical, 7 Cartesian, 8 with P0 (the top pressure)
+  constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : (RF == FZ_RF_CARTESIAN_P0 ? 8 : 0));
+  constexpr int NPL = RF == FZ_RF_CARTESIAN_P0 ? 7 : NP;    // parameters with a local response
   const int numw = c_cfg.numw, nz = c_cfg.nz;
   // workgroup -> (column, region, first wavelength); lane -> wavelength
   const int ct = (int)(blockIdx.x / (unsigned)c_cfg.nbpc);
@@ -868,7 +870,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
         constexpr int dirs_sph[5] = {D_T, D_B, D_G, D_F, D_V};
         constexpr int dirs_car[7] = {D_T, D_P, D_R, D_B, D_G, D_F, D_V};
 #pragma unroll
-        for (int q = 0; q < NP; ++q) {
+        for (int q = 0; q < NPL; ++q) {
           const int dd = RF == FZ_RF_SPHERICAL ? dirs_sph[q] : dirs_car[q];
           if (HEX && dd != D_T && dd != D_P) evol_dir(E, dx[dd], dz, vo, m4vo, loc[dd]);
           else evol_dir(E, dx[dd], dz, v, m4v, loc[dd]);
@@ -882,7 +884,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
           for (int i = 0; i < 4; ++i) loc[D_T][i] += gk * scr[(long)((nz + k) * 4 + i) * sstr];
         }
-        float lp[NP][4];
+        float lp[NPL][4];
         if constexpr (RF == FZ_RF_SPHERICAL) {
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
@@ -917,10 +919,25 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
           }
         }
 #pragma unroll
-        for (int p = 0; p < NP; ++p)
+        for (int p = 0; p < NPL; ++p)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
             outv[p][i] = acc[i][0] * lp[p][0] + acc[i][1] * lp[p][1] + acc[i][2] * lp[p][2] + acc[i][3] * lp[p][3];
+        if constexpr (RF == FZ_RF_CARTESIAN_P0) {
+          // P0 (forward.f90:543-545): the local response is 1 in every Stokes
+          // parameter; below the top it is propagated and then multiplied,
+          // component by component, by PG(K)/PG(NZ) and the propagated Pg|T
+          // response of the same height; the top layer keeps 1
+          if (k == nz - 1) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) outv[7][i] = 1.f;
+          } else {
+            const float ratio = a.atm[A_PG * ps + k * cs + col] / a.atm[A_PG * ps + (nz - 1) * cs + col];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              outv[7][i] = (acc[i][0] + acc[i][1] + acc[i][2] + acc[i][3]) * ratio * outv[1][i];
+          }
+        }
         // running product O_nz ... O_k
         float nacc[4][4];
 #pragma unroll
@@ -1431,12 +1448,20 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   return FZ_OK;
 }
 
+static int rf_nparam(int rf_mode) {
+  return rf_mode == FZ_RF_SPHERICAL ? 5 : rf_mode == FZ_RF_CARTESIAN ? 7 : rf_mode == FZ_RF_CARTESIAN_P0 ? 8 : 0;
+}
+
 static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
   const dim3 grid((unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
 #define FZ_RT_ARGS a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0
   if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro && rf_mode == FZ_RF_SPHERICAL)
     hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (!c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, true>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro) hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (rf_mode == FZ_RF_SPHERICAL)
     hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, true>), grid, blk, 0, st, FZ_RT_ARGS);
@@ -1540,7 +1565,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
                    void* stream) {
   if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
   if (rf_mode != FZ_RF_NONE && !d_jac) return set_err(c, FZ_EARG, "jacobian output missing");
-  if (rf_mode < 0 || rf_mode > 2) return FZ_EARG;
+  if (rf_mode < 0 || rf_mode > 3) return FZ_EARG;
   const Cfg& g = c->cfg;
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1579,7 +1604,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
   r = run_lsf(c, d_stokes, (long)ncol * 4, st);
   if (r) return r;
   if (rf_mode != FZ_RF_NONE) {
-    r = run_lsf(c, d_jac, (long)ncol * (rf_mode == FZ_RF_SPHERICAL ? 5 : 7) * g.nz * 4, st);
+    r = run_lsf(c, d_jac, (long)ncol * rf_nparam(rf_mode) * g.nz * 4, st);
     if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
@@ -1688,7 +1713,7 @@ int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rh
   if (!c || !c->configured || nx < 1 || ny < 1 || !syn3d) return FZ_EARG;
   const Cfg& g = c->cfg;
   const int nz = g.nz, numw = g.numw;
-  const int np = rf_mode == FZ_RF_SPHERICAL ? 5 : (rf_mode == FZ_RF_CARTESIAN ? 7 : 0);
+  const int np = rf_nparam(rf_mode);
   if (np && !dsyn) return FZ_EARG;
   // columns still to be computed (AM_I_DONE == 0, forward.f90:120-140)
   std::vector<int> cols;

@@ -19,10 +19,11 @@ from . import lines as _lines
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libfirtez_gpu.so")
 
-RF_NONE, RF_SPHERICAL, RF_CARTESIAN = 0, 1, 2
-NPARAM = {RF_NONE: 0, RF_SPHERICAL: 5, RF_CARTESIAN: 7}
+RF_NONE, RF_SPHERICAL, RF_CARTESIAN, RF_CARTESIAN_P0 = 0, 1, 2, 3
+NPARAM = {RF_NONE: 0, RF_SPHERICAL: 5, RF_CARTESIAN: 7, RF_CARTESIAN_P0: 8}
 PARAM_NAMES = {RF_SPHERICAL: ("T", "B", "gamma", "phi", "vlos"),
-               RF_CARTESIAN: ("T", "Pg", "rho", "Bx", "By", "Bz", "vz")}
+               RF_CARTESIAN: ("T", "Pg", "rho", "Bx", "By", "Bz", "vz"),
+               RF_CARTESIAN_P0: ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "P0")}
 # device atmosphere rows (d_atm[12][nz][ncol])
 ATM = ("T", "Pg", "rho", "Pe", "mu", "Bx", "By", "Bz", "vx", "vy", "vz", "tau5")
 

@@ -31,7 +31,9 @@ enum {
 enum {
   FZ_RF_NONE = 0,
   FZ_RF_SPHERICAL = 1, /* d/d(T|Pg, |B|, inclination, azimuth, v_los) at every height */
-  FZ_RF_CARTESIAN = 2  /* d/d(T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z): DSYN1D slots 1-7 (rtesolver.f90:209-258) */
+  FZ_RF_CARTESIAN = 2,  /* d/d(T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z): DSYN1D slots 1-7 (rtesolver.f90:209-258) */
+  FZ_RF_CARTESIAN_P0 = 3 /* slots 1-7 and P0 (slot 8): local response 1, propagated, then times
+                            PG(K)/PG(NZ) and the propagated Pg|T response (forward.f90:543-545) */
 };
 
 /* One row of the line table.  Replaces the FORWARD_PARAM per-line arrays
@@ -89,7 +91,7 @@ int fz_set_lsf(fz_ctx* ctx, const double* lsf_kernel, const int32_t* lsf_valid);
  *            vy, vz, log tau5 (rho, Pe, mu, log tau5 are updated like the
  *            reference updates the cube)
  *   d_stokes [ncol][4][numw] float: SYN1D*GAIN1D
- *   d_jac    [ncol][np][nz][4][numw] float (np = 5 spherical, 7 cartesian), the
+ *   d_jac    [ncol][np][nz][4][numw] float (np = 5 spherical, 7 cartesian, 8 with P0), the
  *            propagated response functions times GAIN1D; NULL if rf_mode=NONE
  *   d_status [ncol] int32 or NULL: per-column FZ_E* code
  *   stream   hipStream_t (NULL = default stream); the call is asynchronous. */

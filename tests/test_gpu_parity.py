@@ -207,24 +207,26 @@ def test_forward3d_host_mirror(gpu_ctx, golden):
 def test_hydrostatic_mode_vs_reference(gpu_ctx, golden):
     """HYDROSTATIC: yes against the reference's own output (tests/test_he,
     test_syn2 set-up with a field and a flow): the re-integrated model, Stokes
-    and the seven Cartesian response functions, including the hydrostatic T
-    term and the outgoing-vector quirk of the rho, B and v responses."""
+    and all eight response functions of DSYN1D, including the hydrostatic T
+    term, the outgoing-vector quirk of the rho, B and v responses and the P0
+    (top pressure) response."""
     g = golden["he"]
     ls = lines.setup([((4, 5), int(g["nwave"]), float(g["wave_start"]), float(g["wave_step"]))])
     gpu_ctx.configure(ls, g["model_in"][12].astype(np.float64))
     gpu_ctx.set_gain(g["gain"])
     gpu_ctx.set_hydrostatic(True)
     try:
-        atm_out, st, jac, status = run_gpu(gpu_ctx, model_rows(g["model_in"][None]), 2)
+        atm_out, st, jac, status = run_gpu(gpu_ctx, model_rows(g["model_in"][None]), 3)
     finally:
         gpu_ctx.set_hydrostatic(False)
     assert status[0] == 0
+    assert jac.shape[1] == 8
     for row, name in ((1, "Pg"), (2, "rho"), (3, "Pe"), (4, "mu")):
         report("HE model %s (relative)" % name,
                np.max(np.abs(atm_out[row, :, 0] - g["model_out"][row]) / np.abs(g["model_out"][row])), 1e-6)
     report("HE Stokes vs reference", np.max(np.abs(st[0] - g["stokes"])), STOKES_TOL)
-    names = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz")
-    for p in range(7):
+    names = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "P0")
+    for p in range(8):
         ours = jac[0, p][g["rf_heights"]]
         ref = np.transpose(g["rf"][:, :, p, :], (0, 2, 1))
         sc = np.abs(ref).max()
@@ -332,3 +334,27 @@ def test_eos_outputs_vs_oracle(gpu_ctx, cpu):
         for row in (2, 3, 4):
             worst = max(worst, float(np.max(np.abs(atm_out[row, :, c] - mo[row]) / np.abs(mo[row]))))
     report("EOS rho/Pe/mu vs oracle (relative, 30 columns)", worst, 1e-12)
+
+
+def test_p0_response_vs_oracle(gpu_ctx, cpu):
+    """P0 (slot 8 of DSYN1D, forward.f90:543-545) without HYDROSTATIC on
+    seeded columns: the propagated unit response times PG(K)/PG(NZ) and the
+    propagated Pg|T response, the top layer 1, against the oracle."""
+    import torch
+    ls = lines.fe6301_6302()
+    atm, z = atmos.seeded(3, 3, 64, seed=23)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t).cpu().numpy() == 0)
+    atm_in = t.cpu().numpy()
+    _, st, jac, status = run_gpu(gpu_ctx, atm_in, 3)
+    assert np.all(status == 0) and jac.shape[1] == 8
+    cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 1))
+    cpu.set_gain(gain)
+    for c in (0, 4, 8):
+        _, syn, dsyn, _ = cpu.forward1d(atmos.column_model(atm_in, z, c))
+        for p in (1, 7):
+            ref = np.transpose(dsyn[:, :, p, :], (0, 2, 1))
+            report("P0 mode col %d RF p%d" % (c, p), np.max(np.abs(jac[c, p] - ref)) / np.abs(ref).max(), RF_TOL)
+        report("P0 mode col %d Stokes" % c, np.max(np.abs(st[c] - syn)), STOKES_TOL)
