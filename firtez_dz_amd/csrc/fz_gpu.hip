@@ -1168,6 +1168,7 @@ struct fz_ctx {
   size_t derb_cap = 0;
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
+  bool lm_p0 = false;         // P0 among the inverted parameters
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
   MlpState mlp;
 };
@@ -1838,7 +1839,6 @@ int fz_int_coefs(int nz, int nj, double* coefs) {
 int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq) {
   if (!c || !inv) return FZ_EARG;
   if (!c->configured) return set_err(c, FZ_ECONFIG, "fz_configure first");
-  if (inv->atmpar[7]) return set_err(c, FZ_ECONFIG, "P0 inversion is not supported");
   const int nz = c->cfg.nz, numw = c->cfg.numw;
   LmArgs& L = c->lm;
   L = LmArgs();
@@ -1858,7 +1858,7 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
     if (nj < 1 || nj > nz) return set_err(c, FZ_ECONFIG, "nodes must be in 1..nz");
     if (np + nj > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters");
     L.par[npar] = p;
-    L.slot[npar] = p;  // Cartesian slots: T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z
+    L.slot[npar] = p;  // Cartesian slots: T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z, P0
     L.mcol[npar] = mcol[p];
     L.j0[npar] = np;
     L.nj[npar] = nj;
@@ -1916,6 +1916,7 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   L.coefs = c->d_coefs;
   L.colsum = c->d_colsum;
   c->inv_set = true;
+  c->lm_p0 = inv->atmpar[7] != 0;
   if (ifreep) *ifreep = np;
   if (nfreq) *nfreq = L.nfreq;
   return FZ_OK;
@@ -1935,7 +1936,10 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   const int tile = std::min(c->tile, ncol);
   r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * tile * g.numw);
   if (r) return r;
-  r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * 7 * g.nz * 4 * g.numw);
+  // the P0 slot is synthesised only when P0 is inverted
+  const bool p0 = c->lm_p0;
+  const int nslot = p0 ? 8 : 7;
+  r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * nslot * g.nz * 4 * g.numw);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
   r = run_micro(c, ncol, d_atm, &d_status, true, c->hydro, true, st);
@@ -1948,6 +1952,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   e.nfreq = c->lm.nfreq;
   e.np = c->lm.np;
   e.nstk = c->lm.nfreq / g.numw;
+  e.nslot = nslot;
   for (int s = 0; s < 4; ++s) e.rowstk[s] = c->lm.rowstk[s];
   for (int j = 0; j < e.np; ++j) {
     e.slot_of[j] = c->slot_of[j];
@@ -1969,7 +1974,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     a.jac = c->d_jtile;
     a.jcol0 = c0;
     a.scratch = c->d_scratch;
-    r = launch_rt(c, FZ_RF_CARTESIAN, a, st);
+    r = launch_rt(c, p0 ? FZ_RF_CARTESIAN_P0 : FZ_RF_CARTESIAN, a, st);
     if (r) return r;
     e.ncol_tile = a.ntile;
     e.col0 = c0;
@@ -2064,7 +2069,6 @@ struct DevBuf {
 int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
                     float* d_stokes, double* d_chi2, int* iterations, void* stream) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
-  if (ctl->atmpar[7]) return set_err(c, FZ_ECONFIG, "P0 inversion is not supported");
   const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(c, hipSetDevice(c->device));

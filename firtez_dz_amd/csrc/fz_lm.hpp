@@ -61,11 +61,12 @@ struct LmArgs {
 // ---------------------------------------------------------------- node contraction
 struct EqvArgs {
   int ncol_tile, col0, nz, numw, nfreq, np, nstk;
+  int nslot;                   // response-function slots per column in the tile (7, or 8 with P0)
   int rowstk[4];
   int slot_of[LM_MAXNP];       // response-function slot of each free parameter
   int kbeg[LM_MAXNP], kend[LM_MAXNP];
   const double* coefs;         // [np][nz]
-  const float* jac;            // tile: [ntile][7][nz][4][numw]
+  const float* jac;            // tile: [ntile][nslot][nz][4][numw]
   float* eqv;                  // [ncol][np][nfreq]
 };
 
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
   const int b = f / a.numw, l = f - b * a.numw;
   const int s = a.rowstk[b];
   const long cs = (long)4 * a.numw;
-  const float* jc = a.jac + (long)ct * 7 * a.nz * cs + s * a.numw + l;
+  const float* jc = a.jac + (long)ct * a.nslot * a.nz * cs + s * a.numw + l;
   float* out = a.eqv + ((long)(a.col0 + ct) * a.np) * a.nfreq + f;
   for (int j = 0; j < a.np; ++j) {
     const float* jp = jc + (long)a.slot_of[j] * a.nz * cs;

@@ -205,7 +205,7 @@ int fz_lm_step_dev(fz_ctx* ctx, int ncol, const double* d_obs, const float* d_st
  * SETUP sections of the control file, input_file.f90:1064-1135, 1280-1303,
  * 170-190). */
 typedef struct {
-  int32_t atmpar[8];     /* inverted parameters (T, Pg, rho, Bx, By, Bz, v_z; P0 unsupported) */
+  int32_t atmpar[8];     /* inverted parameters (T, Pg, rho, Bx, By, Bz, v_z, P0) */
   int32_t nslb_max[8];   /* maximum nodes of each parameter (NSLB_MAX, "TEM 4") */
   int32_t invstk[4];     /* inverted Stokes parameters */
   float wstk[4];         /* Stokes weights before normalisation (STKI 1 ...) */

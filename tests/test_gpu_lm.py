@@ -357,3 +357,79 @@ def test_invert3d_host_arrays_match_device_path(gpu_ctx, cpu):
         assert np.array_equal(back[row], dev[row]), row
     assert np.array_equal(chi3d.T.reshape(-1), chi_d.cpu().numpy())
     assert np.array_equal(syn3d.transpose(2, 1, 0).reshape(ncol, -1), st_d.cpu().numpy().reshape(ncol, -1))
+
+
+def test_p0_inversion_step_vs_oracle(gpu_ctx, cpu):
+    """P0 among the inverted parameters (HYDROSTATIC: yes): the node-contracted
+    response functions include the P0 column (DSYN1D slot 8 through
+    P0_COEFS) and one LM step perturbs the Pg column through it
+    (REARANGE_DA_MODEL's P0 case, get_dmodel.f90:1957-1959), as in the oracle."""
+    import torch
+    nodes = {"T": 2, "vz": 1, "P0": 1}
+    ls = lines.fe6301_6302()
+    nx = ny = 2
+    nz = 64
+    atm, z = atmos.seeded(nx, ny, nz, seed=31)
+    gpu_ctx.configure(ls, z)
+    gain = gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t).cpu().numpy() == 0)
+    truth = t.cpu().numpy()
+    ncol = nx * ny
+    atmpar = (1, 0, 0, 0, 0, 0, 1, 1)
+    cpu.setup(ls, resp=True, atmpar=atmpar)
+    cpu.set_gain(gain)
+    cpu.set_hydrostatic(True)
+    gpu_ctx.set_hydrostatic(True)
+    try:
+        numw = gpu_ctx.numw
+        rng = np.random.default_rng(5)
+        obs = []
+        for c in range(ncol):
+            _, syn, _, _ = cpu.forward1d(atmos.column_model(truth, z, c))
+            obs.append(syn.reshape(-1) + rng.normal(0, NOISE, syn.size))
+        obs = np.array(obs)
+        guess = truth.copy()
+        guess[0] += np.float32(120.0)
+        guess[1, -1] *= np.float32(1.3)          # top pressure off by 30 %
+        inv = gpu.inversion(nodes)
+        ifreep, nfreq = gpu_ctx.set_inversion(inv)
+        cfg = oracle.lm_cfg(nz, numw, list(atmpar), [2, 0, 0, 0, 0, 0, 1, 1])
+        coefs = cpu.coefs_block(cfg, {0: cpu.int_coefs(nz, 2), 6: cpu.int_coefs(nz, 1), 7: cpu.int_coefs(nz, 1)})
+        d_atm = torch.from_numpy(guess.copy()).cuda()
+        st, eqv, status = gpu_ctx.forward_nodes_torch(d_atm)
+        torch.cuda.synchronize()
+        assert np.all(status.cpu().numpy() == 0) and eqv.shape[1] == 4
+        h_st = st.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
+        h_eqv, h_cur = eqv.cpu().numpy(), d_atm.cpu().numpy()
+        worst = 0.0
+        for c in range(ncol):
+            _, syn, dsyn, _ = cpu.forward1d(atmos.column_model(h_cur, z, c))
+            ref = cpu.eqv_dsyn(cfg, coefs, dsyn)
+            for j in range(ifreep):
+                worst = max(worst, float(np.max(np.abs(h_eqv[c, j] - ref[:, j])) / np.abs(ref[:, j]).max()))
+        report("EQVDSYN with P0 (HE) vs oracle", worst, RF_TOL)
+        d = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
+        sla0 = np.tile(np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]), (ncol, 1))
+        sla, bsyn = d(sla0), d(np.zeros((ncol, nfreq)))
+        ebest = torch.zeros_like(eqv)
+        best = d_atm.clone()
+        done = torch.zeros(ncol, dtype=torch.int32, device=d_atm.device)
+        delta = torch.zeros((ncol, ifreep), dtype=torch.float64, device=d_atm.device)
+        isig = np.full(nfreq, 1.0 / NOISE ** 2)
+        gpu_ctx.lm_step_torch(d(obs), st, bsyn, sla, eqv, ebest, d(isig), d_atm, best, done, delta)
+        torch.cuda.synchronize()
+        g_cur, g_del = d_atm.cpu().numpy(), delta.cpu().numpy()
+        werr = dict(delta=0.0, pg=0.0)
+        for c in range(ncol):
+            mc = atmos.column_model(h_cur, z, c)
+            r = cpu.lm_step(cfg, coefs, obs[c], h_st[c], np.zeros(nfreq), sla0[c], h_eqv[c].T,
+                            np.zeros_like(h_eqv[c]).T, isig, mc, mc.copy())
+            m_new, dl = r[0], r[5]
+            werr["delta"] = max(werr["delta"], float(np.max(np.abs(g_del[c] - dl) / (np.abs(dl) + 1e-12))))
+            werr["pg"] = max(werr["pg"], float(np.max(np.abs(g_cur[1, :, c] - m_new[1]) / np.abs(m_new[1]))))
+        report("LM step with P0: node perturbation vs oracle", werr["delta"], 1e-6)
+        report("LM step with P0: new Pg column vs oracle (relative)", werr["pg"], 2e-6)
+    finally:
+        gpu_ctx.set_hydrostatic(False)
+        cpu.set_hydrostatic(False)
