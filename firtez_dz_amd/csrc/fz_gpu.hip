@@ -134,7 +134,13 @@ __global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
 constexpr int RHO_G = 8;
 constexpr int RHO_NS = (FZ_NELEM + RHO_G - 1) / RHO_G;
 
-__device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a2)[RHO_NS], int q, int gbase,
+// sh: this group's LDS rows, nii of element j at sh[j], niii at sh[RHO_LD + j],
+// ni of element 0 at sh[FZ_NELEM].  The lanes of a group belong to one wave,
+// whose LDS operations complete in program order, so the fold reads what the
+// stage pass wrote without a workgroup barrier.
+constexpr int RHO_LD = 96;
+
+__device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a2)[RHO_NS], int q, double* sh,
                                     double T, double PG, double& ne, double& nh, double& pelec, double& dens,
                                     double& mu) {
   const double kT = KBOL * T;
@@ -148,25 +154,27 @@ __device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a
   while (err > 1.0e-4 && iter <= 100 && h >= 0.0) {
     const double nold = nnew;
     h = (PG - nnew * kT) / (g_c.deno_sa * kT);
-    double s2 = 0.0, s3 = 0.0, ni0 = 0.0, niii0 = 0.0;
 #pragma unroll
     for (int m = 0; m < RHO_NS; ++m) {
       const int j = q + RHO_G * m;
-      double ni, nii, niii;
-      stages(h * g_c.abfac[j < FZ_NELEM ? j : 0], nnew, a1[m], a2[m], ni, nii, niii);
-      if (m == 0) {
-        ni0 = __shfl(ni, gbase);
-        niii0 = __shfl(niii, gbase);
-      }
-#pragma unroll
-      for (int r = 0; r < RHO_G; ++r) {
-        const int jj = RHO_G * m + r;
-        if (jj == 0 || jj >= FZ_NELEM) continue;
-        s2 += __shfl(nii, gbase + r);
-        s3 += __shfl(niii, gbase + r);
+      if (j < FZ_NELEM) {
+        double ni, nii, niii;
+        stages(h * g_c.abfac[j], nnew, a1[m], a2[m], ni, nii, niii);
+        sh[j] = nii;
+        sh[RHO_LD + j] = niii;
+        if (j == 0) sh[FZ_NELEM] = ni;
       }
     }
-    double est = niii0 - ni0;
+    __builtin_amdgcn_wave_barrier();
+    // the reference's fold: est = (niii_0 - ni_0) + (sum_{j>=1} nii_j) + (sum_{j>=1} niii_j)
+    double s2 = 0.0, s3 = 0.0;
+#pragma unroll 13
+    for (int j = 1; j < FZ_NELEM; ++j) {
+      s2 += sh[j];
+      s3 += sh[RHO_LD + j];
+    }
+    double est = sh[RHO_LD] - sh[FZ_NELEM];
+    __builtin_amdgcn_wave_barrier();
     est = est + s2 + s3;
     nnew = (fmax(est, 0.0) + nold) / 2.0;
     err = fabs((nnew - nold) / nnew);
@@ -192,7 +200,7 @@ struct RhoOut {
   int err;
 };
 
-__device__ __noinline__ RhoOut rho_group_at(const double* __restrict__ base, long stride, int q, int gbase, double T,
+__device__ __noinline__ RhoOut rho_group_at(const double* __restrict__ base, long stride, int q, double* sh, double T,
                                             double PG, double ne) {
   double a1[RHO_NS], a2[RHO_NS];
 #pragma unroll
@@ -203,13 +211,15 @@ __device__ __noinline__ RhoOut rho_group_at(const double* __restrict__ base, lon
   }
   RhoOut o;
   o.ne = ne;
-  o.err = get_rho_group(a1, a2, q, gbase, T, PG, o.ne, o.nh, o.pel, o.dens, o.mu);
+  o.err = get_rho_group(a1, a2, q, sh, T, PG, o.ne, o.nh, o.pel, o.dens, o.mu);
   return o;
 }
 
 __global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
+  __shared__ double shall[256 / RHO_G][2 * RHO_LD];
   const int lane = threadIdx.x & 63;
-  const int q = lane & (RHO_G - 1), gbase = lane & ~(RHO_G - 1);
+  const int q = lane & (RHO_G - 1);
+  double* sh = shall[threadIdx.x / RHO_G];
   const int ct = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / RHO_G);
   const bool active = ct < a.ntile;
   // a group past the tile repeats the last column without writing, so that
@@ -230,7 +240,7 @@ __global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
       a2[m] = j < FZ_NELEM ? sb[(long)(j * 4 + 1) * a.ntile] : 1.0;
     }
     double nh, pel, dens, mu;
-    err = get_rho_group(a1, a2, q, gbase, T, P, ne, nh, pel, dens, mu);
+    err = get_rho_group(a1, a2, q, sh, T, P, ne, nh, pel, dens, mu);
     if (err) break;
     if (active && q == 0) {
       a.atm[A_RHO * ps + k * cs + col] = (float)dens;
@@ -405,9 +415,9 @@ __device__ __forceinline__ double he_step_rhs(const S& s, double lnp, double tem
   return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
 }
 
-__device__ __forceinline__ double he_step_rhs_group(const double* base, long stride, int q, int gbase, double lnp,
+__device__ __forceinline__ double he_step_rhs_group(const double* base, long stride, int q, double* sh, double lnp,
                                                     double tem, double& nel, int& err) {
-  const RhoOut o = rho_group_at(base, stride, q, gbase, tem, exp(lnp), nel);
+  const RhoOut o = rho_group_at(base, stride, q, sh, tem, exp(lnp), nel);
   nel = o.ne;
   if (o.err) err = o.err;
   return -o.mu * MAMU * GRAV / (KBOL * tem);        // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
@@ -417,8 +427,10 @@ __device__ __forceinline__ double he_step_rhs_group(const double* base, long str
 // get_rho_group (same values as he_rk4_kernel's get_rho, bit for bit); the
 // integration scalars are identical in the lanes of a group.
 __global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const double* __restrict__ smid) {
+  __shared__ double shall[256 / RHO_G][2 * RHO_LD];
   const int lane = threadIdx.x & 63;
-  const int q = lane & (RHO_G - 1), gbase = lane & ~(RHO_G - 1);
+  const int q = lane & (RHO_G - 1);
+  double* sh = shall[threadIdx.x / RHO_G];
   const int ct = (int)(((long)blockIdx.x * blockDim.x + threadIdx.x) / RHO_G);
   const bool active = ct < a.ntile;
   const int cc = active ? ct : a.ntile - 1;
@@ -435,7 +447,7 @@ __global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const do
   double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
   double nh, pel, dens, mw;
   {
-    const RhoOut o = rho_group_at(SH + top * hs, a.ntile, q, gbase, (double)A(A_T, top), exp(u0), nel);
+    const RhoOut o = rho_group_at(SH + top * hs, a.ntile, q, sh, (double)A(A_T, top), exp(u0), nel);
     nel = o.ne, nh = o.nh, pel = o.pel, dens = o.dens, mw = o.mu, err = o.err;
   }
   if (wr) {
@@ -448,13 +460,13 @@ __global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const do
   for (int k = nz - 2; k >= 0 && !err; --k) {
     const double dz = c_cfg.he_dz[k];
     const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
-    const double k1 = he_step_rhs_group(SH + (k + 1) * hs, a.ntile, q, gbase, u0, tm0, nel, err) * dz;
+    const double k1 = he_step_rhs_group(SH + (k + 1) * hs, a.ntile, q, sh, u0, tm0, nel, err) * dz;
     const double tmid = he_tmid(k, tm0, tm1);
-    const double k2 = he_step_rhs_group(SM + k * hs, a.ntile, q, gbase, u0 + k1 / 2.0, tmid, nel, err) * dz;
-    const double k3 = he_step_rhs_group(SM + k * hs, a.ntile, q, gbase, u0 + k2 / 2.0, tmid, nel, err) * dz;
-    const double k4 = he_step_rhs_group(SH + k * hs, a.ntile, q, gbase, u0 + k3, tm1, nel, err) * dz;
+    const double k2 = he_step_rhs_group(SM + k * hs, a.ntile, q, sh, u0 + k1 / 2.0, tmid, nel, err) * dz;
+    const double k3 = he_step_rhs_group(SM + k * hs, a.ntile, q, sh, u0 + k2 / 2.0, tmid, nel, err) * dz;
+    const double k4 = he_step_rhs_group(SH + k * hs, a.ntile, q, sh, u0 + k3, tm1, nel, err) * dz;
     const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
-    const RhoOut o = rho_group_at(SH + k * hs, a.ntile, q, gbase, tm1, exp(u1), nel);
+    const RhoOut o = rho_group_at(SH + k * hs, a.ntile, q, sh, tm1, exp(u1), nel);
     nel = o.ne, nh = o.nh, pel = o.pel, dens = o.dens, mw = o.mu;
     if (o.err) err = o.err;
     if (wr) {
