@@ -110,12 +110,16 @@ def main():
     ap.add_argument("--train-nx", type=int, default=512)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--max-passes", type=int, default=3)
+    ap.add_argument("--uneven", action="store_true",
+                    help="with --lm: per-pixel line sets and windows (8-line list, 1-4 lines, 60-300 wavelengths)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (columns are independent; "
               "see the sharding design).", file=sys.stderr)
         sys.exit(2)
 
+    if args.lm and args.uneven:
+        return bench_uneven(args)
     if args.lm:
         return bench_lm(args)
 
@@ -375,6 +379,64 @@ def bench_lm(args):
                                    "epochs": args.epochs, "train_seconds": rep.seconds, "data_seconds": t_data,
                                    "train_loss": rep.train_loss, "valid_loss": rep.valid_loss,
                                    "fallback_threshold_chi2": threshold}}}
+    print(json.dumps(line), flush=True)
+
+
+def bench_uneven(args):
+    """INVERT3D over pixels that each observe their own 1-4 lines of an
+    8-line Fe I / Si I list, each line through a centred window of 60-300
+    wavelengths, in one launch (per-pixel windows): the ragged spectra of all
+    pixels go through one forward and one LM step per iteration."""
+    import torch
+    from firtez_dz_amd import atmos, gpu, lines
+
+    nx = ny = 512 if args.nx == 288 else args.nx
+    nz = 64 if args.nz == 96 else args.nz
+    ncol = nx * ny
+    ls = lines.uneven_list(nwave=300)
+    print("[bench-uneven] seeding %dx%dx%d truth" % (nx, ny, nz), file=sys.stderr, flush=True)
+    atm, z = atmos.seeded(nx, ny, nz, dz_km=15.0, seed=7)
+    ctx = gpu.Context(0)
+    numw = ctx.configure(ls, z)
+    ctx.gain_hsra()
+    truth = torch.from_numpy(atm).cuda()
+    assert int(ctx.hydrostatic_torch(truth).max().item()) == 0
+    init = plain_init(ctx, truth)
+    win = lines.uneven_windows(ls, ncol, seed=17)
+    off = ctx.set_windows(win)
+    nw = np.diff(off)
+    nlin = (win[:, :, 1] > 0).sum(axis=1)
+    tot = int(off[-1])
+    st, _, status = ctx.forward_torch(truth, gpu.RF_NONE)
+    torch.cuda.synchronize()
+    assert int(status.max().item()) == 0
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    obs = (st.double() + 1e-3 * torch.randn(st.shape, generator=gen, device="cuda", dtype=torch.float64)).contiguous()
+    del st
+    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3, smoothing=False)
+    print("[bench-uneven] %d pixels, %d wavelengths (%.0f per pixel, %d..%d), lines per pixel %.2f"
+          % (ncol, tot, nw.mean(), nw.min(), nw.max(), nlin.mean()), file=sys.stderr, flush=True)
+    model = init.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, chi2, its = ctx.invert3d_torch(model, obs, ctl, nx, ny)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    stats = lm_stats(model, truth, chi2.cpu().numpy())
+    print("[bench-uneven] %.1f s, %d iterations, median chi2 %.3g" % (dt, its, stats["median_chi2"]),
+          file=sys.stderr, flush=True)
+    workload = ("INVERT3D %dx%d pixels, nz=%d, per pixel 1-4 of 8 lines (Fe I 6301.5, 6302.5, 6173.3, 15648.5, "
+                "15652.9, 15662.0, 5576.1, Si I 10827.1) in centred windows of 60-300 of 300 wavelengths, "
+                "T/Bx/By/Bz/vz up to 4 nodes (2 cycles), one launch" % (nx, ny, nz))
+    line = {"metric": "LM-converged pixels/s (uneven lines/windows)", "value": ncol / dt, "unit": "pixel/s",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer",
+            "data": "synthetic (seeded truth + 1e-3 noise, seeded windows)",
+            "config": dict({"workload": workload, "wavelengths_total": tot, "wavelengths_per_pixel_mean": float(nw.mean()),
+                            "union_wavelengths": numw, "lines_per_pixel_mean": float(nlin.mean()),
+                            "lm_iterations": its, "seconds": dt,
+                            "column_wavelengths_per_s_of_iterations": tot * its / dt}, **stats)}
     print(json.dumps(line), flush=True)
 
 

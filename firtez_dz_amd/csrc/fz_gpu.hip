@@ -533,6 +533,16 @@ __global__ void __launch_bounds__(64) he_rk4_kernel(MicroTile a, const double* _
 // Launch description of rt_kernel (host side; the kernel receives the
 // pointers as separate restrict-qualified arguments so that loads whose
 // address is uniform across the wave become scalar loads).
+// One radiative-transfer workgroup of a cube with per-pixel spectral
+// windows: column, region, first wavelength (index in the configured grid),
+// wavelengths in this workgroup (<= RT_WAVE) and their position in the
+// pixel's own spectrum.
+struct RtItem {
+  int col;
+  short r, n;
+  int l0, lout;
+};
+
 struct RtArgs {
   int ncol;          // columns in the whole cube (strides of atm/micro)
   int col0;          // first column of this launch
@@ -545,6 +555,9 @@ struct RtArgs {
   float* jac;        // [ncol - jcol0][np][nz][4][numw]
   int jcol0;         // column of jac[0]
   float* scratch;    // [nz][4][ntile*numw]
+  // per-pixel windows (items null: every column uses every wavelength)
+  const RtItem* items;   // this launch's workgroups
+  const int64_t* poff;   // [ncol+1] prefix of the pixels' wavelength counts: Stokes at 4*poff[col]
 };
 
 enum { D_T = 0, D_P, D_R, D_V, D_B, D_G, D_F, NDIR };
@@ -747,7 +760,8 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
                                                      const float* __restrict__ gain, const int32_t* __restrict__ status,
                                                      float* __restrict__ stokes, float* __restrict__ jac,
                                                      float* __restrict__ scratch, int ncol, int col0, int ntile,
-                                                     int jcol0) {
+                                                     int jcol0, const RtItem* __restrict__ items,
+                                                     const int64_t* __restrict__ poff) {
   RtArgs a;
   a.ncol = ncol;
   a.col0 = col0;
@@ -760,21 +774,48 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   a.jac = jac;
   a.jcol0 = jcol0;
   a.scratch = scratch;
+  a.items = items;
+  a.poff = poff;
   // parameters: 5 spherical, 7 Cartesian, 8 with P0 (the top pressure)
   constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : (RF == FZ_RF_CARTESIAN_P0 ? 8 : 0));
   constexpr int NPL = RF == FZ_RF_CARTESIAN_P0 ? 7 : NP;    // parameters with a local response
   const int numw = c_cfg.numw, nz = c_cfg.nz;
   // workgroup -> (column, region, first wavelength); lane -> wavelength
-  const int ct = (int)(blockIdx.x / (unsigned)c_cfg.nbpc);
-  const int bc = (int)(blockIdx.x - (unsigned)ct * (unsigned)c_cfg.nbpc);
-  if ((int)threadIdx.x >= c_cfg.blk_n[bc]) return;
-  const int r = c_cfg.blk_reg[bc];
-  const int l = c_cfg.blk_l0[bc] + (int)threadIdx.x;
-  const long t = (long)ct * numw + l;
-  const long nthr = (long)a.ntile * numw;
+  int ct, r, l;
+  long sbase;        // the column's Stokes: stokes[sbase + s * snumw + lo]
+  int snumw, lo;
+  long t, nthr;      // lane slot and lane count of the tile (scratch rows)
+  long jbase;        // the column's response functions: jac[jbase + ((p*nz + k)*4 + s)*snumw]
+  if (items) {
+    // per-pixel windows: scratch and response functions are ragged too
+    const RtItem it = items[blockIdx.x];
+    if ((int)threadIdx.x >= it.n) return;
+    ct = it.col - col0;
+    r = it.r;
+    l = it.l0 + (int)threadIdx.x;
+    const int64_t p0 = poff[it.col], pt = poff[col0];
+    sbase = 4 * p0;
+    snumw = (int)(poff[it.col + 1] - p0);
+    lo = it.lout + (int)threadIdx.x;
+    t = (p0 - pt) + lo;
+    nthr = poff[col0 + a.ntile] - pt;
+    jbase = (long)NP * nz * 4 * (p0 - poff[a.jcol0]) + lo;
+  } else {
+    ct = (int)(blockIdx.x / (unsigned)c_cfg.nbpc);
+    const int bc = (int)(blockIdx.x - (unsigned)ct * (unsigned)c_cfg.nbpc);
+    if ((int)threadIdx.x >= c_cfg.blk_n[bc]) return;
+    r = c_cfg.blk_reg[bc];
+    l = c_cfg.blk_l0[bc] + (int)threadIdx.x;
+    sbase = (long)(col0 + ct) * 4 * numw;
+    snumw = numw;
+    lo = l;
+    t = (long)ct * numw + l;
+    nthr = (long)a.ntile * numw;
+    jbase = (long)(col0 + ct - a.jcol0) * NP * nz * 4 * numw + l;
+  }
   const int col = a.col0 + ct;
   if (a.status && a.status[col] != 0) {
-    for (int s = 0; s < 4; ++s) a.stokes[((long)col * 4 + s) * numw + l] = __builtin_nanf("");
+    for (int s = 0; s < 4; ++s) a.stokes[sbase + (long)s * snumw + lo] = __builtin_nanf("");
     return;
   }
   const double wd = c_cfg.reg_wstart[r] + c_cfg.reg_wstep[r] * (double)(l - c_cfg.reg_pix0[r]);
@@ -837,7 +878,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     gn[s] = a.gain[l * 4 + s];
-    a.stokes[((long)col * 4 + s) * numw + l] = I[s] * gn[s];
+    a.stokes[sbase + (long)s * snumw + lo] = I[s] * gn[s];
   }
   if constexpr (RF != 0) {
     float acc[4][4];
@@ -845,7 +886,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = (i == j) ? 1.f : 0.f;
-    float* jb = a.jac + (long)(col - a.jcol0) * NP * nz * 4 * numw + l;
+    float* jb = a.jac + jbase;
     for (int k = nz - 1; k >= 0; --k) {
       float outv[NP][4];
       const Layer ly = load_layer(a, col, k);
@@ -965,7 +1006,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
       for (int p = 0; p < NP; ++p)
 #pragma unroll
-        for (int s = 0; s < 4; ++s) jb[((long)(p * nz + k) * 4 + s) * numw] = outv[p][s] * gn[s];
+        for (int s = 0; s < 4; ++s) jb[((long)(p * nz + k) * 4 + s) * snumw] = outv[p][s] * gn[s];
     }
   }
 }
@@ -1181,6 +1222,14 @@ struct fz_ctx {
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
   bool lm_p0 = false;         // P0 among the inverted parameters
+  // per-pixel spectral windows
+  bool win_on = false;
+  int win_ncol = 0;
+  std::vector<int64_t> h_poff;      // [ncol+1]
+  std::vector<int> h_item_first;    // [ncol+1] first RT workgroup of each column
+  RtItem* d_items = nullptr;
+  int64_t* d_poff = nullptr;
+  int32_t* d_win = nullptr;         // [ncol][nreg][3]: first, count, position in the pixel's spectrum
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
   MlpState mlp;
 };
@@ -1256,6 +1305,9 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_stat) (void)hipFree(c->d_stat);
   if (c->d_smid) (void)hipFree(c->d_smid);
   if (c->d_derb) (void)hipFree(c->d_derb);
+  if (c->d_items) (void)hipFree(c->d_items);
+  if (c->d_poff) (void)hipFree(c->d_poff);
+  if (c->d_win) (void)hipFree(c->d_win);
   if (c->d_lsf) (void)hipFree(c->d_lsf);
   c->mlp.release();
   for (int i = 0; i < 3; ++i)
@@ -1301,6 +1353,24 @@ static int zeeman_host(const fz_line& L, std::vector<float>& shift, std::vector<
   }
   return 0;
 }
+
+}  // extern "C"
+
+// drops the per-pixel windows (their regions are those of one configuration)
+static void clear_windows(fz_ctx* c) {
+  if (c->d_items) (void)hipFree(c->d_items);
+  if (c->d_poff) (void)hipFree(c->d_poff);
+  if (c->d_win) (void)hipFree(c->d_win);
+  c->d_items = nullptr;
+  c->d_poff = nullptr;
+  c->d_win = nullptr;
+  c->win_on = false;
+  c->win_ncol = 0;
+  c->h_poff.clear();
+  c->h_item_first.clear();
+}
+
+extern "C" {
 
 int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* regs, int nreg, int nz,
                  const double* z_km) {
@@ -1390,6 +1460,7 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
   std::vector<float> ones(4 * numw, 1.0f);
   for (int l = 0; l < numw; ++l) ones[4 * l + 3] = -1.0f;
   HIPCHK(c, hipMemcpy(c->d_gain, ones.data(), sizeof(float) * 4 * numw, hipMemcpyHostToDevice));
+  clear_windows(c);
   c->configured = true;
   c->have_gain = false;
   return FZ_OK;
@@ -1430,6 +1501,62 @@ int fz_set_lsf(fz_ctx* c, const double* lsf_kernel, const int32_t* lsf_valid) {
   return FZ_OK;
 }
 
+int fz_set_windows(fz_ctx* c, int ncol, const int32_t* windows) {
+  if (!c || !c->configured) return FZ_EARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  clear_windows(c);
+  if (!windows) return FZ_OK;
+  if (ncol < 1) return set_err(c, FZ_EARG, "no columns");
+  const Cfg& g = c->cfg;
+  const int nreg = g.nreg;
+  std::vector<int64_t> poff((size_t)ncol + 1, 0);
+  std::vector<int> first((size_t)ncol + 1, 0);
+  std::vector<RtItem> items;
+  std::vector<int32_t> w((size_t)ncol * nreg * 3);
+  for (int col = 0; col < ncol; ++col) {
+    int pos = 0;
+    for (int r = 0; r < nreg; ++r) {
+      const int f0 = windows[((size_t)col * nreg + r) * 2], cnt = windows[((size_t)col * nreg + r) * 2 + 1];
+      if (cnt < 0 || f0 < 0 || f0 + cnt > g.reg_nwave[r])
+        return set_err(c, FZ_EARG, "window outside its region (column " + std::to_string(col) + ")");
+      w[((size_t)col * nreg + r) * 3 + 0] = f0;
+      w[((size_t)col * nreg + r) * 3 + 1] = cnt;
+      w[((size_t)col * nreg + r) * 3 + 2] = pos;
+      for (int o = 0; o < cnt; o += RT_WAVE) {
+        RtItem it;
+        it.col = col;
+        it.r = (short)r;
+        it.n = (short)std::min(RT_WAVE, cnt - o);
+        it.l0 = g.reg_pix0[r] + f0 + o;
+        it.lout = pos + o;
+        items.push_back(it);
+      }
+      pos += cnt;
+    }
+    if (pos == 0) return set_err(c, FZ_EARG, "a pixel without wavelengths (column " + std::to_string(col) + ")");
+    poff[col + 1] = poff[col] + pos;
+    if (items.size() > (size_t)0x7fffffff) return set_err(c, FZ_EARG, "too many workgroups");
+    first[col + 1] = (int)items.size();
+  }
+  HIPCHK(c, hipMalloc(&c->d_items, sizeof(RtItem) * items.size()));
+  HIPCHK(c, hipMalloc(&c->d_poff, sizeof(int64_t) * poff.size()));
+  HIPCHK(c, hipMalloc(&c->d_win, sizeof(int32_t) * w.size()));
+  HIPCHK(c, hipMemcpy(c->d_items, items.data(), sizeof(RtItem) * items.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_poff, poff.data(), sizeof(int64_t) * poff.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->d_win, w.data(), sizeof(int32_t) * w.size(), hipMemcpyHostToDevice));
+  c->h_poff = poff;
+  c->h_item_first = first;
+  c->win_ncol = ncol;
+  c->win_on = true;
+  return FZ_OK;
+}
+
+int fz_window_offsets(const fz_ctx* c, int64_t* offsets) {
+  if (!c || !c->win_on || !offsets) return FZ_EARG;
+  std::memcpy(offsets, c->h_poff.data(), sizeof(int64_t) * c->h_poff.size());
+  return FZ_OK;
+}
+
 int fz_gain_hsra(fz_ctx* c, float* gain) {
   if (!c || !c->configured) return FZ_EARG;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1465,9 +1592,11 @@ static int rf_nparam(int rf_mode) {
   return rf_mode == FZ_RF_SPHERICAL ? 5 : rf_mode == FZ_RF_CARTESIAN ? 7 : rf_mode == FZ_RF_CARTESIAN_P0 ? 8 : 0;
 }
 
-static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st) {
-  const dim3 grid((unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
-#define FZ_RT_ARGS a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0
+static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, int nitems = 0) {
+  const dim3 grid(a.items ? (unsigned)nitems : (unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
+  if (a.items && nitems == 0) return FZ_OK;
+#define FZ_RT_ARGS                                                                                               \
+  a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0, a.items, a.poff
   if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro && rf_mode == FZ_RF_SPHERICAL)
     hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false>), grid, blk, 0, st, FZ_RT_ARGS);
@@ -1579,6 +1708,12 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
   if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
   if (rf_mode != FZ_RF_NONE && !d_jac) return set_err(c, FZ_EARG, "jacobian output missing");
   if (rf_mode < 0 || rf_mode > 3) return FZ_EARG;
+  if (c->win_on) {
+    if (ncol != c->win_ncol) return set_err(c, FZ_EARG, "column count differs from fz_set_windows");
+    if (rf_mode != FZ_RF_NONE)
+      return set_err(c, FZ_EARG, "per-pixel windows: response functions come through the inversion only");
+    if (c->cfg.lsf_on) return set_err(c, FZ_ECONFIG, "per-pixel windows and an LSF together are not supported");
+  }
   const Cfg& g = c->cfg;
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1609,7 +1744,15 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     a.jac = d_jac;
     a.jcol0 = 0;
     a.scratch = c->d_scratch;
-    r = launch_rt(c, rf_mode, a, st);
+    a.items = nullptr;
+    a.poff = nullptr;
+    int nit = 0;
+    if (c->win_on) {
+      a.items = c->d_items + c->h_item_first[c0];
+      a.poff = c->d_poff;
+      nit = c->h_item_first[c0 + a.ntile] - c->h_item_first[c0];
+    }
+    r = launch_rt(c, rf_mode, a, st, nit);
     if (r) return r;
     HIPCHK(c, hipGetLastError());
     c->rt_launches++;
@@ -1938,6 +2081,10 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
                          void* stream) {
   if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes || !d_eqv) return FZ_EARG;
   if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  if (c->win_on) {
+    if (ncol != c->win_ncol) return set_err(c, FZ_EARG, "column count differs from fz_set_windows");
+    if (c->cfg.lsf_on) return set_err(c, FZ_ECONFIG, "per-pixel windows and an LSF together are not supported");
+  }
   const Cfg& g = c->cfg;
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(c, hipSetDevice(c->device));
@@ -1945,13 +2092,28 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   if (r) return r;
   r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
   if (r) return r;
-  const int tile = std::min(c->tile, ncol);
-  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * tile * g.numw);
-  if (r) return r;
   // the P0 slot is synthesised only when P0 is inverted
   const bool p0 = c->lm_p0;
   const int nslot = p0 ? 8 : 7;
-  r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)tile * nslot * g.nz * 4 * g.numw);
+  // columns per tile: the configured tile, fewer where one tile's response
+  // functions and scratch would pass 24 GiB; sized by the widest tile (the
+  // wavelengths of its columns: ragged with per-pixel windows)
+  const size_t per_w = sizeof(float) * ((size_t)nslot * g.nz * 4 + (size_t)g.nz * (c->hydro ? 8 : 4));
+  const size_t budget = (size_t)24 << 30;
+  int tile = std::min(c->tile, ncol);
+  size_t wmax = 0;
+  for (;;) {
+    wmax = 0;
+    for (int c0 = 0; c0 < ncol; c0 += tile) {
+      const int n = std::min(tile, ncol - c0);
+      wmax = std::max(wmax, c->win_on ? (size_t)(c->h_poff[c0 + n] - c->h_poff[c0]) : (size_t)n * g.numw);
+    }
+    if (wmax * per_w <= budget || tile == 1) break;
+    tile = std::max(1, (int)((double)tile * (double)budget / (double)(wmax * per_w)));
+  }
+  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * wmax);
+  if (r) return r;
+  r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)nslot * g.nz * 4 * wmax);
   if (r) return r;
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
   r = run_micro(c, ncol, d_atm, &d_status, true, c->hydro, true, st);
@@ -1973,6 +2135,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   }
   e.coefs = c->d_coefs;
   e.eqv = d_eqv;
+  e.poff = c->win_on ? c->d_poff : nullptr;
   for (int c0 = 0; c0 < ncol; c0 += tile) {
     RtArgs a;
     a.ncol = ncol;
@@ -1986,12 +2149,20 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     a.jac = c->d_jtile;
     a.jcol0 = c0;
     a.scratch = c->d_scratch;
-    r = launch_rt(c, p0 ? FZ_RF_CARTESIAN_P0 : FZ_RF_CARTESIAN, a, st);
+    a.items = nullptr;
+    a.poff = nullptr;
+    int nit = 0;
+    if (c->win_on) {
+      a.items = c->d_items + c->h_item_first[c0];
+      a.poff = c->d_poff;
+      nit = c->h_item_first[c0 + a.ntile] - c->h_item_first[c0];
+    }
+    r = launch_rt(c, p0 ? FZ_RF_CARTESIAN_P0 : FZ_RF_CARTESIAN, a, st, nit);
     if (r) return r;
     e.ncol_tile = a.ntile;
     e.col0 = c0;
     e.jac = c->d_jtile;
-    const long ne = (long)a.ntile * e.nfreq;
+    const long ne = c->win_on ? (long)e.nstk * (c->h_poff[c0 + a.ntile] - c->h_poff[c0]) : (long)a.ntile * e.nfreq;
     hipLaunchKernelGGL(eqv_kernel, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, e);
     HIPCHK(c, hipGetLastError());
     c->rt_launches++;
@@ -2006,6 +2177,18 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   return FZ_OK;
 }
 
+}  // extern "C"
+
+// per-pixel windows: the LM step reads ragged rows and normalises each
+// pixel's chi^2 by its own number of wavelengths times inverted Stokes
+static void set_lm_windows(const fz_ctx* c, LmArgs& a) {
+  a.poff = c->win_on ? c->d_poff : nullptr;
+  a.nstk = a.nfreq / a.numw;
+  a.inu_pix = c->win_on ? 1 : 0;
+}
+
+extern "C" {
+
 int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stokes, double* d_best_syn, double* d_sla,
                    const float* d_eqv, float* d_eqv_best, const double* d_isigmap, float* d_atm, float* d_atm_best,
                    int32_t* d_done, double* d_delta, void* stream) {
@@ -2013,6 +2196,7 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
       !d_atm || !d_atm_best)
     return FZ_EARG;
   if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  if (c->win_on && ncol != c->win_ncol) return set_err(c, FZ_EARG, "column count differs from fz_set_windows");
   HIPCHK(c, hipSetDevice(c->device));
   LmArgs a = c->lm;
   a.ncol = ncol;
@@ -2028,6 +2212,7 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
   a.done = d_done;
   a.delta = d_delta;
   a.pix_of = nullptr;
+  set_lm_windows(c, a);
   hipLaunchKernelGGL(lm_kernel, dim3(ncol), dim3(LM_THREADS), c->lm_lds, (hipStream_t)stream, a);
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
@@ -2053,6 +2238,8 @@ static int lm_step_active(fz_ctx* c, int ncol, int na, const int32_t* pix_of, co
   a.done = d_done;
   a.delta = nullptr;
   a.pix_of = pix_of;
+  set_lm_windows(c, a);
+  if (c->win_on && (pix_of || ncol != c->win_ncol)) return set_err(c, FZ_EARG, "per-pixel windows: every pixel keeps its row");
   hipLaunchKernelGGL(lm_kernel, dim3(na), dim3(LM_THREADS), c->lm_lds, st, a);
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
@@ -2113,6 +2300,11 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   const float minw = sqrtf(wsum) / sqrtf((float)nstk);
   const double isigma = (double)(float)(ctl->noise > 0 ? ctl->noise : 4.0e-3);
   const int nfreq = nstk * numw;
+  // per-pixel windows: ragged rows of nstk*numw_pix (obs, best synthesis,
+  // response functions), Stokes of 4*numw_pix; every pixel keeps its row
+  const bool win = c->win_on;
+  if (win && ncol != c->win_ncol) return set_err(c, FZ_EARG, "column count differs from fz_set_windows");
+  const size_t nwtot = win ? (size_t)c->h_poff[ncol] : (size_t)ncol * numw;
   std::vector<double> isig(nfreq);
   {
     int b = 0;
@@ -2131,9 +2323,9 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   const size_t natm = (size_t)12 * nz * ncol;
   HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));
   HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * natm));
-  HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * (size_t)ncol * npmax * nfreq));
-  HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * (size_t)ncol * npmax * nfreq));
-  HIPCHK(c, hipMalloc(&b_bsyn.p, sizeof(double) * (size_t)ncol * nfreq));
+  HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * nwtot * npmax * nstk));
+  HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * nwtot * npmax * nstk));
+  HIPCHK(c, hipMalloc(&b_bsyn.p, sizeof(double) * nwtot * nstk));
   HIPCHK(c, hipMalloc(&b_sla.p, sizeof(double) * (size_t)ncol * 9));
   HIPCHK(c, hipMalloc(&b_done.p, sizeof(int32_t) * ncol));
   HIPCHK(c, hipMalloc(&b_stat.p, sizeof(double) * 2));
@@ -2229,12 +2421,12 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
       if (ncol - (int)hs[1] <= 0) break;
       if (hs[0] / (double)ncol >= need) break;
       const int nact = ncol - (int)hs[1];
-      if (nact < ncol) {
+      if (nact < ncol && !win) {
         hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, st, ncol, (const int32_t*)done, (int32_t*)b_idx.p,
                            (int32_t*)b_cnt.p);
         HIPCHK(c, hipGetLastError());
       }
-      na = nact;
+      na = win ? ncol : nact;
     }
     // END_CYCLE (:328-403): current := best, synthesis without response functions
     hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, best, d_atm, (long)natm);
@@ -2255,6 +2447,7 @@ int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho
                 float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
                 int* iterations) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !obs3d || !ctl || !syn3d) return FZ_EARG;
+  if (c->win_on) return set_err(c, FZ_ECONFIG, "per-pixel windows: the cubes of fz_invert3d are dense");
   const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
   int nstk = 0;
   for (int s = 0; s < 4; ++s) nstk += ctl->invstk[s] != 0;

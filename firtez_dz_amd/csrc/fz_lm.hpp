@@ -56,6 +56,13 @@ struct LmArgs {
   // active-pixel launch: workgroup b handles pixel pix_of[b], and stokes/eqv
   // hold the compacted rows (row b); null: workgroup b is pixel b
   const int32_t* pix_of;
+  // per-pixel windows (poff null: none): pixel pix has numw_pix =
+  // poff[pix+1]-poff[pix] wavelengths; its obs/best_syn rows start at
+  // nstk*poff[pix], its Stokes at 4*poff[pix], its eqv/eqv_best at
+  // np*nstk*poff[pix]; isigmap is read per Stokes block (isigmap[b*numw]);
+  // inu_pix: chi^2 normalised by the pixel's own NFREQ
+  const int64_t* poff;
+  int nstk, inu_pix;
 };
 
 // ---------------------------------------------------------------- node contraction
@@ -66,27 +73,60 @@ struct EqvArgs {
   int slot_of[LM_MAXNP];       // response-function slot of each free parameter
   int kbeg[LM_MAXNP], kend[LM_MAXNP];
   const double* coefs;         // [np][nz]
-  const float* jac;            // tile: [ntile][nslot][nz][4][numw]
-  float* eqv;                  // [ncol][np][nfreq]
+  const float* jac;            // tile: [ntile][nslot][nz][4][numw], ragged with windows
+  float* eqv;                  // [ncol][np][nfreq], ragged with windows
+  // per-pixel windows (poff null: none): the tile's response functions are
+  // [nslot][nz][4][numw_col] per column from nslot*nz*4*(poff[col]-poff[col0]);
+  // the output block of pixel col is [np][nstk][numw_col] from np*nstk*poff[col]
+  const int64_t* poff;
 };
 
+// the column of the tile [c0, c0+n) whose ragged range [poff[c], poff[c+1]) holds g
+__device__ __forceinline__ int ragged_column(const int64_t* poff, int c0, int n, int64_t g) {
+  int lo = c0, hi = c0 + n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (poff[mid] <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// EQVDSYN (forward.f90:552-681): one thread per (column, inverted Stokes row,
+// wavelength), all free parameters; FP64 sums over the node's heights
 __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long n = (long)a.ncol_tile * a.nfreq;
-  if (t >= n) return;
-  const int ct = (int)(t / a.nfreq);
-  const int f = (int)(t - (long)ct * a.nfreq);
-  const int b = f / a.numw, l = f - b * a.numw;
-  const int s = a.rowstk[b];
-  const long cs = (long)4 * a.numw;
-  const float* jc = a.jac + (long)ct * a.nslot * a.nz * cs + s * a.numw + l;
-  float* out = a.eqv + ((long)(a.col0 + ct) * a.np) * a.nfreq + f;
+  const float* jc;
+  float* out;
+  long cs, fstride;
+  if (a.poff) {
+    const int64_t base = a.poff[a.col0];
+    if (t >= (long)a.nstk * (a.poff[a.col0 + a.ncol_tile] - base)) return;
+    const int col = ragged_column(a.poff, a.col0, a.ncol_tile, base + t / a.nstk);
+    const int64_t p0 = a.poff[col];
+    const int nw = (int)(a.poff[col + 1] - p0);
+    const int f = (int)(t - (long)a.nstk * (p0 - base));
+    const int b = f / nw, l = f - b * nw;
+    cs = (long)4 * nw;
+    jc = a.jac + (long)a.nslot * a.nz * 4 * (p0 - base) + (long)a.rowstk[b] * nw + l;
+    fstride = (long)a.nstk * nw;
+    out = a.eqv + (long)a.np * a.nstk * p0 + f;
+  } else {
+    if (t >= (long)a.ncol_tile * a.nfreq) return;
+    const int ct = (int)(t / a.nfreq);
+    const int f = (int)(t - (long)ct * a.nfreq);
+    const int b = f / a.numw, l = f - b * a.numw;
+    cs = (long)4 * a.numw;
+    jc = a.jac + (long)ct * a.nslot * a.nz * cs + a.rowstk[b] * a.numw + l;
+    fstride = a.nfreq;
+    out = a.eqv + ((long)(a.col0 + ct) * a.np) * a.nfreq + f;
+  }
   for (int j = 0; j < a.np; ++j) {
     const float* jp = jc + (long)a.slot_of[j] * a.nz * cs;
     const double* cf = a.coefs + (long)j * a.nz;
     double acc = 0.0;
     for (int k = a.kbeg[j]; k <= a.kend[j]; ++k) acc += (double)jp[k * cs] * cf[k];
-    out[(long)j * a.nfreq] = (float)acc;
+    out[(long)j * fstride] = (float)acc;
   }
 }
 
@@ -162,7 +202,24 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   const int row = blockIdx.x;
   const int pix = a.pix_of ? a.pix_of[row] : row;
   if (a.done && a.done[pix] != 0) return;
-  const int np = a.np, nz = a.nz, nf = a.nfreq, numw = a.numw;
+  const int np = a.np, nz = a.nz;
+  int nf = a.nfreq, numw = a.numw;
+  long o_obs = (long)pix * a.nfreq, o_stk = (long)row * 4 * a.numw;
+  long o_ecur = (long)row * np * a.nfreq, o_ebest = (long)pix * np * a.nfreq;
+  double inu = a.inu;
+  if (a.poff) {
+    const int64_t p0 = a.poff[pix];
+    numw = (int)(a.poff[pix + 1] - p0);
+    nf = a.nstk * numw;
+    o_obs = (long)a.nstk * p0;
+    o_stk = 4 * p0;
+    o_ecur = o_ebest = (long)np * a.nstk * p0;
+    if (a.inu_pix) inu = (double)nf;
+  }
+  // inverse noise weight of row f (one value per Stokes block with windows)
+  auto isig = [&](int f) -> double {
+    return a.poff ? a.isigmap[(long)(f / numw) * a.numw] : a.isigmap[f];
+  };
   const int ld = np + 1;                       // LDS row stride of A and V
   const int npe = np + (np & 1);               // even size for the tournament ordering
   // LDS carve-up
@@ -191,7 +248,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm[(long)i * ncol + pix];
   if (t < 9) s_sla[t] = a.sla[(long)pix * 9 + t];
   // columns whose absolute sum overflows REAL(4) are zeroed (forward.f90:672-678)
-  const float* ecur = a.eqv + (long)row * np * nf;
+  const float* ecur = a.eqv + o_ecur;
   for (int j = t; j < np; j += LM_THREADS) {
     float s = 0.f;
     for (int f = 0; f < nf; ++f) s += fabsf(ecur[(long)j * nf + f]);
@@ -200,15 +257,15 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   __syncthreads();
 
   // ---- NEW_CALCULATE_CHI2 (get_dmodel.f90:960-1032)
-  const double* obs = a.obs + (long)pix * nf;
-  const float* stk = a.stokes + (long)row * 4 * numw;
+  const double* obs = a.obs + o_obs;
+  const float* stk = a.stokes + o_stk;
   double part = 0.0;
   for (int f = t; f < nf; f += LM_THREADS) {
     const int b = f / numw, l = f - b * numw;
     const double d = obs[f] - (double)stk[a.rowstk[b] * numw + l];
-    part += d * d * a.isigmap[f];
+    part += d * d * isig(f);
   }
-  double chi = block_sum(part, red) / a.inu;
+  double chi = block_sum(part, red) / inu;
   if (a.regul) {
     if (t == 0) lm_penalties(a, mod, pen_res, pen_hss, nod, nod + np);
     __syncthreads();
@@ -219,8 +276,8 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   if (t == 0) s_accept = chi < s_sla[2] ? 1 : 0;
   __syncthreads();
   const bool accept = s_accept != 0;
-  double* bsyn = a.best_syn + (long)pix * nf;
-  float* ebest = a.eqv_best + (long)pix * np * nf;
+  double* bsyn = a.best_syn + o_obs;
+  float* ebest = a.eqv_best + o_ebest;
   if (accept) {
     // best := current (model, equivalent response functions, synthesis)
     for (int i = t; i < 12 * nz; i += LM_THREADS) a.atm_best[(long)i * ncol + pix] = mod[i];
@@ -266,7 +323,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   // ---- normal matrix and gradient of the normalised problem
   //      J_n = J sqrt(w) / sqrt(nu) * PARAM_NORM,  r = (obs - best) sqrt(w) / sqrt(nu)
   const float* jsrc = accept ? ecur : ebest;
-  const double snu = sqrt(a.inu);
+  const double snu = sqrt(inu);
   const int ntri = np * (np + 1) / 2, nq = ntri + np;
   double acc[LM_SLOTS];
   int qi[LM_SLOTS], qj[LM_SLOTS];
@@ -299,11 +356,11 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       double v = (accept && izero[j]) ? 0.0 : (double)jsrc[(long)j * nf + f];
       int ip = 0;
       while (j >= a.j0[ip] + a.nj[ip]) ++ip;
-      Jn[c * np + j] = v * sqrt(a.isigmap[f]) / snu * (1.0 * c_lm_norm[a.par[ip]]);
+      Jn[c * np + j] = v * sqrt(isig(f)) / snu * (1.0 * c_lm_norm[a.par[ip]]);
     }
     for (int c = t; c < nc; c += LM_THREADS) {
       const int f = f0 + c;
-      rch[c] = (obs[f] - bsyn[f]) * sqrt(a.isigmap[f]) / snu;
+      rch[c] = (obs[f] - bsyn[f]) * sqrt(isig(f)) / snu;
     }
     __syncthreads();
 #pragma unroll

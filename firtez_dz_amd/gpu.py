@@ -90,6 +90,8 @@ def library():
     L.fz_set_tile.argtypes = [vp, ip, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_lsf.argtypes = [vp, vp, vp]
+    L.fz_set_windows.argtypes = [vp, ip, vp]
+    L.fz_window_offsets.argtypes = [vp, vp]
     L.fz_synchronize.argtypes = [vp]
     L.fz_set_profiling.argtypes = [vp, ip]
     L.fz_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
@@ -124,7 +126,7 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
-            "fz_set_hydrostatic", "fz_set_lsf",
+            "fz_set_hydrostatic", "fz_set_lsf", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d",
             "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
@@ -197,6 +199,7 @@ class Context:
         self.numw = 0
         self.nz = 0
         self.lineset = None
+        self.win_offsets = None
 
     def close(self):
         if self.h:
@@ -222,6 +225,7 @@ class Context:
         self.numw = self.lib.fz_numw(self.h)
         self.nz = len(z)
         self.lineset = ls
+        self.win_offsets = None
         return self.numw
 
     def gain_hsra(self) -> np.ndarray:
@@ -251,6 +255,25 @@ class Context:
         v = None if valid is None else np.ascontiguousarray(valid, dtype=np.int32)
         self._check(self.lib.fz_set_lsf(self.h, k.ctypes.data, None if v is None else v.ctypes.data), "fz_set_lsf")
         self._lsf = (k, v)
+
+    def set_windows(self, windows=None):
+        """Per-pixel spectral windows: windows int [ncol][nreg][2] = (first
+        wavelength index in the region, count); None switches them off.
+        Afterwards spectra are ragged (see fz_set_windows) and
+        self.win_offsets [ncol+1] holds each pixel's wavelength offset."""
+        self.win_offsets = None          # the library drops the old windows first
+        if windows is None:
+            self._check(self.lib.fz_set_windows(self.h, 0, None), "fz_set_windows")
+            return None
+        w = np.ascontiguousarray(windows, dtype=np.int32)
+        if w.ndim != 3 or w.shape[2] != 2:
+            raise ValueError("windows must be [ncol][nreg][2]")
+        ncol = w.shape[0]
+        self._check(self.lib.fz_set_windows(self.h, ncol, w.ctypes.data), "fz_set_windows")
+        off = np.zeros(ncol + 1, dtype=np.int64)
+        self._check(self.lib.fz_window_offsets(self.h, off.ctypes.data), "fz_window_offsets")
+        self.win_offsets = off
+        return off
 
     def autotune_torch(self, atm, rf_mode, stokes, jac, status):
         """Pick the microphysics and radiative-transfer tiles by timing forwards
@@ -292,6 +315,12 @@ class Context:
         self._check(self.lib.fz_synchronize(self.h), "fz_synchronize")
 
     # ---------------------------------------------------------------- torch convenience
+    def _stokes_shape(self, ncol):
+        # ragged [4*sum numw_col] with per-pixel windows, else [ncol][4][numw]
+        if self.win_offsets is not None:
+            return (4 * int(self.win_offsets[-1]),)
+        return (ncol, 4, self.numw)
+
     def forward_torch(self, atm, rf_mode=RF_SPHERICAL, stokes=None, jac=None, status=None):
         """atm: torch float32 cuda tensor [12, nz, ncol] (updated in place).
         Returns (stokes [ncol,4,numw], jac [ncol,np,nz,4,numw] or None, status [ncol])."""
@@ -301,7 +330,7 @@ class Context:
         ncol = atm.shape[2]
         dev = atm.device
         if stokes is None:
-            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=dev)
+            stokes = torch.empty(self._stokes_shape(ncol), dtype=torch.float32, device=dev)
         npar = NPARAM[rf_mode]
         if npar and jac is None:
             jac = torch.empty((ncol, npar, self.nz, 4, self.numw), dtype=torch.float32, device=dev)
@@ -337,9 +366,13 @@ class Context:
         ncol = atm.shape[2]
         dev = atm.device
         if stokes is None:
-            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=dev)
+            stokes = torch.empty(self._stokes_shape(ncol), dtype=torch.float32, device=dev)
         if eqv is None:
-            eqv = torch.empty((ncol, self.ifreep, self.nfreq), dtype=torch.float32, device=dev)
+            if self.win_offsets is not None:
+                eqv = torch.empty(self.ifreep * self.nfreq // self.numw * int(self.win_offsets[-1]),
+                                  dtype=torch.float32, device=dev)
+            else:
+                eqv = torch.empty((ncol, self.ifreep, self.nfreq), dtype=torch.float32, device=dev)
         if status is None:
             status = torch.zeros(ncol, dtype=torch.int32, device=dev)
         stream = torch.cuda.current_stream(dev).cuda_stream
@@ -444,9 +477,10 @@ class Context:
         import torch
         ncol = nx * ny
         assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous() and atm.shape[2] == ncol
-        assert obs.is_cuda and obs.dtype == torch.float64 and obs.is_contiguous() and obs.shape[0] == ncol
+        assert obs.is_cuda and obs.dtype == torch.float64 and obs.is_contiguous()
+        assert obs.shape[0] == ncol or self.win_offsets is not None
         if stokes is None:
-            stokes = torch.empty((ncol, 4, self.numw), dtype=torch.float32, device=atm.device)
+            stokes = torch.empty(self._stokes_shape(ncol), dtype=torch.float32, device=atm.device)
         if chi2 is None:
             chi2 = torch.empty(ncol, dtype=torch.float64, device=atm.device)
         its = ctypes.c_int()

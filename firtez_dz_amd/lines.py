@@ -102,6 +102,60 @@ def fe6301_6302(nwave: int = 112, start: float = -624.0, step: float = 20.0) -> 
     return setup([((4, 5), nwave, start, step)])
 
 
+# Fe I 6301.5, 6302.5, 6173.3, 15648.5, 15652.9, 15662.0, 5576.1 and Si I
+# 10827.1 (lines_database.dat ids)
+UNEVEN_IDS = (4, 5, 2, 3, 14, 18, 20, 24)
+
+
+def uneven_list(nwave: int = 300, step_6300: float = 10.0) -> LineSetup:
+    """The eight lines of UNEVEN_IDS, one region each, nwave points centred on
+    the line; the step grows with the wavelength (Doppler widths scale with
+    lambda0), 10 mA at 6300 A."""
+    rows = []
+    db = database()
+    for lid in UNEVEN_IDS:
+        step = round(step_6300 * db[lid]["lambda0"] / 6300.0, 1)
+        rows.append(((lid,), nwave, -step * (nwave // 2), step))
+    return setup(rows)
+
+
+def uneven_windows(ls: LineSetup, ncol: int, seed: int, nlines=(1, 4), width=(60, 300)) -> np.ndarray:
+    """Per-pixel windows for fz_set_windows: each pixel observes nlines[0] to
+    nlines[1] regions drawn without replacement, each a window of
+    width[0]..width[1] points (capped at the region) centred in the region.
+    Returns int32 [ncol][nreg][2] = (first index, count)."""
+    rng = np.random.default_rng(seed)
+    nreg = len(ls.regions)
+    w = np.zeros((ncol, nreg, 2), np.int32)
+    for c in range(ncol):
+        k = int(rng.integers(nlines[0], min(nlines[1], nreg) + 1))
+        for r in rng.choice(nreg, size=k, replace=False):
+            nw = ls.regions[r].nwave
+            cnt = int(rng.integers(min(width[0], nw), min(width[1], nw) + 1))
+            w[c, r] = ((nw - cnt) // 2, cnt)
+    return w
+
+
+def window_setup(ls: LineSetup, win_col) -> LineSetup:
+    """The line setup whose regions are one pixel's windows (regions with a
+    zero count dropped): its dense wavelength grid is that pixel's ragged
+    spectrum, in the same order."""
+    regs = []
+    for r, (f0, cnt) in zip(ls.regions, np.asarray(win_col)):
+        if cnt > 0:
+            regs.append(Region(list(r.lines), int(cnt), r.wave_start + r.wave_step * int(f0), r.wave_step))
+    return LineSetup(list(ls.lines), regs)
+
+
+def window_index(ls: LineSetup, win_col) -> np.ndarray:
+    """Indices into the full (union) wavelength axis of one pixel's windows."""
+    idx, p0 = [], 0
+    for r, (f0, cnt) in zip(ls.regions, np.asarray(win_col)):
+        idx.append(p0 + int(f0) + np.arange(int(cnt)))
+        p0 += r.nwave
+    return np.concatenate(idx).astype(np.int64)
+
+
 # ---------------------------------------------------------------- C structs
 class CLine(ctypes.Structure):
     _fields_ = [("zn", ctypes.c_int32), ("ion", ctypes.c_int32),

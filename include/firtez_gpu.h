@@ -85,6 +85,28 @@ int fz_set_gain(fz_ctx* ctx, const float* gain);
  * fz_configure resets it. */
 int fz_set_lsf(fz_ctx* ctx, const double* lsf_kernel, const int32_t* lsf_valid);
 
+/* Per-pixel spectral windows: each of the ncol pixels observes its own
+ * sub-range of every configured region.  The reference reads one set of
+ * regions for the whole map (NREAD_LINES, input_file.f90:253-367: NUML
+ * regions of NUMWAVE points, NUMW = SUM(NUMWAVE) at :359); a map whose
+ * pixels carry different lines or window widths is run here in one launch instead of one run per
+ * line set.  windows[(col*nreg + r)*2 + {0,1}] = first wavelength index in
+ * region r and count (count 0: the region is not observed in that pixel).
+ * With windows set, fz_forward_dev (rf_mode NONE), fz_forward_nodes_dev,
+ * fz_lm_step_dev and fz_invert3d_dev take ncol equal to the ncol given here and
+ * read/write ragged spectra: pixel col's wavelengths are its windows in region
+ * order, numw_col = offsets[col+1]-offsets[col] of fz_window_offsets; Stokes
+ * rows [4][numw_col] start at 4*offsets[col]; observations and best synthesis
+ * rows [nstk][numw_col] at nstk*offsets[col]; node response functions
+ * [np][nstk][numw_col] at np*nstk*offsets[col].  Each pixel's chi^2 is
+ * normalised by its own nstk*numw_col.  Not combined with an LSF, response
+ * functions per height or the host-array entry points.  windows NULL switches
+ * windows off; fz_configure resets them. */
+int fz_set_windows(fz_ctx* ctx, int ncol, const int32_t* windows);
+
+/* offsets[0..ncol] (int64) of the ragged spectra set by fz_set_windows. */
+int fz_window_offsets(const fz_ctx* ctx, int64_t* offsets);
+
 /* FORWARD3D on device-resident data (forward.f90:43-306, per column the
  * FORWARD1D of forward.f90:857-1029 with HYDROSTATIC=no).
  *   d_atm    [12][nz][ncol] float, in/out: T, Pg, rho, Pe, mu, Bx, By, Bz, vx,
