@@ -36,6 +36,10 @@
 namespace fz {
 
 constexpr int MAXL = 16, MAXR = 8, MAXS = 32, MAXC = 1024, MAXNZ = 512, MAXBLK = 128;
+// internal response set of the inversion forward when neither Pg, rho nor P0
+// is inverted and HYDROSTATIC is off: slots T|Pg, Bx, By, Bz, v_z only (the
+// Pg|T and rho|T directional derivatives are not formed)
+constexpr int RF_CART_TBV = 4;
 constexpr int RT_WAVE = 64;   // wavelengths per radiative-transfer workgroup (one wave)
 constexpr int NATM = 12;
 enum { A_T = 0, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VX, A_VY, A_VZ, A_TAU };
@@ -82,9 +86,9 @@ __device__ __forceinline__ int freg(int r, int q) { return F_REG + 4 * r + q; }
 // split into:
 //   saha_kernel  one thread per (height, column): partition functions and the
 //                Saha factors of all 92 elements -> sbuf[k][e][4][column]
-//   rho_kernel   one thread per column, heights bottom->top: only the warm-
-//                started electron-density fixed point (GET_RHO), reading the
-//                factors from sbuf -> rho, mu, Pe and (n_e, N_H) per height
+//   rho_group_kernel  eight lanes per column, heights bottom->top: only the
+//                warm-started electron-density fixed point (GET_RHO), reading
+//                the factors from sbuf -> rho, mu, Pe and (n_e, N_H) per height
 //   opac_kernel  one thread per (height, column): implicit EOS derivatives,
 //                continuum and line opacity, damping -> the per-height record
 //   tau_kernel   one thread per column: the REAL(4) log tau5 trapezoid
@@ -253,28 +257,6 @@ __global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
   if (active && q == 0) a.status[col] = err;
 }
 
-__global__ void __launch_bounds__(64) rho_kernel(MicroTile a) {
-  const int ct = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ct >= a.ntile) return;
-  const int nz = c_cfg.nz, col = a.col0 + ct;
-  const long cs = a.ncol, ps = (long)nz * a.ncol;
-  double ne = 0.0;
-  int err = ERR_OK;
-  for (int k = 0; k < nz; ++k) {
-    const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
-    const SahaView s(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile);
-    double nh, pel, dens, mu;
-    err = get_rho(s, T, P, ne, nh, pel, dens, mu);
-    if (err) break;
-    a.atm[A_RHO * ps + k * cs + col] = (float)dens;
-    a.atm[A_MU * ps + k * cs + col] = (float)mu;
-    a.atm[A_PEL * ps + k * cs + col] = (float)pel;
-    a.eos[(long)k * a.ntile + ct] = ne;
-    a.eos[((long)nz + k) * a.ntile + ct] = nh;
-  }
-  a.status[col] = err;
-}
-
 // The implicit EOS derivatives (GET_DNHYDDTEMP and its four solves,
 // chemical_equilibrium.f90:2008-2052) per (height, column), kept apart from
 // the opacities so that neither kernel carries the other's registers.
@@ -380,8 +362,8 @@ __global__ void __launch_bounds__(64) tau_kernel(MicroTile a) {
 // ---------------------------------------------------------------- hydrostatic equilibrium
 // RK4_INTEGRATION_LOG (hydrostatic_tools.f90:150-222, RK4_LOG :350-396) over
 // a tile: the Saha factors at the heights come from saha_kernel, those at the
-// RK4 mid-step temperatures from saha_mid_kernel, and he_rk4_kernel (one
-// thread per column, top -> bottom) runs only the fixed points.
+// RK4 mid-step temperatures from saha_mid_kernel, and he_rk4_group_kernel
+// (eight lanes per column, top -> bottom) runs only the fixed points.
 __device__ __forceinline__ double he_tmid(int k, double tm0, double tm1) {
   const double dz = c_cfg.he_dz[k];
   return tm0 / dz * c_cfg.he_a[k] + tm1 / dz * c_cfg.he_b[k];
@@ -407,14 +389,6 @@ __global__ void __launch_bounds__(256) saha_mid_kernel(MicroTile a, double* __re
   }
 }
 
-template <class S>
-__device__ __forceinline__ double he_step_rhs(const S& s, double lnp, double tem, double& nel, int& err) {
-  double nh, pel, dens, mw;
-  const int e = get_rho(s, tem, exp(lnp), nel, nh, pel, dens, mw);
-  if (e) err = e;
-  return -mw * MAMU * GRAV / (KBOL * tem);          // RK4_SOLVE_LOG (hydrostatic_tools.f90:53-71)
-}
-
 __device__ __forceinline__ double he_step_rhs_group(const double* base, long stride, int q, double* sh, double lnp,
                                                     double tem, double& nel, int& err) {
   const RhoOut o = rho_group_at(base, stride, q, sh, tem, exp(lnp), nel);
@@ -424,8 +398,10 @@ __device__ __forceinline__ double he_step_rhs_group(const double* base, long str
 }
 
 // RK4_INTEGRATION_LOG by groups of RHO_G lanes per column, each fixed point as
-// get_rho_group (same values as he_rk4_kernel's get_rho, bit for bit); the
-// integration scalars are identical in the lanes of a group.
+// get_rho_group (the values of the one-thread get_rho, bit for bit); the
+// integration scalars are identical in the lanes of a group.  Writes Pg, rho,
+// Pe, mu of every height and, for the forward, (n_e, N_H) with n_e = Pe/(k T)
+// from the REAL(4) Pe and T (forward.f90:904).
 __global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const double* __restrict__ smid) {
   __shared__ double shall[256 / RHO_G][2 * RHO_LD];
   const int lane = threadIdx.x & 63;
@@ -482,51 +458,6 @@ __global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const do
     for (int k = 0; k < nz; ++k)
       a.eos[(long)k * a.ntile + cc] = (double)A(A_PEL, k) / KBOL / (double)A(A_T, k);
   if (wr) a.status[col] = err;
-}
-
-// writes Pg, rho, Pe, mu of every height and, for the forward, (n_e, N_H)
-// with n_e = Pe/(k T) from the REAL(4) Pe and T (forward.f90:904)
-__global__ void __launch_bounds__(64) he_rk4_kernel(MicroTile a, const double* __restrict__ smid) {
-  const int ct = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ct >= a.ntile) return;
-  const int nz = c_cfg.nz, col = a.col0 + ct;
-  const long cs = a.ncol, ps = (long)nz * a.ncol;
-  auto A = [&](int p, int k) -> float& { return a.atm[p * ps + k * cs + col]; };
-  auto SH = [&](int k) { return SahaView(a.sbuf + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile); };
-  auto SM = [&](int k) { return SahaView(smid + (long)k * FZ_NELEM * 4 * a.ntile + ct, a.ntile); };
-  int err = 0;
-  const int top = nz - 1;
-  double u0 = log((double)A(A_PG, top));
-  double nel = (double)A(A_PEL, top) / KBOL / (double)A(A_T, top);
-  double nh, pel, dens, mw;
-  err = get_rho(SH(top), (double)A(A_T, top), exp(u0), nel, nh, pel, dens, mw);
-  A(A_PG, top) = (float)exp(u0);
-  A(A_RHO, top) = (float)dens;
-  A(A_PEL, top) = (float)pel;
-  A(A_MU, top) = (float)mw;
-  if (a.eos) a.eos[((long)nz + top) * a.ntile + ct] = nh;
-  for (int k = nz - 2; k >= 0 && !err; --k) {
-    const double dz = c_cfg.he_dz[k];
-    const double tm0 = (double)A(A_T, k + 1), tm1 = (double)A(A_T, k);
-    const double k1 = he_step_rhs(SH(k + 1), u0, tm0, nel, err) * dz;
-    const double tmid = he_tmid(k, tm0, tm1);
-    const double k2 = he_step_rhs(SM(k), u0 + k1 / 2.0, tmid, nel, err) * dz;
-    const double k3 = he_step_rhs(SM(k), u0 + k2 / 2.0, tmid, nel, err) * dz;
-    const double k4 = he_step_rhs(SH(k), u0 + k3, tm1, nel, err) * dz;
-    const double u1 = u0 + 1.0 / 6.0 * (k1 + 2.0 * k2 + 2.0 * k3 + k4);
-    const int e = get_rho(SH(k), tm1, exp(u1), nel, nh, pel, dens, mw);
-    if (e) err = e;
-    A(A_PG, k) = (float)exp(u1);
-    A(A_RHO, k) = (float)dens;
-    A(A_PEL, k) = (float)pel;
-    A(A_MU, k) = (float)mw;
-    if (a.eos) a.eos[((long)nz + k) * a.ntile + ct] = nh;
-    u0 = u1;
-  }
-  if (a.eos)
-    for (int k = 0; k < nz; ++k)
-      a.eos[(long)k * a.ntile + ct] = (double)A(A_PEL, k) / KBOL / (double)A(A_T, k);
-  a.status[col] = err;
 }
 
 // ---------------------------------------------------------------- radiative transfer
@@ -777,7 +708,9 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   a.items = items;
   a.poff = poff;
   // parameters: 5 spherical, 7 Cartesian, 8 with P0 (the top pressure)
-  constexpr int NP = RF == FZ_RF_SPHERICAL ? 5 : (RF == FZ_RF_CARTESIAN ? 7 : (RF == FZ_RF_CARTESIAN_P0 ? 8 : 0));
+  constexpr int NP = (RF == FZ_RF_SPHERICAL || RF == RF_CART_TBV)
+                         ? 5
+                         : (RF == FZ_RF_CARTESIAN ? 7 : (RF == FZ_RF_CARTESIAN_P0 ? 8 : 0));
   constexpr int NPL = RF == FZ_RF_CARTESIAN_P0 ? 7 : NP;    // parameters with a local response
   const int numw = c_cfg.numw, nz = c_cfg.nz;
   // workgroup -> (column, region, first wavelength); lane -> wavelength
@@ -924,7 +857,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
         constexpr int dirs_car[7] = {D_T, D_P, D_R, D_B, D_G, D_F, D_V};
 #pragma unroll
         for (int q = 0; q < NPL; ++q) {
-          const int dd = RF == FZ_RF_SPHERICAL ? dirs_sph[q] : dirs_car[q];
+          const int dd = (RF == FZ_RF_SPHERICAL || RF == RF_CART_TBV) ? dirs_sph[q] : dirs_car[q];
           if (HEX && dd != D_T && dd != D_P) evol_dir(E, dx[dd], dz, vo, m4vo, loc[dd]);
           else evol_dir(E, dx[dd], dz, v, m4v, loc[dd]);
         }
@@ -960,15 +893,19 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
           const float dB[3] = {bx * iB, by * iB, bz * iB};
           const float dG[3] = {bz * bx * iBt, bz * by * iBt, hasB ? -bt / B2 : 0.f};
           const float dF[3] = {-by * ibt2, bx * ibt2, 0.f};
+          // slots T, (Pg, rho,) Bx, By, Bz, v_z
+          constexpr int o = RF == RF_CART_TBV ? 0 : 2;
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             lp[0][i] = loc[D_T][i];
-            lp[1][i] = loc[D_P][i];
-            lp[2][i] = loc[D_R][i];
+            if constexpr (RF != RF_CART_TBV) {
+              lp[1][i] = loc[D_P][i];
+              lp[2][i] = loc[D_R][i];
+            }
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-              lp[3 + j][i] = dB[j] * loc[D_B][i] + dG[j] * loc[D_G][i] + dF[j] * loc[D_F][i];
-            lp[6][i] = loc[D_V][i];
+              lp[1 + o + j][i] = dB[j] * loc[D_B][i] + dG[j] * loc[D_G][i] + dF[j] * loc[D_F][i];
+            lp[4 + o][i] = loc[D_V][i];
           }
         }
 #pragma unroll
@@ -1222,6 +1159,7 @@ struct fz_ctx {
   bool hydro = false;         // HYDROSTATIC: yes
   size_t lm_lds = 0;
   bool lm_p0 = false;         // P0 among the inverted parameters
+  bool lm_pr = false;         // Pg or rho among the inverted parameters
   // per-pixel spectral windows
   bool win_on = false;
   int win_ncol = 0;
@@ -1589,7 +1527,11 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
 }
 
 static int rf_nparam(int rf_mode) {
-  return rf_mode == FZ_RF_SPHERICAL ? 5 : rf_mode == FZ_RF_CARTESIAN ? 7 : rf_mode == FZ_RF_CARTESIAN_P0 ? 8 : 0;
+  return rf_mode == FZ_RF_SPHERICAL ? 5
+         : rf_mode == FZ_RF_CARTESIAN ? 7
+         : rf_mode == FZ_RF_CARTESIAN_P0 ? 8
+         : rf_mode == RF_CART_TBV ? 5
+         : 0;
 }
 
 static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, int nitems = 0) {
@@ -1602,6 +1544,10 @@ static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, in
     hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
     hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (rf_mode == RF_CART_TBV) {
+    if (c->hydro) return set_err(c, FZ_ECONFIG, "reduced response set without HYDROSTATIC only");
+    hipLaunchKernelGGL((rt_kernel<RF_CART_TBV, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  }
   else if (c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
     hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, true>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro) hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false>), grid, blk, 0, st, FZ_RT_ARGS);
@@ -2072,6 +2018,7 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   L.colsum = c->d_colsum;
   c->inv_set = true;
   c->lm_p0 = inv->atmpar[7] != 0;
+  c->lm_pr = inv->atmpar[1] != 0 || inv->atmpar[2] != 0;
   if (ifreep) *ifreep = np;
   if (nfreq) *nfreq = L.nfreq;
   return FZ_OK;
@@ -2092,9 +2039,13 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   if (r) return r;
   r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
   if (r) return r;
-  // the P0 slot is synthesised only when P0 is inverted
+  // the P0 slot is synthesised only when P0 is inverted; the Pg and rho
+  // slots only when they are inverted or feed another response (P0, HYDROSTATIC)
   const bool p0 = c->lm_p0;
-  const int nslot = p0 ? 8 : 7;
+  const bool tbv = !p0 && !c->lm_pr && !c->hydro;
+  const int rf = p0 ? FZ_RF_CARTESIAN_P0 : (tbv ? RF_CART_TBV : FZ_RF_CARTESIAN);
+  const int nslot = rf_nparam(rf);
+  static const int tbv_slot[8] = {0, -1, -1, 1, 2, 3, 4, -1};
   // columns per tile: the configured tile, fewer where one tile's response
   // functions and scratch would pass 24 GiB; sized by the widest tile (the
   // wavelengths of its columns: ragged with per-pixel windows)
@@ -2129,10 +2080,24 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   e.nslot = nslot;
   for (int s = 0; s < 4; ++s) e.rowstk[s] = c->lm.rowstk[s];
   for (int j = 0; j < e.np; ++j) {
-    e.slot_of[j] = c->slot_of[j];
+    e.slot_of[j] = tbv ? tbv_slot[c->slot_of[j]] : c->slot_of[j];
     e.kbeg[j] = c->lm.kbeg[j];
     e.kend[j] = c->lm.kend[j];
   }
+  e.nparam = c->lm.nparam;
+  for (int ip = 0; ip < e.nparam; ++ip) {
+    e.j0[ip] = c->lm.j0[ip];
+    e.nj[ip] = c->lm.nj[ip];
+  }
+  // one pass per parameter when its node supports overlap at most pairwise
+  // and move up with the node index (the hat basis of GET_INT_COEFS does)
+  e.pairs = 1;
+  for (int ip = 0; ip < e.nparam; ++ip)
+    for (int j = e.j0[ip]; j < e.j0[ip] + e.nj[ip]; ++j) {
+      if (e.kbeg[j] > e.kend[j]) e.pairs = 0;
+      if (j + 1 < e.j0[ip] + e.nj[ip] && (e.kbeg[j + 1] < e.kbeg[j] || e.kend[j + 1] < e.kend[j])) e.pairs = 0;
+      if (j + 2 < e.j0[ip] + e.nj[ip] && e.kbeg[j + 2] <= e.kend[j]) e.pairs = 0;
+    }
   e.coefs = c->d_coefs;
   e.eqv = d_eqv;
   e.poff = c->win_on ? c->d_poff : nullptr;
@@ -2157,7 +2122,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
       a.poff = c->d_poff;
       nit = c->h_item_first[c0 + a.ntile] - c->h_item_first[c0];
     }
-    r = launch_rt(c, p0 ? FZ_RF_CARTESIAN_P0 : FZ_RF_CARTESIAN, a, st, nit);
+    r = launch_rt(c, rf, a, st, nit);
     if (r) return r;
     e.ncol_tile = a.ntile;
     e.col0 = c0;

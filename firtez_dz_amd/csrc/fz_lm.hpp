@@ -18,9 +18,10 @@
 namespace fz {
 
 constexpr int LM_MAXNP = 64;     // free parameters per pixel (nodes summed over parameters)
-constexpr int LM_THREADS = 256;
+// one wave per pixel: the Jacobi rounds and reductions are wave-local, so
+// the barriers are cheap and a CU holds many pixels at once
+constexpr int LM_THREADS = 64;
 constexpr int LM_CH = 64;        // spectral rows per LDS chunk
-constexpr int LM_SLOTS = (LM_MAXNP * (LM_MAXNP + 1) / 2 + LM_MAXNP + LM_THREADS - 1) / LM_THREADS;
 
 // get_dmodel.f90:31 (PARAM_NORMS) and :1903-1919 (hard bounds)
 __constant__ double c_lm_norm[8] = {6000.0, 1.0e5, 1.0e-5, 1.0e3, 1.0e3, 1.0e3, 1.0e6, 6.0e2};
@@ -72,6 +73,12 @@ struct EqvArgs {
   int rowstk[4];
   int slot_of[LM_MAXNP];       // response-function slot of each free parameter
   int kbeg[LM_MAXNP], kend[LM_MAXNP];
+  // the nodes of inverted parameter ip are j0[ip] .. j0[ip]+nj[ip]-1; pairs:
+  // every height lies in the support of at most two consecutive nodes of a
+  // parameter and the supports move up with the node index (hat functions),
+  // so one pass over the heights feeds two running sums
+  int nparam, pairs;
+  int j0[8], nj[8];
   const double* coefs;         // [np][nz]
   const float* jac;            // tile: [ntile][nslot][nz][4][numw], ragged with windows
   float* eqv;                  // [ncol][np][nfreq], ragged with windows
@@ -120,6 +127,31 @@ __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
     jc = a.jac + (long)ct * a.nslot * a.nz * cs + a.rowstk[b] * a.numw + l;
     fstride = a.nfreq;
     out = a.eqv + ((long)(a.col0 + ct) * a.np) * a.nfreq + f;
+  }
+  if (a.pairs) {
+    // each node's sum is formed in the same order (heights ascending from
+    // its support's start) as the per-node loop below: identical results,
+    // half the reads where the hat supports overlap
+    for (int ip = 0; ip < a.nparam; ++ip) {
+      const int jb = a.j0[ip], je = jb + a.nj[ip] - 1;
+      const float* jp = jc + (long)a.slot_of[jb] * a.nz * cs;
+      int jl = jb;                                   // lowest unfinished node
+      double alo = 0.0, ahi = 0.0;                   // sums of nodes jl, jl+1
+      for (int k = a.kbeg[jb]; k <= a.kend[je]; ++k) {
+        while (k > a.kend[jl]) {
+          out[(long)jl * fstride] = (float)alo;
+          alo = ahi;
+          ahi = 0.0;
+          ++jl;
+        }
+        const double v = (double)jp[k * cs];
+        if (k >= a.kbeg[jl]) alo += v * a.coefs[(long)jl * a.nz + k];
+        if (jl < je && k >= a.kbeg[jl + 1]) ahi += v * a.coefs[(long)(jl + 1) * a.nz + k];
+      }
+      out[(long)jl * fstride] = (float)alo;
+      if (jl < je) out[(long)(jl + 1) * fstride] = (float)ahi;
+    }
+    return;
   }
   for (int j = 0; j < a.np; ++j) {
     const float* jp = jc + (long)a.slot_of[j] * a.nz * cs;
@@ -240,6 +272,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   float* bum = mod + 12 * nz;                  // [nz]
   float* ftmp = bum + nz;                      // [nz]
   int* izero = (int*)(ftmp + nz);              // [np] zeroed response columns
+  int* qtab = izero + np;                      // [ntri + np] (i << 16 | j) of each normal-matrix / gradient sum
   __shared__ double s_sla[9];
   __shared__ int s_accept, s_conv;
 
@@ -325,15 +358,11 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   const float* jsrc = accept ? ecur : ebest;
   const double snu = sqrt(inu);
   const int ntri = np * (np + 1) / 2, nq = ntri + np;
-  double acc[LM_SLOTS];
-  int qi[LM_SLOTS], qj[LM_SLOTS];
-#pragma unroll
-  for (int u = 0; u < LM_SLOTS; ++u) {
-    acc[u] = 0.0;
-    const int q = t + u * LM_THREADS;
-    int i = -1, j = -1;
+  // sums of the normal matrix (upper triangle, row-major) and of the
+  // gradient, accumulated in LDS over the chunks in row order
+  for (int q = t; q < nq; q += LM_THREADS) {
+    int i, j;
     if (q < ntri) {
-      // row-major upper triangle: find i with q in [i*np - i(i-1)/2, ...)
       int r = 0, base = 0;
       while (q >= base + (np - r)) {
         base += np - r;
@@ -341,13 +370,15 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       }
       i = r;
       j = r + (q - base);
-    } else if (q < nq) {
+    } else {
       i = q - ntri;
-      j = -2;  // gradient entry
+      j = 0xffff;                              // gradient entry
     }
-    qi[u] = i;
-    qj[u] = j;
+    qtab[q] = (i << 16) | j;
   }
+  for (int e = t; e < npe * ld; e += LM_THREADS) A[e] = 0.0;
+  for (int i = t; i < np; i += LM_THREADS) g[i] = 0.0;
+  __syncthreads();
   for (int f0 = 0; f0 < nf; f0 += LM_CH) {
     const int nc = min(LM_CH, nf - f0);
     for (int e = t; e < nc * np; e += LM_THREADS) {
@@ -363,35 +394,28 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       rch[c] = (obs[f] - bsyn[f]) * sqrt(isig(f)) / snu;
     }
     __syncthreads();
-#pragma unroll
-    for (int u = 0; u < LM_SLOTS; ++u) {
-      const int i = qi[u], j = qj[u];
-      if (i < 0) continue;
-      double s = acc[u];
-      if (j >= 0) {
+    for (int q = t; q < nq; q += LM_THREADS) {
+      const int i = qtab[q] >> 16, j = qtab[q] & 0xffff;
+      if (j != 0xffff) {
+        double s = A[i * ld + j];
         for (int c = 0; c < nc; ++c) s += Jn[c * np + i] * Jn[c * np + j];
+        A[i * ld + j] = s;
       } else {
+        double s = g[i];
         for (int c = 0; c < nc; ++c) s += Jn[c * np + i] * rch[c];
+        g[i] = s;
       }
-      acc[u] = s;
     }
     __syncthreads();
   }
   const double lam = s_sla[8];
   // A = H + REG_HSS, diag += diag * lambda; g = J^T r - REG_PEN
-  for (int e = t; e < npe * ld; e += LM_THREADS) A[e] = 0.0;
-  __syncthreads();
-#pragma unroll
-  for (int u = 0; u < LM_SLOTS; ++u) {
-    const int i = qi[u], j = qj[u];
-    if (i < 0) continue;
-    if (j >= 0) {
-      A[i * ld + j] = acc[u];
-      A[j * ld + i] = acc[u];
-    } else {
-      g[i] = acc[u] - (a.regul ? pen_res[i] : 0.0);
-    }
+  for (int e = t; e < np * np; e += LM_THREADS) {
+    const int i = e / np, j = e - i * np;
+    if (j < i) A[i * ld + j] = A[j * ld + i];
   }
+  for (int i = t; i < np; i += LM_THREADS)
+    if (a.regul) g[i] -= pen_res[i];
   __syncthreads();
   if (a.regul)
     for (int e = t; e < np * np; e += LM_THREADS) {
@@ -660,7 +684,7 @@ inline size_t lm_lds_bytes(int np, int nz, bool regul) {
   const int npe = np + (np & 1), ld = np + 1;
   size_t d = (size_t)2 * npe * ld + (size_t)LM_CH * np + LM_CH + 3 * np + LM_THREADS + npe + nz + 4 * np + np +
              (regul ? (size_t)np * np : 0);
-  size_t f = (size_t)12 * nz + 2 * nz + np;   // floats + ints
+  size_t f = (size_t)12 * nz + 2 * nz + np + (size_t)np * (np + 1) / 2 + np;   // floats + ints
   return d * 8 + f * 4;
 }
 

@@ -66,7 +66,12 @@ NODES_C1 = {"T": 2, "Bx": 1, "By": 1, "Bz": 1, "vz": 1}
 NODES_C2 = {"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}
 
 
-@pytest.mark.parametrize("nodes", [NODES_C1, NODES_C2], ids=["cycle1", "cycle2"])
+NODES_PG = {"T": 2, "Pg": 1, "Bx": 1, "By": 1, "Bz": 1, "vz": 1}
+
+
+# cycle1/cycle2 invert T, B and v_z (the device forms only those five
+# response slots); with-Pg adds Pg|T and runs the full Cartesian set
+@pytest.mark.parametrize("nodes", [NODES_C1, NODES_C2, NODES_PG], ids=["cycle1", "cycle2", "with-Pg"])
 def test_eqvdsyn_vs_oracle(gpu_ctx, cpu, nodes):
     import torch
     P = problem(gpu_ctx, cpu, nodes)
