@@ -1,0 +1,380 @@
+"""Reference control files through the device path.
+
+`parse` reads a FIRTEZ-dz control file (READ_INPUT_FILE,
+input_file.f90:371-707, with the line reader of :60-98 and the field
+readers SPLIT_READ_LINE_WEIGHTS / _STRING / GET_NUMBER, :1310-1420,
+1868-1916) into a `Control`; `run` executes it the way the reference's main
+program does -- one FORWARD3D for MODE synthesis / tau / hydrostatic
+(invert3d.f90:158-177), INVERT3D for MODE inversion -- and writes the
+reference's output files (DEALLOCATE_AND_WRITE, pre_post_duties.f90:108-355):
+
+  synthesis    OUTPATH/<FILEPROFILE>, OUTPATH/out_<FILEMODEL>, and with
+               WRITE RESPONSE FUNCTION: yes OUTPATH/rfout_<FILEMODEL>
+  inversion    OUTPATH/out_<FILEPROFILE> (inverted Stokes parameters only),
+               OUTPATH/out_<FILEMODEL>; observations from DATAPATH/<FILEPROFILE>
+  tau          OUTPATH/out_<FILEMODEL> (log tau5 from the forward's EOS)
+  hydrostatic  OUTPATH/out_<FILEMODEL> (Pg, rho, Pe, mu from RK4 from the top)
+
+Supported fields: LINES, FILEPROFILE, FILEMODEL, MODE, HYDROSTATIC,
+HYDROSTATIC DERIVATIVES, STOKES SETUP, INVERSION SETUP (parameters with their
+maximum nodes, POW, MAXITER), VERTICAL REGULARIZATION TYPE / NORM, WRITE
+RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
+SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
+package's table of the reference's lines_database.dat (SLDPATH is read but
+not needed).  Fields of features this runner does not implement (BOX,
+CONTINUUM NORMALIZATION, COUPLED INVERSION, LINE OPTICAL DEPTH, NLTE
+DEPARTURE COEFFICIENTS, LINE SPREAD FUNCTION -- the library's fz_set_lsf
+takes LSF_KERNEL directly --, AUTO Stokes weights, the ASSIST_*
+initialisers, FULL_STOKES) raise NotImplementedError instead of being
+ignored.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import gpu as _gpu
+from . import lines as _lines
+from . import modelio
+
+# INVERSION SETUP names (NSET_VAR_INVERSION, input_file.f90:1100-1115) -> INV_PARAMS
+PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
+# default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
+_REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
+_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "COUPLED INVERSION:", "LINE OPTICAL DEPTH:",
+                "NLTE DEPARTURE COEFFICIENTS:", "LINE SPREAD FUNCTION:")
+
+
+@dataclass
+class Control:
+    lines: List[Tuple[Tuple[int, ...], int, float, float]] = field(default_factory=list)
+    fileprofile: Optional[str] = None
+    filemodel: Optional[str] = None
+    mode: Optional[str] = None              # synthesis, inversion, tau, hydrostatic
+    hydrostatic: bool = False
+    hydro_ders: bool = True
+    invstk: List[bool] = field(default_factory=lambda: [False] * 4)
+    wstk: List[float] = field(default_factory=lambda: [0.0] * 4)
+    params: Dict[str, Optional[int]] = field(default_factory=dict)   # reference name -> NSLB_MAX (None: NZ)
+    cycpow: int = 2
+    maxsteps: int = 5
+    vregul: bool = False
+    pen_typ: Dict[str, int] = field(default_factory=dict)
+    pen_alp: Dict[str, float] = field(default_factory=dict)
+    write_rf: bool = False
+    datapath: str = ""
+    modpath: str = ""
+    outpath: str = ""
+    sldpath: str = ""
+    noise: float = 4.0e-3
+    svdtol: float = 0.0                     # 0: the library's default (RSVDTOL 1e-4)
+    solver: int = 1
+    node_expansion: int = 1
+
+
+def _number(tok: str) -> float:
+    """GET_NUMBER (input_file.f90:1310-1352): integers, reals and the
+    E/D exponent forms."""
+    t = tok.replace("D", "E").replace("d", "E")
+    return float(t)
+
+
+def _nint(v: float) -> int:
+    """Fortran NINT: halves away from zero."""
+    return int(np.sign(v) * np.floor(abs(v) + 0.5))
+
+
+def _field(line: str, match: str, default):
+    """SPLIT_READ_LINE_WEIGHTS: (present, value) where value is the second
+    column when given, else default."""
+    cols = line.split()
+    if not cols or cols[0] != match:
+        return False, None
+    if len(cols) > 1:
+        return True, _number(cols[1])
+    return True, default
+
+
+def _string(line: str, match: str):
+    cols = line.split()
+    if len(cols) >= 2 and cols[0] == match:
+        return cols[1]
+    return None
+
+
+def read_lines(text: str) -> List[str]:
+    """The reader of input_file.f90:60-98: adjusted lines, no empty lines, no
+    comment lines, '#' comments cut, everything after END: dropped."""
+    out = []
+    for raw in text.splitlines():
+        ln = raw.strip()
+        if not ln or ln.startswith("#"):
+            continue
+        if "#" in ln:
+            ln = ln[:ln.index("#")]
+        if ln.strip() == "END:":
+            break
+        out.append(ln)
+    return out
+
+
+def parse(text: str) -> Control:
+    """Control from the text of a control file."""
+    ls = read_lines(text)
+    c = Control()
+    nstk = 0
+    i = 0
+
+    def section(start):
+        """lines of the section that starts after index start (up to the next
+        line with a colon)"""
+        j = start + 1
+        while j < len(ls) and ":" not in ls[j]:
+            j += 1
+        return ls[start + 1:j], j - 1
+
+    while i < len(ls):
+        key = ls[i].strip()
+        if key in _UNSUPPORTED:
+            raise NotImplementedError("control field %s is not supported by this path" % key)
+        if key == "LINES:":
+            body, i = section(i)
+            for row in body:
+                cols = row.split()
+                ids = tuple(int(v) for v in cols[0].split(",") if v)
+                c.lines.append((ids, int(_number(cols[1])), _number(cols[2]), _number(cols[3])))
+        elif key in ("FILEPROFILE:", "FILEMODEL:", "MODE:", "HYDROSTATIC:", "HYDROSTATIC DERIVATIVES:",
+                     "WRITE RESPONSE FUNCTION:"):
+            i += 1
+            val = ls[i].strip() if i < len(ls) else ""
+            if key == "FILEPROFILE:":
+                c.fileprofile = val
+            elif key == "FILEMODEL:":
+                c.filemodel = val
+            elif key == "MODE:":
+                m = val[:1].lower()
+                c.mode = {"s": "synthesis", "i": "inversion", "t": "tau", "h": "hydrostatic"}.get(m)
+                if c.mode is None:
+                    raise ValueError('"MODE:" must be synthesis, inversion, tau or hydrostatic')
+            elif key == "HYDROSTATIC:":
+                c.hydrostatic = val[:1] in ("y", "Y")
+            elif key == "HYDROSTATIC DERIVATIVES:":
+                c.hydro_ders = val[:1] not in ("n", "N")
+            else:
+                if val[:1] in ("y", "Y"):
+                    c.write_rf = True
+        elif key == "STOKES SETUP:":
+            body, i = section(i)
+            for row in body:
+                for s, name in enumerate(("STKI", "STKQ", "STKU", "STKV")):
+                    ok, w = _field(row, name, 1)
+                    if ok:
+                        c.invstk[s] = True
+                        c.wstk[s] = float(w)
+                        nstk += 1
+                if _field(row, "AUTO", 1)[0]:
+                    raise NotImplementedError("automatic Stokes weights are not supported by this path")
+        elif key == "INVERSION SETUP:":
+            body, i = section(i)
+            for row in body:
+                for name in PARAM_NAMES:
+                    ok, v = _field(row, name, None)
+                    if ok:
+                        c.params[name] = None if v is None else _nint(v)
+                ok, v = _field(row, "POW", 1)
+                if ok:
+                    c.cycpow = _nint(v)
+                ok, v = _field(row, "MAXITER", 1)
+                if ok:
+                    c.maxsteps = _nint(v)
+        elif key in ("VERTICAL REGULARIZATION TYPE:", "VERTICAL REGULARIZATION NORM:"):
+            body, i = section(i)
+            c.vregul = True
+            for row in body:
+                for name in PARAM_NAMES:
+                    ok, v = _field(row, name, _REG_DEF[name])
+                    if ok:
+                        if key.endswith("TYPE:"):
+                            c.pen_typ[name] = _nint(v)
+                        else:
+                            c.pen_alp[name] = float(v)
+        elif key == "MISC SETUP:":
+            body, i = section(i)
+            for row in body:
+                for name in ("TEMP", "PGAS", "BVEC", "VLOS", "FULL_STOKES"):
+                    if _field(row, name, 0)[0]:
+                        raise NotImplementedError("MISC SETUP %s is not supported by this path" % name)
+                ok, v = _field(row, "SVDTOL", 0)
+                if ok:
+                    c.svdtol = float(v)
+                ok, v = _field(row, "NOISE", 0)
+                if ok:
+                    c.noise = float(np.float32(v))
+                ok, v = _field(row, "LINEAR_SOLVER", 1)
+                if ok:
+                    c.solver = _nint(v)
+                ok, v = _field(row, "PERTURBATION_TYPE", 1)
+                if ok:
+                    c.node_expansion = _nint(v)
+                for name, attr in (("DATAPATH", "datapath"), ("MODPATH", "modpath"), ("OUTPATH", "outpath"),
+                                   ("SLDPATH", "sldpath")):
+                    s = _string(row, name)
+                    if s is not None:
+                        setattr(c, attr, s)
+        i += 1
+    # READ_INPUT_FILE's checks and defaults (input_file.f90:580-705)
+    if c.filemodel is None:
+        raise ValueError('the control file must contain a non-empty "FILEMODEL:" field')
+    if c.mode is None:
+        raise ValueError('the control file must contain a "MODE:" field')
+    if c.mode == "synthesis" and nstk == 0:
+        c.invstk = [True] * 4
+        c.wstk = [1.0] * 4
+    if c.mode == "inversion":
+        if not any(c.invstk):
+            raise ValueError('"STOKES SETUP:" is mandatory in inversion mode')
+        if not c.params:
+            raise ValueError('"INVERSION SETUP:" must name at least one parameter in inversion mode')
+    if c.mode in ("synthesis", "inversion"):
+        if not c.lines:
+            raise ValueError('missing spectral lines ("LINES:")')
+        if c.fileprofile is None:
+            raise ValueError('"FILEPROFILE:" is required in synthesis and inversion modes')
+    return c
+
+
+def parse_file(path: str) -> Control:
+    with open(path) as fh:
+        return parse(fh.read())
+
+
+def line_setup(c: Control) -> _lines.LineSetup:
+    """The configured regions; a mode without LINES (tau, hydrostatic) gets
+    one wavelength of Fe I 6301.5, which leaves the model outputs unchanged
+    (the reference sets up placeholder lines the same way, SETUP_FAKE_LINES)."""
+    if c.lines:
+        return _lines.setup(c.lines)
+    return _lines.setup([((4,), 1, 0.0, 1.0)])
+
+
+def _wave_index(ls: _lines.LineSetup):
+    """WAVE and INDEX of WAVE_INIT (pre_post_duties.f90:592-621)."""
+    wave = ls.wave()
+    index = np.concatenate([np.full(r.nwave, ls.lines[r.lines[0]].lambda0) for r in ls.regions])
+    return wave, index
+
+
+def control_for_inversion(c: Control, nz: int) -> _gpu.CInvertControl:
+    nodes = {PARAM_NAMES[k]: (nz if v is None else v) for k, v in c.params.items()}
+    return _gpu.invert_control(nodes, invstk=[int(b) for b in c.invstk],
+                               wstk=[c.wstk[s] if c.invstk[s] else 0.0 for s in range(4)], noise=c.noise,
+                               cycpow=c.cycpow, maxsteps=c.maxsteps, solver=c.solver,
+                               node_expansion=c.node_expansion, svdtol=c.svdtol, smoothing=True,
+                               regul=c.vregul, pen_typ={PARAM_NAMES[k]: v for k, v in c.pen_typ.items()},
+                               pen_alp={PARAM_NAMES[k]: v for k, v in c.pen_alp.items()})
+
+
+def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> dict:
+    """Executes the control file in workdir (its relative paths are taken from
+    there, as the reference takes them from its working directory).  Returns
+    the paths written and, for an inversion, chi^2 and the LM iteration count."""
+    own = ctx is None
+    if own:
+        ctx = _gpu.Context(0)
+    try:
+        def path(p, name):
+            return os.path.join(workdir, p + name)
+
+        cube = modelio.read_model(path(c.modpath, c.filemodel))
+        z = cube["z_km"]
+        nz, ny, nx = cube["T"].shape
+        for name in ("vx", "vy"):
+            cube.setdefault(name, np.zeros_like(cube["T"], order="F"))
+        ls = line_setup(c)
+        ctx.configure(ls, z)
+        ctx.gain_hsra()
+        ctx.set_hydrostatic(c.hydrostatic)
+        if c.hydrostatic and not c.hydro_ders and c.mode == "inversion":
+            raise NotImplementedError("HYDROSTATIC DERIVATIVES: no is not supported by this path")
+        out = {"written": []}
+        wave, index = _wave_index(ls)
+        numw = ctx.numw
+        if c.mode == "hydrostatic":
+            _hydrostatic_cube(ctx, cube)
+        elif c.mode in ("synthesis", "tau"):
+            want_rf = c.mode == "synthesis" and c.write_rf
+            rf_mode = _gpu.RF_NONE
+            if want_rf:
+                rf_mode = _gpu.RF_CARTESIAN_P0 if "P0" in c.params else _gpu.RF_CARTESIAN
+            res = ctx.forward3d(cube, rf_mode=rf_mode)
+            syn3d, dsyn = (res if isinstance(res, tuple) else (res, None))
+            if c.mode == "synthesis":
+                st = _stokes4(syn3d, numw, ny, nx)
+                for s in range(4):
+                    if not c.invstk[s]:
+                        st[s] = 0.0
+                p = path(c.outpath, c.fileprofile)
+                modelio.write_profiles(p, st, wave, index)
+                out["written"].append(p)
+            if want_rf:
+                npar = _gpu.NPARAM[rf_mode]
+                # DSYN(NFREQ, IFREEP, NY, NX): NFREQ = Stokes-major blocks of
+                # NUMW, IFREEP = parameter-major blocks of NZ
+                d = np.asarray(dsyn).reshape((numw, 4, nz, npar, ny, nx), order="F")
+                w = np.zeros((8, 4, numw, nz, ny, nx), np.float32)
+                logv = np.zeros(8, bool)
+                for k in c.params:
+                    slot = _gpu.INV_PARAMS.index(PARAM_NAMES[k])
+                    logv[slot] = True
+                    w[slot] = d[:, :, :, slot].transpose(1, 0, 2, 3, 4)
+                for s in range(4):
+                    if not c.invstk[s]:
+                        w[:, s] = 0.0
+                p = path(c.outpath, "rfout_" + c.filemodel)
+                modelio.write_rfs(p, w, z, wave, logv, index)
+                out["written"].append(p)
+        else:
+            prof, pwave, _ = modelio.read_profiles(path(c.datapath, c.fileprofile))
+            if prof.shape[1] != numw or prof.shape[2] != ny or prof.shape[3] != nx:
+                raise ValueError("observed profiles (%s) do not match the lines and the model" % (prof.shape,))
+            rows = [s for s in range(4) if c.invstk[s]]
+            obs3d = np.asfortranarray(np.concatenate([prof[s] for s in rows], axis=0).astype(np.float64))
+            syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz))
+            st = _stokes4(syn3d, numw, ny, nx)
+            for s in range(4):
+                if not c.invstk[s]:
+                    st[s] = 0.0
+            p = path(c.outpath, "out_" + c.fileprofile)
+            modelio.write_profiles(p, st, wave, index)
+            out["written"].append(p)
+            out["chi2"] = chi2
+            out["iterations"] = its
+        p = path(c.outpath, "out_" + c.filemodel)
+        modelio.write_model(p, cube, x=cube["x"][0, 0, :], y=cube["y"][0, :, 0])
+        out["written"].append(p)
+        return out
+    finally:
+        if own:
+            ctx.close()
+
+
+def _stokes4(syn3d, numw, ny, nx):
+    """SYN3D(4*NUMW, NY, NX) (Stokes-major blocks) -> STOKES(4, NUMW, NY, NX)."""
+    return np.asarray(syn3d, np.float64).reshape((numw, 4, ny, nx), order="F").transpose(1, 0, 2, 3).copy()
+
+
+def _hydrostatic_cube(ctx, cube):
+    """RK4_INTEGRATION_LOG from the top on every column (MODE hydrostatic:
+    FORWARD1D stops after the integration, forward.f90:891-909)."""
+    import torch
+    nz, ny, nx = cube["T"].shape
+    atm = torch.from_numpy(modelio.to_device_rows(cube)).cuda()
+    st = ctx.hydrostatic_torch(atm)
+    if int(st.max()) != 0:
+        raise _gpu.FzError("hydrostatic integration failed in %d columns" % int((st != 0).sum()))
+    a = atm.cpu().numpy()
+    for i, name in ((1, "Pg"), (2, "rho"), (3, "Pe"), (4, "mu")):
+        cube[name][...] = np.asfortranarray(a[i].reshape(nz, nx, ny).transpose(0, 2, 1))

@@ -40,6 +40,7 @@ constexpr int MAXL = 16, MAXR = 8, MAXS = 32, MAXC = 1024, MAXNZ = 512, MAXBLK =
 // is inverted and HYDROSTATIC is off: slots T|Pg, Bx, By, Bz, v_z only (the
 // Pg|T and rho|T directional derivatives are not formed)
 constexpr int RF_CART_TBV = 4;
+
 constexpr int RT_WAVE = 64;   // wavelengths per radiative-transfer workgroup (one wave)
 constexpr int NATM = 12;
 enum { A_T = 0, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VX, A_VY, A_VZ, A_TAU };
@@ -1139,7 +1140,13 @@ struct fz_ctx {
   // inversion
   bool inv_set = false;
   LmArgs lm;                  // per-cycle constants; per-call pointers filled at launch
-  int slot_of[LM_MAXNP];
+  std::vector<int> slot_of;   // [np] physical parameter (= Cartesian slot) of each free parameter
+  std::vector<int> kbeg, kend;  // [np] support of each node's hat function
+  int* d_eqvnode = nullptr;   // [4][np]: kbeg, kend, Cartesian slot, five-slot index
+  bool lm_big = false;        // more than LM_MAXNP free parameters: work arrays in device memory
+  size_t lm_work = 0;         // bytes of one pixel's work arrays (lm_big)
+  double* d_lmwork = nullptr;
+  size_t lmwork_cap = 0;
   double* d_coefs = nullptr;  // [np][nz]
   double* d_colsum = nullptr; // [np]
   float* d_jtile = nullptr;   // per-height response functions of one tile
@@ -1237,6 +1244,8 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_scratch) (void)hipFree(c->d_scratch);
   if (c->d_coefs) (void)hipFree(c->d_coefs);
   if (c->d_colsum) (void)hipFree(c->d_colsum);
+  if (c->d_eqvnode) (void)hipFree(c->d_eqvnode);
+  if (c->d_lmwork) (void)hipFree(c->d_lmwork);
   if (c->d_jtile) (void)hipFree(c->d_jtile);
   if (c->d_sbuf) (void)hipFree(c->d_sbuf);
   if (c->d_eosb) (void)hipFree(c->d_eosb);
@@ -1526,6 +1535,14 @@ static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   return FZ_OK;
 }
 
+// floats of radiative-transfer scratch per lane (column-wavelength): the
+// incoming Stokes vector of every layer (and W_k with HYDROSTATIC)
+static size_t rt_scratch_per_lane(const fz_ctx* c, int rf_mode) {
+  if (rf_mode == FZ_RF_NONE) return 0;
+  const size_t nz = (size_t)c->cfg.nz;
+  return nz * (c->hydro ? 8 : 4);
+}
+
 static int rf_nparam(int rf_mode) {
   return rf_mode == FZ_RF_SPHERICAL ? 5
          : rf_mode == FZ_RF_CARTESIAN ? 7
@@ -1667,9 +1684,13 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
   if (r) return r;
   r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
   if (r) return r;
-  const int tile = std::min(c->tile, ncol);
+  // columns per radiative-transfer launch: the configured tile, fewer where
+  // its scratch would pass 48 GiB
+  const size_t per_col = sizeof(float) * rt_scratch_per_lane(c, rf_mode) * g.numw;
+  int tile = std::min(c->tile, ncol);
+  if (per_col) tile = (int)std::max<size_t>(1, std::min<size_t>((size_t)tile, ((size_t)48 << 30) / per_col));
   if (rf_mode != FZ_RF_NONE) {
-    r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * tile * g.numw);
+    r = ensure(c, &c->d_scratch, &c->scratch_cap, per_col * tile);
     if (r) return r;
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[0], st));
@@ -1748,7 +1769,7 @@ int fz_autotune(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes,
   // microphysics tile: Saha factors, EOS and derivative buffers per column
   const size_t per_mcol = sizeof(double) * (size_t)nz * (FZ_NELEM * 4 * (c->hydro ? 2 : 1) + 2 + 8);
   // radiative-transfer tile: the Stokes scratch per column
-  const size_t per_rcol = sizeof(float) * (size_t)nz * (c->hydro ? 8 : 4) * numw;
+  const size_t per_rcol = sizeof(float) * rt_scratch_per_lane(c, rf_mode) * numw;
   const int mcand[] = {4096, 8192, 16384, 32768, 65536, 131072, 262144};
   const int rcand[] = {2048, 4096, 8192, 16384, 32768};
   float best = 1e30f;
@@ -1849,8 +1870,9 @@ int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rh
       if (stat[i]) rc = set_err(c, stat[i], "column " + std::to_string(cols[i]) + " failed");
   }
   if (rc == FZ_OK) {
-    // model updated in place: rho, Pe, mu, log tau5 (SPLIT_MODEL3D, forward.f90:1225-1270)
-    float* outs[NATM] = {nullptr, nullptr, rho3d, pel3d, mw3d, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tau3d5};
+    // model updated in place: Pg (re-integrated with HYDROSTATIC), rho, Pe,
+    // mu, log tau5 (SPLIT_MODEL3D, forward.f90:1225-1270)
+    float* outs[NATM] = {nullptr, pg3d, rho3d, pel3d, mw3d, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, tau3d5};
     for (int p = 0; p < NATM; ++p) {
       if (!outs[p]) continue;
       hipLaunchKernelGGL(soa_to_cube, dim3(tg), dim3(tb), 0, 0, d_atm + (size_t)p * nz * n, d_tmp, nz, n);
@@ -1952,12 +1974,13 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   L.nfreq = nstk * numw;
   static const int mcol[8] = {A_T, A_PG, A_RHO, A_BX, A_BY, A_BZ, A_VZ, A_PG};
   std::vector<double> coefs;
+  c->slot_of.clear();
   int np = 0, npar = 0;
   for (int p = 0; p < 8; ++p) {
     if (!inv->atmpar[p]) continue;
     const int nj = inv->nodes[p];
     if (nj < 1 || nj > nz) return set_err(c, FZ_ECONFIG, "nodes must be in 1..nz");
-    if (np + nj > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters");
+    if (np + nj > LM_MAXNP_BIG) return set_err(c, FZ_ECONFIG, "more than 2048 free parameters");
     L.par[npar] = p;
     L.slot[npar] = p;  // Cartesian slots: T|Pg, Pg|T, rho|T, Bx, By, Bz, v_z, P0
     L.mcol[npar] = mcol[p];
@@ -1966,7 +1989,7 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
     std::vector<double> cf((size_t)nz * nj);
     fz_int_coefs(nz, nj, cf.data());
     coefs.insert(coefs.end(), cf.begin(), cf.end());
-    for (int j = 0; j < nj; ++j) c->slot_of[np + j] = p;
+    for (int j = 0; j < nj; ++j) c->slot_of.push_back(p);
     np += nj;
     ++npar;
   }
@@ -1974,6 +1997,8 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   L.np = np;
   L.nparam = npar;
   std::vector<double> colsum(np);
+  c->kbeg.assign(np, 0);
+  c->kend.assign(np, 0);
   for (int j = 0; j < np; ++j) {
     double w = 0.0;
     int kb = nz, ke = -1;
@@ -1986,8 +2011,8 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
       }
     }
     colsum[j] = w;
-    L.kbeg[j] = kb;
-    L.kend[j] = ke;
+    c->kbeg[j] = kb;
+    c->kend[j] = ke;
   }
   L.regul = inv->regul ? 1 : 0;
   L.solver = inv->solver;
@@ -2002,18 +2027,37 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   L.inu = inv->inu == 0.0 ? (double)L.nfreq : inv->inu;
   L.mpert = inv->mpert == 0.0 ? 0.15 : inv->mpert;
   L.maxsteps = (double)inv->maxsteps;
-  c->lm_lds = lm_lds_bytes(np, nz, L.regul != 0);
+  // up to LM_MAXNP free parameters the step's work arrays live in LDS;
+  // beyond, in a per-pixel slice of device memory
+  c->lm_big = np > LM_MAXNP;
+  c->lm_lds = c->lm_big ? 0 : lm_lds_bytes(np, nz, L.regul != 0, LM_THREADS);
+  c->lm_work = c->lm_big ? (lm_lds_bytes(np, nz, L.regul != 0, LM_BIG_THREADS) + 255) / 256 * 256 : 0;
   if (c->lm_lds > 160 * 1024 - 512) return set_err(c, FZ_ECONFIG, "inversion set-up exceeds the LDS of one workgroup");
   HIPCHK(c, hipSetDevice(c->device));
   if (c->d_coefs) (void)hipFree(c->d_coefs);
   if (c->d_colsum) (void)hipFree(c->d_colsum);
+  if (c->d_eqvnode) (void)hipFree(c->d_eqvnode);
   c->d_coefs = nullptr;
   c->d_colsum = nullptr;
+  c->d_eqvnode = nullptr;
+  // node table of the contraction: supports, Cartesian slot, five-slot index
+  static const int tbv_slot[8] = {0, -1, -1, 1, 2, 3, 4, -1};
+  std::vector<int> node((size_t)4 * np);
+  for (int j = 0; j < np; ++j) {
+    node[j] = c->kbeg[j];
+    node[np + j] = c->kend[j];
+    node[2 * np + j] = c->slot_of[j];
+    node[3 * np + j] = tbv_slot[c->slot_of[j]];
+  }
   HIPCHK(c, hipMalloc((void**)&c->d_coefs, sizeof(double) * coefs.size()));
   HIPCHK(c, hipMalloc((void**)&c->d_colsum, sizeof(double) * np));
+  HIPCHK(c, hipMalloc((void**)&c->d_eqvnode, sizeof(int) * node.size()));
   HIPCHK(c, hipMemcpy(c->d_coefs, coefs.data(), sizeof(double) * coefs.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_colsum, colsum.data(), sizeof(double) * np, hipMemcpyHostToDevice));
-  HIPCHK(c, hipFuncSetAttribute((const void*)lm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lm_lds));
+  HIPCHK(c, hipMemcpy(c->d_eqvnode, node.data(), sizeof(int) * node.size(), hipMemcpyHostToDevice));
+  if (!c->lm_big)
+    HIPCHK(c, hipFuncSetAttribute((const void*)lm_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)c->lm_lds));
   L.coefs = c->d_coefs;
   L.colsum = c->d_colsum;
   c->inv_set = true;
@@ -2045,11 +2089,10 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   const bool tbv = !p0 && !c->lm_pr && !c->hydro;
   const int rf = p0 ? FZ_RF_CARTESIAN_P0 : (tbv ? RF_CART_TBV : FZ_RF_CARTESIAN);
   const int nslot = rf_nparam(rf);
-  static const int tbv_slot[8] = {0, -1, -1, 1, 2, 3, 4, -1};
   // columns per tile: the configured tile, fewer where one tile's response
   // functions and scratch would pass 24 GiB; sized by the widest tile (the
   // wavelengths of its columns: ragged with per-pixel windows)
-  const size_t per_w = sizeof(float) * ((size_t)nslot * g.nz * 4 + (size_t)g.nz * (c->hydro ? 8 : 4));
+  const size_t per_w = sizeof(float) * ((size_t)nslot * g.nz * 4 + rt_scratch_per_lane(c, rf));
   const size_t budget = (size_t)24 << 30;
   int tile = std::min(c->tile, ncol);
   size_t wmax = 0;
@@ -2062,7 +2105,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     if (wmax * per_w <= budget || tile == 1) break;
     tile = std::max(1, (int)((double)tile * (double)budget / (double)(wmax * per_w)));
   }
-  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * (size_t)g.nz * (c->hydro ? 8 : 4) * wmax);
+  r = ensure(c, &c->d_scratch, &c->scratch_cap, sizeof(float) * rt_scratch_per_lane(c, rf) * wmax);
   if (r) return r;
   r = ensure(c, &c->d_jtile, &c->jtile_cap, sizeof(float) * (size_t)nslot * g.nz * 4 * wmax);
   if (r) return r;
@@ -2079,11 +2122,10 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   e.nstk = c->lm.nfreq / g.numw;
   e.nslot = nslot;
   for (int s = 0; s < 4; ++s) e.rowstk[s] = c->lm.rowstk[s];
-  for (int j = 0; j < e.np; ++j) {
-    e.slot_of[j] = tbv ? tbv_slot[c->slot_of[j]] : c->slot_of[j];
-    e.kbeg[j] = c->lm.kbeg[j];
-    e.kend[j] = c->lm.kend[j];
-  }
+  const int np = e.np;
+  e.kbeg = c->d_eqvnode;
+  e.kend = c->d_eqvnode + np;
+  e.slot_of = c->d_eqvnode + (tbv ? 3 : 2) * np;
   e.nparam = c->lm.nparam;
   for (int ip = 0; ip < e.nparam; ++ip) {
     e.j0[ip] = c->lm.j0[ip];
@@ -2091,12 +2133,14 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
   }
   // one pass per parameter when its node supports overlap at most pairwise
   // and move up with the node index (the hat basis of GET_INT_COEFS does)
+  const std::vector<int>& kb = c->kbeg;
+  const std::vector<int>& ke = c->kend;
   e.pairs = 1;
   for (int ip = 0; ip < e.nparam; ++ip)
     for (int j = e.j0[ip]; j < e.j0[ip] + e.nj[ip]; ++j) {
-      if (e.kbeg[j] > e.kend[j]) e.pairs = 0;
-      if (j + 1 < e.j0[ip] + e.nj[ip] && (e.kbeg[j + 1] < e.kbeg[j] || e.kend[j + 1] < e.kend[j])) e.pairs = 0;
-      if (j + 2 < e.j0[ip] + e.nj[ip] && e.kbeg[j + 2] <= e.kend[j]) e.pairs = 0;
+      if (kb[j] > ke[j]) e.pairs = 0;
+      if (j + 1 < e.j0[ip] + e.nj[ip] && (kb[j + 1] < kb[j] || ke[j + 1] < ke[j])) e.pairs = 0;
+      if (j + 2 < e.j0[ip] + e.nj[ip] && kb[j + 2] <= ke[j]) e.pairs = 0;
     }
   e.coefs = c->d_coefs;
   e.eqv = d_eqv;
@@ -2152,6 +2196,31 @@ static void set_lm_windows(const fz_ctx* c, LmArgs& a) {
   a.inu_pix = c->win_on ? 1 : 0;
 }
 
+// lm_kernel over nrows rows: one wave per row with the work arrays in LDS,
+// or, beyond LM_MAXNP free parameters, four waves per row with the work
+// arrays in device memory, in batches of rows whose slices fit 2 GiB
+static int launch_lm(fz_ctx* c, LmArgs a, int nrows, hipStream_t st) {
+  if (!c->lm_big) {
+    a.work = nullptr;
+    a.work_stride = 0;
+    a.row0 = 0;
+    hipLaunchKernelGGL(lm_kernel<false>, dim3(nrows), dim3(LM_THREADS), c->lm_lds, st, a);
+    HIPCHK(c, hipGetLastError());
+    return FZ_OK;
+  }
+  const int batch = (int)std::max<size_t>(1, std::min<size_t>((size_t)nrows, ((size_t)2 << 30) / c->lm_work));
+  int r = ensure(c, (float**)&c->d_lmwork, &c->lmwork_cap, c->lm_work * batch);
+  if (r) return r;
+  a.work = c->d_lmwork;
+  a.work_stride = (long)(c->lm_work / sizeof(double));
+  for (int r0 = 0; r0 < nrows; r0 += batch) {
+    a.row0 = r0;
+    hipLaunchKernelGGL(lm_kernel<true>, dim3(std::min(batch, nrows - r0)), dim3(LM_BIG_THREADS), 0, st, a);
+    HIPCHK(c, hipGetLastError());
+  }
+  return FZ_OK;
+}
+
 extern "C" {
 
 int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stokes, double* d_best_syn, double* d_sla,
@@ -2178,9 +2247,7 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
   a.delta = d_delta;
   a.pix_of = nullptr;
   set_lm_windows(c, a);
-  hipLaunchKernelGGL(lm_kernel, dim3(ncol), dim3(LM_THREADS), c->lm_lds, (hipStream_t)stream, a);
-  HIPCHK(c, hipGetLastError());
-  return FZ_OK;
+  return launch_lm(c, a, ncol, (hipStream_t)stream);
 }
 
 // fz_lm_step_dev over the na active pixels pix_of[0..na): stokes and eqv are
@@ -2205,9 +2272,7 @@ static int lm_step_active(fz_ctx* c, int ncol, int na, const int32_t* pix_of, co
   a.pix_of = pix_of;
   set_lm_windows(c, a);
   if (c->win_on && (pix_of || ncol != c->win_ncol)) return set_err(c, FZ_EARG, "per-pixel windows: every pixel keeps its row");
-  hipLaunchKernelGGL(lm_kernel, dim3(na), dim3(LM_THREADS), c->lm_lds, st, a);
-  HIPCHK(c, hipGetLastError());
-  return FZ_OK;
+  return launch_lm(c, a, na, st);
 }
 
 // GET_NJEVALS (get_dmodel.f90:375-407) with TOFFSET = 1 (pre_post_duties.f90:645)
@@ -2283,7 +2348,7 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   int npmax = 0;
   for (int p = 0; p < 8; ++p)
     if (ctl->atmpar[p]) npmax += ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
-  if (npmax > LM_MAXNP) return set_err(c, FZ_ECONFIG, "more than 64 free parameters in the last cycle");
+  if (npmax > LM_MAXNP_BIG) return set_err(c, FZ_ECONFIG, "more than 2048 free parameters in the last cycle");
   DevBuf b_isig, b_best, b_eqv, b_eqvb, b_bsyn, b_sla, b_done, b_stat, b_t0, b_t1, b_atmc, b_idx, b_cnt;
   const size_t natm = (size_t)12 * nz * ncol;
   HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));

@@ -17,11 +17,16 @@
 
 namespace fz {
 
-constexpr int LM_MAXNP = 64;     // free parameters per pixel (nodes summed over parameters)
+constexpr int LM_MAXNP = 64;     // free parameters per pixel with the work arrays in LDS
 // one wave per pixel: the Jacobi rounds and reductions are wave-local, so
 // the barriers are cheap and a CU holds many pixels at once
 constexpr int LM_THREADS = 64;
-constexpr int LM_CH = 64;        // spectral rows per LDS chunk
+constexpr int LM_CH = 64;        // spectral rows per chunk of the normal matrix
+// more free parameters (up to NZ nodes per parameter, the reference's
+// default): the same step with its work arrays in a per-pixel slice of
+// device memory, four waves per pixel
+constexpr int LM_MAXNP_BIG = 2048;
+constexpr int LM_BIG_THREADS = 256;
 
 // get_dmodel.f90:31 (PARAM_NORMS) and :1903-1919 (hard bounds)
 __constant__ double c_lm_norm[8] = {6000.0, 1.0e5, 1.0e-5, 1.0e3, 1.0e3, 1.0e3, 1.0e6, 6.0e2};
@@ -35,7 +40,6 @@ struct LmArgs {
   int slot[8];                 // response-function slot in the Cartesian per-height output
   int mcol[8];                 // model column
   int j0[8], nj[8];            // first free parameter, nodes
-  int kbeg[LM_MAXNP], kend[LM_MAXNP];  // support of each node's hat function
   int rowstk[4];               // Stokes index of each inverted Stokes block
   int regul, solver, node_expansion, pen_typ[8];
   double pen_alp[8];
@@ -64,6 +68,11 @@ struct LmArgs {
   // inu_pix: chi^2 normalised by the pixel's own NFREQ
   const int64_t* poff;
   int nstk, inu_pix;
+  // work arrays in device memory (more than LM_MAXNP free parameters):
+  // workgroup b handles row row0 + b and uses work[b * work_stride ...]
+  double* work;
+  long work_stride;
+  int row0;
 };
 
 // ---------------------------------------------------------------- node contraction
@@ -71,8 +80,9 @@ struct EqvArgs {
   int ncol_tile, col0, nz, numw, nfreq, np, nstk;
   int nslot;                   // response-function slots per column in the tile (7, or 8 with P0)
   int rowstk[4];
-  int slot_of[LM_MAXNP];       // response-function slot of each free parameter
-  int kbeg[LM_MAXNP], kend[LM_MAXNP];
+  const int* slot_of;          // [np] response-function slot of each free parameter
+  const int* kbeg;             // [np] support of each node's hat function
+  const int* kend;             // [np]
   // the nodes of inverted parameter ip are j0[ip] .. j0[ip]+nj[ip]-1; pairs:
   // every height lies in the support of at most two consecutive nodes of a
   // parameter and the supports move up with the node index (hat functions),
@@ -163,11 +173,12 @@ __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
 }
 
 // ---------------------------------------------------------------- LM step
+template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
   const int t = threadIdx.x;
   red[t] = v;
   __syncthreads();
-  for (int s = LM_THREADS / 2; s > 0; s >>= 1) {
+  for (int s = NT / 2; s > 0; s >>= 1) {
     if (t < s) red[t] += red[t + s];
     __syncthreads();
   }
@@ -228,10 +239,13 @@ __device__ void lm_penalties(const LmArgs& a, const float* mod, double* pen_res,
   }
 }
 
-__global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
-  extern __shared__ double smem[];
+template <bool BIG>
+__global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(LmArgs a) {
+  constexpr int NT = BIG ? LM_BIG_THREADS : LM_THREADS;
+  extern __shared__ double smem_dyn[];
   const int t = threadIdx.x;
-  const int row = blockIdx.x;
+  const int row = a.row0 + (int)blockIdx.x;
+  double* smem = BIG ? a.work + (long)blockIdx.x * a.work_stride : smem_dyn;
   const int pix = a.pix_of ? a.pix_of[row] : row;
   if (a.done && a.done[pix] != 0) return;
   const int np = a.np, nz = a.nz;
@@ -262,8 +276,8 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   double* g = rch + LM_CH;                     // [np]
   double* dv = g + np;                         // [np] eigenvalues / projections
   double* del = dv + np;                       // [np]
-  double* red = del + np;                      // [LM_THREADS]
-  double* rot = red + LM_THREADS;              // [npe/2][2]
+  double* red = del + np;                      // [NT]
+  double* rot = red + NT;              // [npe/2][2]
   double* nzp = rot + npe;                     // [nz]
   double* nod = nzp + nz;                      // [4][np] node work vectors
   double* pen_res = nod + 4 * np;              // [np]
@@ -278,11 +292,11 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
 
   const long ncol = a.ncol;
   // ---- load the current model and the pixel's scalars
-  for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm[(long)i * ncol + pix];
+  for (int i = t; i < 12 * nz; i += NT) mod[i] = a.atm[(long)i * ncol + pix];
   if (t < 9) s_sla[t] = a.sla[(long)pix * 9 + t];
   // columns whose absolute sum overflows REAL(4) are zeroed (forward.f90:672-678)
   const float* ecur = a.eqv + o_ecur;
-  for (int j = t; j < np; j += LM_THREADS) {
+  for (int j = t; j < np; j += NT) {
     float s = 0.f;
     for (int f = 0; f < nf; ++f) s += fabsf(ecur[(long)j * nf + f]);
     izero[j] = (s > 3.40282347e38f) ? 1 : 0;
@@ -293,12 +307,12 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   const double* obs = a.obs + o_obs;
   const float* stk = a.stokes + o_stk;
   double part = 0.0;
-  for (int f = t; f < nf; f += LM_THREADS) {
+  for (int f = t; f < nf; f += NT) {
     const int b = f / numw, l = f - b * numw;
     const double d = obs[f] - (double)stk[a.rowstk[b] * numw + l];
     part += d * d * isig(f);
   }
-  double chi = block_sum(part, red) / inu;
+  double chi = block_sum<NT>(part, red) / inu;
   if (a.regul) {
     if (t == 0) lm_penalties(a, mod, pen_res, pen_hss, nod, nod + np);
     __syncthreads();
@@ -313,15 +327,15 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   float* ebest = a.eqv_best + o_ebest;
   if (accept) {
     // best := current (model, equivalent response functions, synthesis)
-    for (int i = t; i < 12 * nz; i += LM_THREADS) a.atm_best[(long)i * ncol + pix] = mod[i];
-    for (long i = t; i < (long)np * nf; i += LM_THREADS) ebest[i] = izero[i / nf] ? 0.f : ecur[i];
-    for (int f = t; f < nf; f += LM_THREADS) {
+    for (int i = t; i < 12 * nz; i += NT) a.atm_best[(long)i * ncol + pix] = mod[i];
+    for (long i = t; i < (long)np * nf; i += NT) ebest[i] = izero[i / nf] ? 0.f : ecur[i];
+    for (int f = t; f < nf; f += NT) {
       const int b = f / numw, l = f - b * numw;
       bsyn[f] = (double)stk[a.rowstk[b] * numw + l];
     }
   } else {
     // current := best
-    for (int i = t; i < 12 * nz; i += LM_THREADS) mod[i] = a.atm_best[(long)i * ncol + pix];
+    for (int i = t; i < 12 * nz; i += NT) mod[i] = a.atm_best[(long)i * ncol + pix];
   }
   __syncthreads();
   if (!accept && a.regul) {
@@ -360,7 +374,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   const int ntri = np * (np + 1) / 2, nq = ntri + np;
   // sums of the normal matrix (upper triangle, row-major) and of the
   // gradient, accumulated in LDS over the chunks in row order
-  for (int q = t; q < nq; q += LM_THREADS) {
+  for (int q = t; q < nq; q += NT) {
     int i, j;
     if (q < ntri) {
       int r = 0, base = 0;
@@ -376,12 +390,12 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
     }
     qtab[q] = (i << 16) | j;
   }
-  for (int e = t; e < npe * ld; e += LM_THREADS) A[e] = 0.0;
-  for (int i = t; i < np; i += LM_THREADS) g[i] = 0.0;
+  for (int e = t; e < npe * ld; e += NT) A[e] = 0.0;
+  for (int i = t; i < np; i += NT) g[i] = 0.0;
   __syncthreads();
   for (int f0 = 0; f0 < nf; f0 += LM_CH) {
     const int nc = min(LM_CH, nf - f0);
-    for (int e = t; e < nc * np; e += LM_THREADS) {
+    for (int e = t; e < nc * np; e += NT) {
       const int c = e / np, j = e - c * np;
       const int f = f0 + c;
       double v = (accept && izero[j]) ? 0.0 : (double)jsrc[(long)j * nf + f];
@@ -389,12 +403,12 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       while (j >= a.j0[ip] + a.nj[ip]) ++ip;
       Jn[c * np + j] = v * sqrt(isig(f)) / snu * (1.0 * c_lm_norm[a.par[ip]]);
     }
-    for (int c = t; c < nc; c += LM_THREADS) {
+    for (int c = t; c < nc; c += NT) {
       const int f = f0 + c;
       rch[c] = (obs[f] - bsyn[f]) * sqrt(isig(f)) / snu;
     }
     __syncthreads();
-    for (int q = t; q < nq; q += LM_THREADS) {
+    for (int q = t; q < nq; q += NT) {
       const int i = qtab[q] >> 16, j = qtab[q] & 0xffff;
       if (j != 0xffff) {
         double s = A[i * ld + j];
@@ -410,21 +424,21 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
   }
   const double lam = s_sla[8];
   // A = H + REG_HSS, diag += diag * lambda; g = J^T r - REG_PEN
-  for (int e = t; e < np * np; e += LM_THREADS) {
+  for (int e = t; e < np * np; e += NT) {
     const int i = e / np, j = e - i * np;
     if (j < i) A[i * ld + j] = A[j * ld + i];
   }
-  for (int i = t; i < np; i += LM_THREADS)
+  for (int i = t; i < np; i += NT)
     if (a.regul) g[i] -= pen_res[i];
   __syncthreads();
   if (a.regul)
-    for (int e = t; e < np * np; e += LM_THREADS) {
+    for (int e = t; e < np * np; e += NT) {
       const int i = e / np, j = e - i * np;
       A[i * ld + j] += pen_hss[e];
     }
   __syncthreads();
-  for (int i = t; i < np; i += LM_THREADS) A[i * ld + i] = A[i * ld + i] + A[i * ld + i] * lam;
-  for (int e = t; e < npe * ld; e += LM_THREADS) {
+  for (int i = t; i < np; i += NT) A[i * ld + i] = A[i * ld + i] + A[i * ld + i] * lam;
+  for (int e = t; e < npe * ld; e += NT) {
     const int i = e / ld, j = e - i * ld;
     V[e] = (i == j) ? 1.0 : 0.0;
   }
@@ -435,18 +449,17 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
     const int half = npe / 2;
     for (int sweep = 0; sweep < 60; ++sweep) {
       double o = 0.0, tt = 0.0;
-      for (int e = t; e < np * np; e += LM_THREADS) {
+      for (int e = t; e < np * np; e += NT) {
         const int i = e / np, j = e - i * np;
         const double x = A[i * ld + j];
         tt += x * x;
         if (i != j) o += x * x;
       }
-      const double off = block_sum(o, red), tot = block_sum(tt, red);
+      const double off = block_sum<NT>(o, red), tot = block_sum<NT>(tt, red);
       if (off <= 1e-30 * tot || off == 0.0) break;
       for (int rnd = 0; rnd < npe - 1; ++rnd) {
         // pair i: players L[i], L[npe-1-i] with L = [0, (rnd+k)%(npe-1)+1 ...]
-        if (t < half) {
-          const int i = t;
+        for (int i = t; i < half; i += NT) {
           const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
           const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
           const int p = min(p0, q0), q = max(p0, q0);
@@ -465,7 +478,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
         }
         __syncthreads();
         // columns of A and V: A <- A J, V <- V J
-        for (int e = t; e < half * npe; e += LM_THREADS) {
+        for (int e = t; e < half * npe; e += NT) {
           const int i = e / npe, k = e - i * npe;
           const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
           const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
@@ -481,7 +494,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
         }
         __syncthreads();
         // rows of A: A <- J^T A
-        for (int e = t; e < half * npe; e += LM_THREADS) {
+        for (int e = t; e < half * npe; e += NT) {
           const int i = e / npe, k = e - i * npe;
           const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
           const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
@@ -499,7 +512,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
     double mx = 0.0;
     for (int i = 0; i < np; ++i) mx = fmax(mx, fabs(A[i * ld + i]));
     const double thr = mx * (a.svdtol > 0 ? a.svdtol : 10.0e-4);
-    for (int e = t; e < np; e += LM_THREADS) {
+    for (int e = t; e < np; e += NT) {
       const double d = A[e * ld + e];
       double pr = 0.0;
       if (!(fabs(d) < thr)) {
@@ -509,74 +522,79 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       dv[e] = pr;
     }
     __syncthreads();
-    for (int i = t; i < np; i += LM_THREADS) {
+    for (int i = t; i < np; i += NT) {
       double s = 0.0;
       for (int e = 0; e < np; ++e) s += V[i * ld + e] * dv[e];
       del[i] = s;
     }
   } else {
-    // NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453), by one thread
-    if (t == 0) {
-      double* xv = nod;
-      double* pv = nod + np;
-      double* rv = nod + 2 * np;
-      double* qv = nod + 3 * np;
-      double* ck = dv;
-      for (int i = 0; i < np; ++i) {
-        xv[i] = 0.0;
-        pv[i] = g[i];
-        rv[i] = g[i];   // bet - alp x with x = 0
-      }
-      double crho = 0.0;
-      for (int i = 0; i < np; ++i) crho += rv[i] * rv[i];
-      for (int i = 0; i < np; ++i) {
+    // NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453): the matrix-vector and
+    // dot products spread over the workgroup; every thread holds the same
+    // scalars, so the iteration and its stopping test are uniform
+    double* xv = nod;
+    double* pv = nod + np;
+    double* rv = nod + 2 * np;
+    double* qv = nod + 3 * np;
+    auto dot = [&](const double* x, const double* y) {
+      double s = 0.0;
+      for (int i = t; i < np; i += NT) s += x[i] * y[i];
+      return block_sum<NT>(s, red);
+    };
+    auto matvec = [&](const double* x, double* y) {
+      for (int i = t; i < np; i += NT) {
         double s = 0.0;
-        for (int j = 0; j < np; ++j) s += A[i * ld + j] * pv[j];
-        qv[i] = s;
+        for (int j = 0; j < np; ++j) s += A[i * ld + j] * x[j];
+        y[i] = s;
       }
-      double calpha = 0.0;
-      for (int i = 0; i < np; ++i) calpha += pv[i] * qv[i];
-      calpha = crho / calpha;
-      for (int i = 0; i < np; ++i) xv[i] = xv[i] + calpha * pv[i];
-      for (int i = 0; i < np; ++i) rv[i] = rv[i] - calpha * qv[i];
-      double pcond = 1.0e29, dcond = -1.0e29;
-      bool conv = false;
-      for (int it = 1; it <= 100000; ++it) {
-        const double crho0 = crho;
-        crho = 0.0;
-        for (int i = 0; i < np; ++i) crho += rv[i] * rv[i];
-        const double cbeta = crho / crho0;
-        for (int i = 0; i < np; ++i) pv[i] = rv[i] + cbeta * pv[i];
-        for (int i = 0; i < np; ++i) {
-          double s = 0.0;
-          for (int j = 0; j < np; ++j) s += A[i * ld + j] * pv[j];
-          qv[i] = s;
-        }
-        calpha = 0.0;
-        for (int i = 0; i < np; ++i) calpha += pv[i] * qv[i];
-        calpha = crho / calpha;
-        for (int i = 0; i < np; ++i) xv[i] = xv[i] + calpha * pv[i];
-        for (int i = 0; i < np; ++i) rv[i] = rv[i] - calpha * qv[i];
-        double cond = 0.0;
-        for (int i = 0; i < np; ++i) {
-          double s = 0.0;
-          for (int j = 0; j < np; ++j) s += A[i * ld + j] * xv[j];
-          ck[i] = s;
-          const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
-          cond += e * e;
-        }
-        cond = cond / (double)np;
-        if (cond < 1.0e-3 || dcond > 1.0e30) {
-          for (int i = 0; i < np; ++i) del[i] = xv[i];
-          conv = true;
-          break;
-        }
-        dcond = cond - pcond;
-        pcond = cond;
-      }
-      if (!conv)
-        for (int i = 0; i < np; ++i) del[i] = xv[i] / 10.0;
+      __syncthreads();
+    };
+    for (int i = t; i < np; i += NT) {
+      xv[i] = 0.0;
+      pv[i] = g[i];
+      rv[i] = g[i];   // bet - alp x with x = 0
     }
+    __syncthreads();
+    double crho = dot(rv, rv);
+    matvec(pv, qv);
+    double calpha = crho / dot(pv, qv);
+    for (int i = t; i < np; i += NT) {
+      xv[i] = xv[i] + calpha * pv[i];
+      rv[i] = rv[i] - calpha * qv[i];
+    }
+    __syncthreads();
+    double pcond = 1.0e29, dcond = -1.0e29;
+    bool conv = false;
+    for (int it = 1; it <= 100000; ++it) {
+      const double crho0 = crho;
+      crho = dot(rv, rv);
+      const double cbeta = crho / crho0;
+      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
+      __syncthreads();
+      matvec(pv, qv);
+      calpha = crho / dot(pv, qv);
+      for (int i = t; i < np; i += NT) {
+        xv[i] = xv[i] + calpha * pv[i];
+        rv[i] = rv[i] - calpha * qv[i];
+      }
+      __syncthreads();
+      double cs = 0.0;
+      for (int i = t; i < np; i += NT) {
+        double s = 0.0;
+        for (int j = 0; j < np; ++j) s += A[i * ld + j] * xv[j];
+        const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
+        cs += e * e;
+      }
+      const double cond = block_sum<NT>(cs, red) / (double)np;
+      if (cond < 1.0e-3 || dcond > 1.0e30) {
+        for (int i = t; i < np; i += NT) del[i] = xv[i];
+        conv = true;
+        break;
+      }
+      dcond = cond - pcond;
+      pcond = cond;
+    }
+    if (!conv)
+      for (int i = t; i < np; i += NT) del[i] = xv[i] / 10.0;
   }
   __syncthreads();
   // NaN filter (get_dmodel.f90:806-822)
@@ -598,8 +616,8 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
     float* col = mod + a.mcol[ip] * nz;
     const double factor = c_lm_norm[p], inf = c_lm_low[p], sup = c_lm_upp[p];
     const double absolute = c_lm_norm[p] * a.mpert;
-    for (int k = t; k < nz; k += LM_THREADS) bum[k] = col[k];
-    for (int j = t; j < nj; j += LM_THREADS) {
+    for (int k = t; k < nz; k += NT) bum[k] = col[k];
+    for (int j = t; j < nj; j += NT) {
       const double* cf = a.coefs + (long)(j0 + j) * nz;
       double s = 0.0;
       for (int k = 0; k < nz; ++k) s = s + (double)col[k] * cf[k];
@@ -616,26 +634,26 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       del[j0 + j] = x;
     }
     __syncthreads();
-    for (int k = t; k < nz; k += LM_THREADS) {
+    for (int k = t; k < nz; k += NT) {
       double s = 0.0;
       for (int j = 0; j < nj; ++j) s = s + sumv[j] * a.coefs[(long)(j0 + j) * nz + k];
       nzp[k] = s;
     }
     __syncthreads();
     if (a.node_expansion == 2) {
-      for (int k = t; k < nz; k += LM_THREADS) {
+      for (int k = t; k < nz; k += NT) {
         float o = (float)nzp[k];
         if ((double)o < inf) o = (float)inf;
         if ((double)o > sup) o = (float)sup;
         col[k] = o - col[k];
       }
     } else if (a.node_expansion == 1) {
-      for (int k = t; k < nz; k += LM_THREADS) {
+      for (int k = t; k < nz; k += NT) {
         nzp[k] = nzp[k] - (double)col[k];
         ftmp[k] = (float)nzp[k];
       }
       __syncthreads();
-      for (int j = t; j < nj; j += LM_THREADS) {
+      for (int j = t; j < nj; j += NT) {
         const double* cf = a.coefs + (long)(j0 + j) * nz;
         double s = 0.0;
         for (int k = 0; k < nz; ++k) s = s + (double)ftmp[k] * cf[k];
@@ -643,7 +661,7 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
         del[j0 + j] = itd[j];
       }
       __syncthreads();
-      for (int k = t; k < nz; k += LM_THREADS) {
+      for (int k = t; k < nz; k += NT) {
         double s = 0.0;
         for (int j = 0; j < nj; ++j) s = s + itd[j] * a.coefs[(long)(j0 + j) * nz + k];
         const float o = col[k] + (float)s;
@@ -667,22 +685,23 @@ __global__ void __launch_bounds__(LM_THREADS) lm_kernel(LmArgs a) {
       }
     }
     __syncthreads();
-    for (int k = t; k < nz; k += LM_THREADS) col[k] = col[k] + bum[k];
+    for (int k = t; k < nz; k += NT) col[k] = col[k] + bum[k];
     __syncthreads();
   }
 
   // ---- write back
-  for (int i = t; i < 12 * nz; i += LM_THREADS) a.atm[(long)i * ncol + pix] = mod[i];
+  for (int i = t; i < 12 * nz; i += NT) a.atm[(long)i * ncol + pix] = mod[i];
   if (t < 9) a.sla[(long)pix * 9 + t] = s_sla[t];
   if (a.delta)
-    for (int i = t; i < np; i += LM_THREADS) a.delta[(long)pix * np + i] = del[i];
+    for (int i = t; i < np; i += NT) a.delta[(long)pix * np + i] = del[i];
   if (t == 0 && a.done && s_sla[0] > a.maxsteps) a.done[pix] = 1;
 }
 
-// LDS bytes of lm_kernel for np free parameters and nz heights
-inline size_t lm_lds_bytes(int np, int nz, bool regul) {
+// bytes of lm_kernel's work arrays (LDS, or the device-memory slice of one
+// pixel) for np free parameters, nz heights and nt threads
+inline size_t lm_lds_bytes(int np, int nz, bool regul, int nt) {
   const int npe = np + (np & 1), ld = np + 1;
-  size_t d = (size_t)2 * npe * ld + (size_t)LM_CH * np + LM_CH + 3 * np + LM_THREADS + npe + nz + 4 * np + np +
+  size_t d = (size_t)2 * npe * ld + (size_t)LM_CH * np + LM_CH + 3 * np + nt + npe + nz + 4 * np + np +
              (regul ? (size_t)np * np : 0);
   size_t f = (size_t)12 * nz + 2 * nz + np + (size_t)np * (np + 1) / 2 + np;   // floats + ints
   return d * 8 + f * 4;

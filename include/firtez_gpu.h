@@ -108,10 +108,10 @@ int fz_set_windows(fz_ctx* ctx, int ncol, const int32_t* windows);
 int fz_window_offsets(const fz_ctx* ctx, int64_t* offsets);
 
 /* FORWARD3D on device-resident data (forward.f90:43-306, per column the
- * FORWARD1D of forward.f90:857-1029 with HYDROSTATIC=no).
+ * FORWARD1D of forward.f90:857-1029; HYDROSTATIC as set by fz_set_hydrostatic).
  *   d_atm    [12][nz][ncol] float, in/out: T, Pg, rho, Pe, mu, Bx, By, Bz, vx,
- *            vy, vz, log tau5 (rho, Pe, mu, log tau5 are updated like the
- *            reference updates the cube)
+ *            vy, vz, log tau5 (rho, Pe, mu, log tau5 -- and Pg with
+ *            HYDROSTATIC -- are updated like the reference updates the cube)
  *   d_stokes [ncol][4][numw] float: SYN1D*GAIN1D
  *   d_jac    [ncol][np][nz][4][numw] float (np = 5 spherical, 7 cartesian, 8 with P0), the
  *            propagated response functions times GAIN1D; NULL if rf_mode=NONE
@@ -121,7 +121,8 @@ int fz_forward_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_st
                    int32_t* d_status, void* stream);
 
 /* FORWARD3D on host arrays in the reference's layout (forward.f90:43-306):
- * cubes(NZ,NY,NX) column-major float (TEM3D..TAU3D5, updated in place),
+ * cubes(NZ,NY,NX) column-major float (TEM3D..TAU3D5; PG3D, RHO3D, PEL3D,
+ * MW3D and TAU3D5 are written back as SPLIT_MODEL3D does, forward.f90:1225-1270),
  * am_i_done(NY,NX) (columns with nonzero are skipped), syn3d(NFREQ,NY,NX)
  * double with NFREQ = 4*NUMW in Stokes-major blocks, dsyn(NFREQ,IFREEP,NY,NX)
  * float with IFREEP = np*NZ (parameter-major, height within) or NULL. */

@@ -1,0 +1,129 @@
+"""Control-file reader (firtez_dz_amd/control.py) on the reference's own
+control files (tests/golden/control/*.dat are the files of the reference's
+tests/test_*/control_file.dat, kept as data) and on hand-made cases of the
+reader rules of input_file.f90:60-98, 1310-1420."""
+import os
+
+import numpy as np
+import pytest
+
+from firtez_dz_amd import control, lines
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CTL = os.path.join(HERE, "golden", "control")
+
+
+def load(name):
+    return control.parse_file(os.path.join(CTL, name + ".dat"))
+
+
+def test_syn1():
+    c = load("test_syn1")
+    assert c.mode == "synthesis" and not c.hydrostatic
+    assert c.lines == [((4, 5), 500, -700.0, 5.0)]
+    assert c.fileprofile == "syn_profile.bin" and c.filemodel == "test_atmos.bin"
+    assert c.invstk == [True] * 4 and c.wstk == [1.0] * 4
+    assert (c.modpath, c.sldpath, c.outpath) == ("../../default/", "../../default/", "./out_dir/")
+    assert not c.write_rf and not c.params
+    ls = control.line_setup(c)
+    assert ls.numw == 500 and [l.id for l in ls.lines] == [4, 5]
+    # WAVE / INDEX of WAVE_INIT: start + step*j, the main line's lambda0
+    wave, index = control._wave_index(ls)
+    assert wave[0] == -700.0 and wave[-1] == -700.0 + 5.0 * 499
+    assert np.all(index == ls.lines[0].lambda0)
+
+
+def test_syn2_stops_at_end():
+    c = load("test_syn2")
+    # the row after END: is not read (input_file.f90:86-89)
+    assert c.lines == [((72,), 500, -700.0, 5.0)]
+    assert c.hydrostatic and c.mode == "synthesis"
+
+
+def test_rf():
+    c = load("test_rf")
+    assert c.write_rf and c.params == {"TEM": None, "BZ": None}
+    assert c.mode == "synthesis"
+
+
+def test_he_tau():
+    he, t1, t2 = load("test_he"), load("test_tau1"), load("test_tau2")
+    assert he.mode == "hydrostatic" and he.hydrostatic and not he.lines
+    assert t1.mode == "tau" and not t1.hydrostatic
+    assert t2.mode == "tau" and t2.hydrostatic
+    # modes without LINES get one placeholder wavelength
+    assert control.line_setup(t1).numw == 1
+
+
+def test_inv1():
+    c = load("test_inv1")
+    assert c.mode == "inversion" and c.hydrostatic and c.hydro_ders
+    assert c.params == {"TEM": None, "VLOS": None}
+    assert c.vregul and c.pen_typ == {"TEM": 1, "VLOS": 1}
+    assert c.pen_alp == {"TEM": pytest.approx(0.01), "VLOS": pytest.approx(0.1)}
+    assert c.solver == 2 and c.node_expansion == 2
+    assert c.datapath == "../test_syn1/out_dir/"
+    ctl = control.control_for_inversion(c, 256)
+    assert list(ctl.atmpar) == [1, 0, 0, 0, 0, 0, 1, 0]
+    assert list(ctl.nslb_max)[0] == 256 and list(ctl.nslb_max)[6] == 256
+    assert ctl.regul == 1 and ctl.solver == 2 and ctl.node_expansion == 2 and ctl.smoothing == 1
+    assert list(ctl.pen_typ)[0] == 1 and ctl.pen_alp[6] == pytest.approx(0.1)
+
+
+def test_reader_rules():
+    text = """
+# a comment line
+LINES:
+4,5 112 -624.0 20.0   # trailing comment
+  6 80 -400 10
+MODE:
+inversion
+FILEMODEL:
+m.bin
+FILEPROFILE:
+p.bin
+STOKES SETUP:
+STKI
+STKV 2.0
+INVERSION SETUP:
+TEM 4
+BZ 2
+VLOS
+POW 3
+MAXITER 7
+MISC SETUP:
+NOISE 1.0e-3
+SVDTOL 1D-5
+OUTPATH ./o/
+END:
+MODE:
+synthesis
+"""
+    c = control.parse(text)
+    assert c.lines == [((4, 5), 112, -624.0, 20.0), ((6,), 80, -400.0, 10.0)]
+    assert c.mode == "inversion"
+    assert c.invstk == [True, False, False, True] and c.wstk[3] == 2.0
+    assert c.params == {"TEM": 4, "BZ": 2, "VLOS": None}
+    assert c.cycpow == 3 and c.maxsteps == 7
+    assert c.noise == pytest.approx(1e-3) and c.svdtol == pytest.approx(1e-5)
+    assert c.outpath == "./o/"
+    ctl = control.control_for_inversion(c, 64)
+    assert list(ctl.nslb_max)[:8] == [4, 0, 0, 0, 0, 2, 64, 0]
+    assert list(ctl.wstk) == [1.0, 0.0, 0.0, 2.0]
+    # halves round away from zero like NINT
+    assert control._nint(2.5) == 3 and control._nint(-2.5) == -3
+
+
+def test_reader_errors():
+    with pytest.raises(ValueError):
+        control.parse("MODE:\nsynthesis\n")                       # no FILEMODEL
+    with pytest.raises(ValueError):
+        control.parse("FILEMODEL:\nm.bin\n")                      # no MODE
+    with pytest.raises(ValueError):
+        control.parse("FILEMODEL:\nm.bin\nMODE:\ninversion\nLINES:\n4 10 0 1\nFILEPROFILE:\np\n")  # no Stokes
+    with pytest.raises(ValueError):
+        control.parse("FILEMODEL:\nm.bin\nMODE:\nsynthesis\nFILEPROFILE:\np\n")   # no lines
+    with pytest.raises(NotImplementedError):
+        control.parse("BOX:\n1 1\n1 1\n1 1\nFILEMODEL:\nm\nMODE:\ntau\n")
+    with pytest.raises(NotImplementedError):
+        control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nTEMP\n")

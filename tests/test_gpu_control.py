@@ -1,0 +1,130 @@
+"""The reference's own test control files run through the device path
+(firtez_dz_amd/control.py) in the reference's directory layout
+(tests/test_*/control_file.dat, default/*.bin, relative paths unchanged),
+their output files read back with the reference's formats and compared with
+the reference's outputs held in tests/golden (syn1.npz, rf.npz, he.npz,
+written by the reference's Fortran)."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from firtez_dz_amd import control, modelio
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+STOKES_TOL = 1e-4
+RF_TOL = 2e-3
+
+
+def report(name, err, tol):
+    print("[control] %-44s max err %.3e (tol %.1e)" % (name, err, tol), flush=True)
+    assert err <= tol, name
+
+
+@pytest.fixture
+def tree(tmp_path):
+    (tmp_path / "default").mkdir()
+    for f in ("test_atmos.bin", "rf_test_atmos.bin"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / "default" / f)
+    for t in ("test_syn1", "test_syn2", "test_rf", "test_he", "test_tau1", "test_tau2", "test_inv1"):
+        d = tmp_path / "tests" / t
+        (d / "out_dir").mkdir(parents=True)
+        shutil.copy(os.path.join(GOLD, "control", t + ".dat"), d / "control_file.dat")
+    return tmp_path
+
+
+def run(tree, name, ctx, edit=None):
+    """Runs tests/<name>/control_file.dat; returns (control, result, output
+    directory): OUTPATH relative to the test directory, the directory itself
+    when the file sets none."""
+    wd = str(tree / "tests" / name)
+    c = control.parse_file(os.path.join(wd, "control_file.dat"))
+    if edit:
+        edit(c)
+    print("[control] running %s (%s)" % (name, c.mode), flush=True)
+    return c, control.run(c, wd, ctx), os.path.join(wd, c.outpath)
+
+
+def model_rel(cube, ref, rows):
+    names = {1: "Pg", 2: "rho", 3: "Pe", 4: "mu"}
+    return max(float(np.max(np.abs(cube[names[r]][:, 0, 0] - ref[r]) / np.abs(ref[r]))) for r in rows)
+
+
+def test_syn1_and_tau1(gpu_ctx, golden, tree):
+    g = golden["syn1"]
+    _, out, wd = run(tree, "test_syn1", gpu_ctx)
+    st, wave, index = modelio.read_profiles(os.path.join(wd, "syn_profile.bin"))
+    assert st.shape == (4, 500, 1, 1)
+    assert wave[0] == -700.0 and np.allclose(index, 6301.5010, atol=1e-3)
+    report("test_syn1 profiles vs reference", float(np.max(np.abs(st[:, :, 0, 0] - g["stokes"]))), STOKES_TOL)
+    m = modelio.read_model(os.path.join(wd, "out_test_atmos.bin"))
+    report("test_syn1 out model rho/Pe/mu (relative)", model_rel(m, g["model_out"], (2, 3, 4)), 1e-6)
+    report("test_syn1 out model log tau5", float(np.max(np.abs(m["tau5"][:, 0, 0] - g["model_out"][11]))), 1e-5)
+    # MODE tau: the same model outputs without lines
+    _, _, wd = run(tree, "test_tau1", gpu_ctx)
+    m = modelio.read_model(os.path.join(wd, "out_test_atmos.bin"))
+    report("test_tau1 out model rho/Pe/mu (relative)", model_rel(m, g["model_out"], (2, 3, 4)), 1e-6)
+    report("test_tau1 out model log tau5", float(np.max(np.abs(m["tau5"][:, 0, 0] - g["model_out"][11]))), 1e-5)
+
+
+def test_rf(gpu_ctx, golden, tree):
+    g = golden["rf"]
+    _, out, wd = run(tree, "test_rf", gpu_ctx)
+    st, _, _ = modelio.read_profiles(os.path.join(wd, "syn_profile.bin"))
+    report("test_rf profiles vs reference", float(np.max(np.abs(st[:, :, 0, 0] - g["stokes"]))), STOKES_TOL)
+    d, z, wave, logv = modelio.read_rfs(os.path.join(wd, "rfout_rf_test_atmos.bin"))
+    assert list(logv) == [True, False, False, False, False, True, False, False]
+    assert d.shape == (8, 4, 500, 256, 1, 1)
+    for p, name in ((0, "T"), (5, "Bz")):
+        ours = d[p][:, :, g["rf_heights"], 0, 0]                 # (4, nw, nzs)
+        ref = np.transpose(g["rf"][:, :, p, :], (2, 1, 0))       # (4, nw, nzs)
+        report("test_rf rfout d/d%s vs reference" % name, float(np.max(np.abs(ours - ref)) / np.abs(ref).max()),
+               RF_TOL)
+    assert not np.any(d[[1, 2, 3, 4, 6, 7]])
+
+
+def test_hydrostatic_modes(gpu_ctx, golden, tree):
+    g = golden["he"]
+    for name in ("test_he", "test_tau2", "test_syn2"):
+        _, _, wd = run(tree, name, gpu_ctx)
+        m = modelio.read_model(os.path.join(wd, "out_test_atmos.bin"))
+        report("%s out model Pg/rho/Pe/mu (relative)" % name, model_rel(m, g["model_out"], (1, 2, 3, 4)), 1e-6)
+        if name != "test_he":
+            report("%s out model log tau5" % name, float(np.max(np.abs(m["tau5"][:, 0, 0] - g["model_out"][11]))),
+                   1e-5)
+    st, _, _ = modelio.read_profiles(str(tree / "tests" / "test_syn2" / "out_dir" / "syn_profile.bin"))
+    assert st.shape == (4, 500, 1, 1) and np.all(np.isfinite(st))
+
+
+def test_inversion_after_synthesis(gpu_ctx, tree):
+    """test_inv1 unchanged: it reads test_syn1's output and inverts T and
+    v_z with HYDROSTATIC (Pg re-integrated from the top), vertical
+    regularisation, the CG solver and PERTURBATION_TYPE 2, its nodes growing
+    to NZ = 256 per parameter (512 free parameters: the LM step with its work
+    arrays in device memory).  The start model is the one that made the
+    observations, but the hydrostatic integration replaces its Pg, so the
+    start misfit is that of the hydrostatic synthesis of the same model; the
+    inversion must lower it."""
+    run(tree, "test_syn1", gpu_ctx)
+
+    def hydro_start(c):
+        c.hydrostatic = True
+        c.fileprofile = "he_start.bin"
+
+    run(tree, "test_syn1", gpu_ctx, hydro_start)
+    out_syn1 = tree / "tests" / "test_syn1" / "out_dir"
+    obs, _, _ = modelio.read_profiles(str(out_syn1 / "syn_profile.bin"))
+    start, _, _ = modelio.read_profiles(str(out_syn1 / "he_start.bin"))
+    c, out, wd = run(tree, "test_inv1", gpu_ctx)
+    st, _, _ = modelio.read_profiles(os.path.join(wd, "out_syn_profile.bin"))
+    chi = float(np.asarray(out["chi2"]).reshape(-1)[0])
+    e0 = float(np.sqrt(np.mean((start - obs) ** 2)))
+    e1 = float(np.sqrt(np.mean((st - obs) ** 2)))
+    print("[control] test_inv1: %d LM iterations, chi2 %.4g; rms misfit start %.3e -> %.3e"
+          % (out["iterations"], chi, e0, e1), flush=True)
+    assert np.isfinite(chi) and e1 < 0.5 * e0
+    m = modelio.read_model(os.path.join(wd, "out_test_atmos.bin"))
+    assert np.all(np.isfinite(m["T"])) and np.all(m["T"] >= 2500.0)

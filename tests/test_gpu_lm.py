@@ -27,7 +27,7 @@ def report(name, err, tol):
     assert err <= tol, name
 
 
-def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7):
+def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7, solver=1):
     import torch
     assert torch.cuda.is_available()
     ls = lines.fe6301_6302()
@@ -49,14 +49,14 @@ def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7):
     guess[0] += 250.0 * np.linspace(1.0, 0.2, nz, dtype=np.float32)[:, None]
     guess[5:8] *= np.float32(0.7)
     guess[10] += np.float32(1.0e5)
-    inv = gpu.inversion(nodes)
+    inv = gpu.inversion(nodes, solver=solver)
     ifreep, nfreq = gpu_ctx.set_inversion(inv)
     atmpar = [0] * 8
     nd = [0] * 8
     for k, v in nodes.items():
         i = gpu.INV_PARAMS.index(k)
         atmpar[i], nd[i] = 1, v
-    cfg = oracle.lm_cfg(nz, numw, atmpar, nd)
+    cfg = oracle.lm_cfg(nz, numw, atmpar, nd, solver=solver)
     coefs = cpu.coefs_block(cfg, {i: cpu.int_coefs(nz, nd[i]) for i in range(8) if atmpar[i]})
     return dict(z=z, gain=gain, true=true, obs=obs, guess=guess, cfg=cfg, coefs=coefs, ifreep=ifreep,
                 nfreq=nfreq, numw=numw, ncol=ncol, nz=nz, ls=ls)
@@ -135,10 +135,20 @@ def _oracle_step(cpu, P, obs, syn, bsyn, sla, eqv, ebest, isig, cur, best):
     return out
 
 
-@pytest.mark.parametrize("nodes", [NODES_C1, NODES_C2], ids=["cycle1", "cycle2"])
-def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes):
+NODES_BIG = {"T": 40, "Bz": 16, "vz": 24}
+
+
+# cycle1/cycle2: Jacobi eigen-decomposition with the work arrays in LDS;
+# cg: the conjugate-gradient solver (LINEAR_SOLVER 2) on the six free
+# parameters of a first cycle (on larger, worse-conditioned systems its
+# iterate path and stopping iteration follow the rounding of the dot
+# products, which the device forms in a tree); big: 80 free parameters,
+# the work arrays in device memory
+@pytest.mark.parametrize("nodes,solver", [(NODES_C1, 1), (NODES_C2, 1), (NODES_C1, 2), (NODES_BIG, 1)],
+                         ids=["cycle1", "cycle2", "cg", "big"])
+def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver):
     import torch
-    P = problem(gpu_ctx, cpu, nodes)
+    P = problem(gpu_ctx, cpu, nodes, solver=solver)
     ncol, nf, npr, numw = P["ncol"], P["nfreq"], P["ifreep"], P["numw"]
     d = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
     atm = d(P["guess"], torch.float32)
@@ -175,8 +185,11 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes):
             werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
                                                       (np.abs(mc[row]) + 1e-3)))
     report("LM step (accept) chi2 vs oracle (relative)", werr["chi"], 1e-11)
-    report("LM step (accept) node perturbation vs oracle", werr["delta"], 1e-6)
-    report("LM step (accept) new model vs oracle (relative)", werr["model"], 2e-6)
+    # the CG iteration compounds the rounding of its dot products (a tree of
+    # partial sums on the device, a running sum in the oracle) over its steps
+    dtol, mtol = (1e-6, 2e-6) if solver == 1 else (1e-4, 1e-4)
+    report("LM step (accept) node perturbation vs oracle", werr["delta"], dtol)
+    report("LM step (accept) new model vs oracle (relative)", werr["model"], mtol)
 
     # second step with a worse synthesis: every pixel rejects
     worse = (st + 0.05).contiguous()
@@ -198,8 +211,8 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes):
         for row in (0, 5, 6, 7, 10):
             werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
                                                       (np.abs(mc[row]) + 1e-3)))
-    report("LM step (reject) node perturbation vs oracle", werr["delta"], 1e-6)
-    report("LM step (reject) new model vs oracle (relative)", werr["model"], 2e-6)
+    report("LM step (reject) node perturbation vs oracle", werr["delta"], dtol)
+    report("LM step (reject) new model vs oracle (relative)", werr["model"], mtol)
 
 
 def test_lm_loop_lowers_chi2_like_oracle(gpu_ctx, cpu):
