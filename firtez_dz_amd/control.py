@@ -284,6 +284,7 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
     own = ctx is None
     if own:
         ctx = _gpu.Context(0)
+    hydro_before = ctx.hydrostatic
     try:
         def path(p, name):
             return os.path.join(workdir, p + name)
@@ -357,8 +358,12 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
         out["written"].append(p)
         return out
     finally:
+        # a caller's context leaves as it came (HYDROSTATIC is a context
+        # setting that outlives the run)
         if own:
             ctx.close()
+        else:
+            ctx.set_hydrostatic(hydro_before)
 
 
 def _stokes4(syn3d, numw, ny, nx):

@@ -200,6 +200,7 @@ class Context:
         self.nz = 0
         self.lineset = None
         self.win_offsets = None
+        self.hydrostatic = False
 
     def close(self):
         if self.h:
@@ -242,6 +243,7 @@ class Context:
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside every
         forward, and the hydrostatic term in dI/dT."""
         self._check(self.lib.fz_set_hydrostatic(self.h, int(on)), "fz_set_hydrostatic")
+        self.hydrostatic = bool(on)
 
     def set_lsf(self, lsf_kernel=None, valid=None):
         """MLSF: LSF_KERNEL(NUMW) and LSF_VALID(NUML) as LSF_INIT leaves them
