@@ -21,7 +21,7 @@ constexpr int LM_MAXNP = 64;     // free parameters per pixel with the work arra
 // one wave per pixel: the Jacobi rounds and reductions are wave-local, so
 // the barriers are cheap and a CU holds many pixels at once
 constexpr int LM_THREADS = 64;
-constexpr int LM_CH = 64;        // spectral rows per chunk of the normal matrix
+constexpr int LM_CH = 32;        // spectral rows per chunk of the normal matrix
 // more free parameters (up to NZ nodes per parameter, the reference's
 // default): the same step with its work arrays in a per-pixel slice of
 // device memory, four waves per pixel

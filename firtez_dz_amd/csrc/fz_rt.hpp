@@ -97,7 +97,9 @@ __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E)
   const float e2 = dot(E.e, E.e), r2 = dot(E.r, E.r);
   E.er = dot(E.e, E.r);
   E.d = e2 - r2;
-  E.th = hypotf(E.d, 2.0f * E.er);
+  // hypot without its scaling unless the squares could overflow
+  E.th = (fabsf(E.d) < 1.0e18f && fabsf(E.er) < 1.0e18f) ? sqrtf(E.d * E.d + 4.0f * E.er * E.er)
+                                                          : hypotf(E.d, 2.0f * E.er);
   float l1, l2;
   if (E.d >= 0.f) {
     l1 = sqrtf(0.5f * (E.th + E.d));
@@ -110,9 +112,14 @@ __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E)
   E.l2 = l2;
   E.sig = E.er > 0.f ? 1.f : (E.er < 0.f ? -1.f : 0.f);
   if (l1 < 40.f) {
+    // cosh and sinh from one exponential; sinh by its Taylor series below
+    // 0.5 (truncation 1e-11 relative), where E - 1/E would cancel
     const float ex = expf(-E.ei);
-    E.che = ex * coshf(l1);
-    E.she = ex * sinhf(l1);
+    const float El = expf(l1), iEl = 1.0f / El;
+    const float x2 = l1 * l1;
+    const float shs = l1 * (1.f + x2 * (1.f / 6.f) * (1.f + x2 * (1.f / 20.f) * (1.f + x2 * (1.f / 42.f) * (1.f + x2 * (1.f / 72.f)))));
+    E.che = ex * (0.5f * (El + iEl));
+    E.she = ex * (l1 < 0.5f ? shs : 0.5f * (El - iEl));
   } else {
     const float p = expf(l1 - E.ei), m = expf(-l1 - E.ei);
     E.che = 0.5f * (p + m);

@@ -22,6 +22,8 @@ GEOMETRY = 20
 SOURCE = 4
 SOURCE_DER = 10
 EVOL_BUILD = 160                      # evol_build: scaling, eigenvalues, exp/cosh/sinh/cos/sin, O assembly
+                                      # (cosh, sinh from one exp, a reciprocal and a 4-term series for small
+                                      # arguments: kept at 160, an undercount of about 15)
 FORMAL = 40                           # I_k = (1-O) S + O I_{k-1}
 M4V = 28                              # v = I_in - S, M4raw v
 EVOL_DIR = 257                        # evol_dir: one directional derivative (dO) v
