@@ -1143,6 +1143,9 @@ struct fz_ctx {
   std::vector<int> slot_of;   // [np] physical parameter (= Cartesian slot) of each free parameter
   std::vector<int> kbeg, kend;  // [np] support of each node's hat function
   int* d_eqvnode = nullptr;   // [4][np]: kbeg, kend, Cartesian slot, five-slot index
+  int* d_eqvtab_j = nullptr;  // [nparam][nz] node table of the pairwise contraction
+  double* d_eqvtab_c = nullptr;  // [nparam][nz][2]
+  bool eqv_pairs = false;
   bool lm_big = false;        // more than LM_MAXNP free parameters: work arrays in device memory
   size_t lm_work = 0;         // bytes of one pixel's work arrays (lm_big)
   double* d_lmwork = nullptr;
@@ -1245,6 +1248,8 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_coefs) (void)hipFree(c->d_coefs);
   if (c->d_colsum) (void)hipFree(c->d_colsum);
   if (c->d_eqvnode) (void)hipFree(c->d_eqvnode);
+  if (c->d_eqvtab_j) (void)hipFree(c->d_eqvtab_j);
+  if (c->d_eqvtab_c) (void)hipFree(c->d_eqvtab_c);
   if (c->d_lmwork) (void)hipFree(c->d_lmwork);
   if (c->d_jtile) (void)hipFree(c->d_jtile);
   if (c->d_sbuf) (void)hipFree(c->d_sbuf);
@@ -2055,6 +2060,46 @@ int fz_set_inversion(fz_ctx* c, const fz_inversion* inv, int* ifreep, int* nfreq
   HIPCHK(c, hipMemcpy(c->d_coefs, coefs.data(), sizeof(double) * coefs.size(), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_colsum, colsum.data(), sizeof(double) * np, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->d_eqvnode, node.data(), sizeof(int) * node.size(), hipMemcpyHostToDevice));
+  // the pairwise contraction applies when every parameter's node supports
+  // overlap at most pairwise and move up with the node index (the hat basis
+  // of GET_INT_COEFS); its per-height table
+  c->eqv_pairs = true;
+  for (int ip = 0; ip < npar; ++ip)
+    for (int j = L.j0[ip]; j < L.j0[ip] + L.nj[ip]; ++j) {
+      const int je = L.j0[ip] + L.nj[ip] - 1;
+      if (c->kbeg[j] > c->kend[j]) c->eqv_pairs = false;
+      if (j + 1 <= je && (c->kbeg[j + 1] < c->kbeg[j] || c->kend[j + 1] < c->kend[j])) c->eqv_pairs = false;
+      if (j + 2 <= je && c->kbeg[j + 2] <= c->kend[j]) c->eqv_pairs = false;
+    }
+  if (c->d_eqvtab_j) (void)hipFree(c->d_eqvtab_j);
+  if (c->d_eqvtab_c) (void)hipFree(c->d_eqvtab_c);
+  c->d_eqvtab_j = nullptr;
+  c->d_eqvtab_c = nullptr;
+  if (c->eqv_pairs) {
+    std::vector<int> tj((size_t)npar * nz, 0);
+    std::vector<double> tc((size_t)npar * nz * 2, 0.0);
+    for (int ip = 0; ip < npar; ++ip) {
+      const int jb = L.j0[ip], je = jb + L.nj[ip] - 1;
+      int jl = jb;
+      for (int k = c->kbeg[jb]; k <= c->kend[je]; ++k) {
+        while (k > c->kend[jl]) ++jl;
+        int w = jl;
+        if (k >= c->kbeg[jl]) {
+          w |= 1 << 30;
+          tc[((size_t)ip * nz + k) * 2] = coefs[(size_t)jl * nz + k];
+        }
+        if (jl < je && k >= c->kbeg[jl + 1]) {
+          w |= 1 << 29;
+          tc[((size_t)ip * nz + k) * 2 + 1] = coefs[(size_t)(jl + 1) * nz + k];
+        }
+        tj[(size_t)ip * nz + k] = w;
+      }
+    }
+    HIPCHK(c, hipMalloc((void**)&c->d_eqvtab_j, sizeof(int) * tj.size()));
+    HIPCHK(c, hipMalloc((void**)&c->d_eqvtab_c, sizeof(double) * tc.size()));
+    HIPCHK(c, hipMemcpy(c->d_eqvtab_j, tj.data(), sizeof(int) * tj.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_eqvtab_c, tc.data(), sizeof(double) * tc.size(), hipMemcpyHostToDevice));
+  }
   if (!c->lm_big)
     HIPCHK(c, hipFuncSetAttribute((const void*)lm_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)c->lm_lds));
@@ -2131,17 +2176,9 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     e.j0[ip] = c->lm.j0[ip];
     e.nj[ip] = c->lm.nj[ip];
   }
-  // one pass per parameter when its node supports overlap at most pairwise
-  // and move up with the node index (the hat basis of GET_INT_COEFS does)
-  const std::vector<int>& kb = c->kbeg;
-  const std::vector<int>& ke = c->kend;
-  e.pairs = 1;
-  for (int ip = 0; ip < e.nparam; ++ip)
-    for (int j = e.j0[ip]; j < e.j0[ip] + e.nj[ip]; ++j) {
-      if (kb[j] > ke[j]) e.pairs = 0;
-      if (j + 1 < e.j0[ip] + e.nj[ip] && (kb[j + 1] < kb[j] || ke[j + 1] < ke[j])) e.pairs = 0;
-      if (j + 2 < e.j0[ip] + e.nj[ip] && kb[j + 2] <= ke[j]) e.pairs = 0;
-    }
+  e.pairs = c->eqv_pairs ? 1 : 0;
+  e.tab_j = c->d_eqvtab_j;
+  e.tab_c = c->d_eqvtab_c;
   e.coefs = c->d_coefs;
   e.eqv = d_eqv;
   e.poff = c->win_on ? c->d_poff : nullptr;

@@ -83,6 +83,12 @@ struct EqvArgs {
   const int* slot_of;          // [np] response-function slot of each free parameter
   const int* kbeg;             // [np] support of each node's hat function
   const int* kend;             // [np]
+  // per parameter and height (pairs): the lowest unfinished node in bits
+  // 0-15, bit 30 set when it takes height k, bit 29 when the next node does;
+  // tab_c the two coefficients, so each height's loads are independent of
+  // the running node index
+  const int* tab_j;            // [nparam][nz]
+  const double* tab_c;         // [nparam][nz][2]
   // the nodes of inverted parameter ip are j0[ip] .. j0[ip]+nj[ip]-1; pairs:
   // every height lies in the support of at most two consecutive nodes of a
   // parameter and the supports move up with the node index (hat functions),
@@ -145,18 +151,21 @@ __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
     for (int ip = 0; ip < a.nparam; ++ip) {
       const int jb = a.j0[ip], je = jb + a.nj[ip] - 1;
       const float* jp = jc + (long)a.slot_of[jb] * a.nz * cs;
+      const int* tj = a.tab_j + (long)ip * a.nz;
+      const double* tc = a.tab_c + (long)ip * a.nz * 2;
       int jl = jb;                                   // lowest unfinished node
       double alo = 0.0, ahi = 0.0;                   // sums of nodes jl, jl+1
       for (int k = a.kbeg[jb]; k <= a.kend[je]; ++k) {
-        while (k > a.kend[jl]) {
+        const int w = tj[k];
+        while (jl < (w & 0xffff)) {
           out[(long)jl * fstride] = (float)alo;
           alo = ahi;
           ahi = 0.0;
           ++jl;
         }
         const double v = (double)jp[k * cs];
-        if (k >= a.kbeg[jl]) alo += v * a.coefs[(long)jl * a.nz + k];
-        if (jl < je && k >= a.kbeg[jl + 1]) ahi += v * a.coefs[(long)(jl + 1) * a.nz + k];
+        if (w & (1 << 30)) alo += v * tc[2 * k];
+        if (w & (1 << 29)) ahi += v * tc[2 * k + 1];
       }
       out[(long)jl * fstride] = (float)alo;
       if (jl < je) out[(long)(jl + 1) * fstride] = (float)ahi;
