@@ -198,7 +198,11 @@ typedef struct {
 int fz_int_coefs(int nz, int nj, double* coefs);
 
 /* Configure the inversion of this context (after fz_configure); returns the
- * number of free parameters IFREEP through *ifreep and NFREQ through *nfreq. */
+ * number of free parameters IFREEP through *ifreep and NFREQ through *nfreq.
+ * Replaces SET_INVERSION_CONSTANTS / INITIALIZE_CYCLE's per-cycle set-up
+ * (pre_post_duties.f90:625-700, invert3d.f90:228-325).  IFREEP up to 2048:
+ * up to 64 the LM step keeps its work arrays in LDS, beyond in device memory
+ * that the library allocates per launch batch. */
 int fz_set_inversion(fz_ctx* ctx, const fz_inversion* inv, int* ifreep, int* nfreq);
 
 /* FORWARD3D in inversion mode on device data: Stokes as fz_forward_dev and
