@@ -230,21 +230,23 @@ __device__ void lm_penalties(const LmArgs& a, const float* mod, double* pen_res,
       }
       pen_res[ji + j] = s;
     }
-    for (int i = 0; i < jd; ++i)
-      for (int j = 0; j < jd; ++j) {
-        double s = 0.0;
-        if (a.pen_typ[p] == 0) {
-          if (i == j) s = ((1.0 / (double)jd) * alp) * ((1.0 / (double)jd) * alp);
-        } else if (a.pen_typ[p] == 1) {
-          // (L^T L)_ij = sum_k L_ki L_kj, L_kk = alp (k>0), L_k,k-1 = -alp
-          for (int k = 1; k < jd; ++k) {
-            const double lki = (k == i) ? alp : ((k - 1 == i) ? -alp : 0.0);
-            const double lkj = (k == j) ? alp : ((k - 1 == j) ? -alp : 0.0);
-            s += lki * lkj;
-          }
-        }
-        pen_hss[(ji + i) * np + (ji + j)] = s;
+    // L^T L into the zeroed block: type 0 is diagonal; type 1, with
+    // L_kk = alp (k>0) and L_k,k-1 = -alp, is tridiagonal:
+    // (L^T L)_ii = sum over k = i (i>0), then k = i+1 (i+1<jd) of alp^2,
+    // (L^T L)_i,i+-1 = -alp^2 from the single k = max(i, j)
+    for (int i = 0; i < jd; ++i) {
+      double* row = pen_hss + (long)(ji + i) * np + ji;
+      if (a.pen_typ[p] == 0) {
+        row[i] = ((1.0 / (double)jd) * alp) * ((1.0 / (double)jd) * alp);
+      } else if (a.pen_typ[p] == 1) {
+        double d = 0.0;
+        if (i > 0) d += alp * alp;
+        if (i + 1 < jd) d += (-alp) * (-alp);
+        row[i] = d;
+        if (i > 0) row[i - 1] = alp * -alp;
+        if (i + 1 < jd) row[i + 1] = -alp * alp;
       }
+    }
   }
 }
 
@@ -549,10 +551,13 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
       for (int i = t; i < np; i += NT) s += x[i] * y[i];
       return block_sum<NT>(s, red);
     };
+    // A is exactly symmetric here (its lower triangle mirrored, a symmetric
+    // penalty Hessian, the diagonal scaled): row i is read as column i, so
+    // the lanes of a wave read consecutive entries
     auto matvec = [&](const double* x, double* y) {
       for (int i = t; i < np; i += NT) {
         double s = 0.0;
-        for (int j = 0; j < np; ++j) s += A[i * ld + j] * x[j];
+        for (int j = 0; j < np; ++j) s += A[j * ld + i] * x[j];
         y[i] = s;
       }
       __syncthreads();
@@ -589,7 +594,7 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
       double cs = 0.0;
       for (int i = t; i < np; i += NT) {
         double s = 0.0;
-        for (int j = 0; j < np; ++j) s += A[i * ld + j] * xv[j];
+        for (int j = 0; j < np; ++j) s += A[j * ld + i] * xv[j];
         const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
         cs += e * e;
       }

@@ -358,3 +358,40 @@ def test_p0_response_vs_oracle(gpu_ctx, cpu):
             ref = np.transpose(dsyn[:, :, p, :], (0, 2, 1))
             report("P0 mode col %d RF p%d" % (c, p), np.max(np.abs(jac[c, p] - ref)) / np.abs(ref).max(), RF_TOL)
         report("P0 mode col %d Stokes" % c, np.max(np.abs(st[c] - syn)), STOKES_TOL)
+
+
+@pytest.mark.parametrize("hydro", [False, True])
+def test_tiled_forward_identical(gpu_ctx, hydro):
+    """Six radiative-transfer tiles (the last one ragged) against one: each
+    column's arithmetic does not depend on the tiling, so Stokes, Jacobians,
+    status and the model outputs (Pg under HYDROSTATIC) must be bit-identical
+    to the single-tile forward, also with profiling on."""
+    import torch
+    ls = lines.fe6301_6302()
+    atm, z = atmos.seeded(16, 16, 64, seed=11)
+    gpu_ctx.configure(ls, z)
+    gpu_ctx.gain_hsra()
+    t0 = torch.from_numpy(atm.copy()).cuda()
+    assert np.all(gpu_ctx.hydrostatic_torch(t0).cpu().numpy() == 0)
+    gpu_ctx.set_hydrostatic(hydro)
+    try:
+        outs = []
+        for tile, prof in ((8192, False), (48, False), (48, True)):
+            gpu_ctx.set_tile(tile, 256)
+            gpu_ctx.set_profiling(prof)
+            t = t0.clone()
+            st, jac, status = gpu_ctx.forward_torch(t, 1)
+            torch.cuda.synchronize()
+            if prof:
+                micro_ms, rt_ms, launches = gpu_ctx.last_timing()
+                assert launches == 6 and micro_ms > 0 and rt_ms > 0
+            outs.append([x.cpu().numpy() for x in (st, jac, status, t)])
+    finally:
+        gpu_ctx.set_tile(8192, 256)
+        gpu_ctx.set_profiling(False)
+        gpu_ctx.set_hydrostatic(False)
+    assert np.all(outs[0][2] == 0)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    print("[parity] tiled forward (6 tiles, HYDROSTATIC %s): bit-identical" % hydro, flush=True)
