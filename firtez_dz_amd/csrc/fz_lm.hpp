@@ -578,23 +578,38 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     __syncthreads();
     double pcond = 1.0e29, dcond = -1.0e29;
     bool conv = false;
-    for (int it = 1; it <= 100000; ++it) {
+    // each iteration forms the next direction before its stopping test, so
+    // one pass over A gives both A x (the test) and A p (the next step);
+    // each of the two sums keeps the order of a separate product
+    {
       const double crho0 = crho;
       crho = dot(rv, rv);
       const double cbeta = crho / crho0;
       for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
       __syncthreads();
       matvec(pv, qv);
+    }
+    for (int it = 1; it <= 100000; ++it) {
       calpha = crho / dot(pv, qv);
       for (int i = t; i < np; i += NT) {
         xv[i] = xv[i] + calpha * pv[i];
         rv[i] = rv[i] - calpha * qv[i];
       }
       __syncthreads();
+      const double crho0 = crho;
+      crho = dot(rv, rv);
+      const double cbeta = crho / crho0;
+      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
+      __syncthreads();
       double cs = 0.0;
       for (int i = t; i < np; i += NT) {
-        double s = 0.0;
-        for (int j = 0; j < np; ++j) s += A[j * ld + i] * xv[j];
+        double s = 0.0, sq = 0.0;
+        for (int j = 0; j < np; ++j) {
+          const double aij = A[j * ld + i];
+          s += aij * xv[j];
+          sq += aij * pv[j];
+        }
+        qv[i] = sq;
         const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
         cs += e * e;
       }
