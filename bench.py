@@ -43,27 +43,65 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
 
-def cpu_baseline_reference(nz, seconds=10.0, procs=16):
+def cpu_baseline_reference(nz, seconds=10.0):
     """The reference's own Fortran column path (oracle/_ref, compiled from its
-    sources) on the host cores, one process per core, in a child process that
-    never touches the GPU; None when the reference build is absent."""
+    sources) on the host cores, one process per core of this process's
+    affinity mask (at most 16), in a child process that never touches the
+    GPU; None when the reference build is absent."""
     import subprocess
     lib = os.path.join(ROOT, "oracle", "_ref", "libfzref.so")
     if not os.path.exists(lib):
         return None
-    r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--nz", str(nz), "--seconds", str(seconds),
-                        "--procs", str(procs)], cwd=ROOT, capture_output=True, text=True, timeout=seconds * 4 + 120)
+    r = subprocess.run([sys.executable, "-m", "oracle.refbench", "--nz", str(nz), "--seconds", str(seconds)],
+                       cwd=ROOT, capture_output=True, text=True, timeout=seconds * 4 + 120)
     if r.returncode != 0:
         print("[bench] reference CPU baseline failed: " + r.stderr[-400:], file=sys.stderr, flush=True)
         return None
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def cpu_baseline(ls, nz, gain, seconds=12.0, threads=16):
+def oracle_error(ls, z, gain, atm_h, st_h, jac_h, cols):
+    """Error of the timed cube against the FP64 oracle on sampled columns:
+    Stokes |dS| in units of I_c; spherical Jacobians |dJ| / max|J| per
+    parameter (the oracle's Cartesian response functions through the
+    closed-form chain rule to |B|, inclination, azimuth).  Runs after the
+    timed region, on the host."""
+    import oracle
+    from firtez_dz_amd import atmos
+    cpu = oracle.Cpu()
+    cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 0))
+    cpu.set_gain(gain)
+    models = np.stack([atmos.column_model(atm_h, z, int(c)) for c in cols])
+    _, syn, dsyn = cpu.forward_batch(models)
+    ds = np.abs(st_h - np.transpose(syn, (0, 2, 1))).reshape(-1)
+    errs = []
+    for i, m in enumerate(models):
+        tr = lambda p: np.transpose(dsyn[i, :, :, p, :], (0, 2, 1))
+        bx, by, bz = (float(m[j, 0]) for j in (5, 6, 7))
+        B = np.sqrt(bx * bx + by * by + bz * bz)
+        g, f = np.arccos(bz / B), np.arctan2(by, bx)
+        dx, dy, dzz = tr(3), tr(4), tr(5)
+        ref = [tr(0), (bx * dx + by * dy + bz * dzz) / B,
+               B * np.cos(g) * np.cos(f) * dx + B * np.cos(g) * np.sin(f) * dy - B * np.sin(g) * dzz,
+               -B * np.sin(g) * np.sin(f) * dx + B * np.sin(g) * np.cos(f) * dy, tr(6)]
+        for p in range(5):
+            sc = np.abs(ref[p]).max()
+            if sc > 0:
+                errs.append(np.abs(jac_h[i, p] - ref[p]).reshape(-1) / sc)
+    e = np.concatenate(errs)
+    return {"columns": len(cols), "stokes_max_abs_Ic": float(ds.max()), "stokes_p99_abs_Ic": float(np.percentile(ds, 99)),
+            "jacobian_max_rel": float(e.max()), "jacobian_p99_rel": float(np.percentile(e, 99)),
+            "tolerance": {"stokes_abs_Ic": 1e-4, "jacobian_rel": 2e-3},
+            "oracle": "FP64 C restatement of the reference column path"}
+
+
+def cpu_baseline(ls, nz, gain, seconds=12.0, threads=0):
     """The oracle (FP64 C restatement, OpenMP over columns) on a bounded sample
     of the same workload: columns of the seeded cube, full Jacobians."""
     import oracle
     from firtez_dz_amd import atmos
+    threads = threads or min(len(os.sched_getaffinity(0)), 16)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     cpu = oracle.Cpu()
     numw = cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 0))
     cpu.set_gain(gain)
@@ -173,14 +211,14 @@ def main():
     fl = flops.per_column_lambda(args.nz, 2, ncomp, npar)
     # per launch: the tile's columns x wavelengths x flop, over the average launch time
     launch_ms = rt_avg_ms / nlaunch
-    # HBM bytes of rt_kernel<1> per column, from the FETCH_SIZE and WRITE_SIZE
-    # counters of one launch of 9216 columns of this workload (nz = 96, 112
-    # wavelengths, spherical Jacobian), each scaled by the counter's reading for
-    # a 256 MiB device copy (FETCH_SIZE reads half the copied bytes: x2;
-    # WRITE_SIZE exact: x1): 3.53 GB read + 9.53 GB written
+    # HBM bytes of rt_kernel<1> per column of this workload (nz = 96, 112
+    # wavelengths, spherical Jacobian) from the FETCH_SIZE and WRITE_SIZE
+    # counters of the kernel (two rocprofv3 --pmc runs of bench.py, each
+    # counter scaled by its reading for a device copy of known size); a
+    # constant of the profile, not measured in this run
     traffic = None
     if args.nz == 96 and numw == 112:
-        traffic = 1.4178e6 * ncol / nlaunch
+        traffic = flops.RT_HBM_BYTES_PER_COLUMN * ncol / nlaunch
     achieved = fl * ncol * numw / (rt_avg_ms * 1e-3) / 1e12
     print("[bench] step %.1f ms; micro %.1f ms; rt %.1f ms (%d launches); %.0f kflop/column-lambda"
           % (ms_step, np.mean(micro_ms), rt_avg_ms, nlaunch, fl / 1e3), file=sys.stderr, flush=True)
@@ -195,16 +233,26 @@ def main():
                    "tiles": {"rt_columns": args.tile, "micro_columns": tiles[1] if tiles else None,
                              "autotuned": tiles is not None}},
         # compute-bound: FP32 arithmetic on the vector ALUs (no matrix work in
-        # this kernel); priced against MI355X's 157.3 TFLOP/s FP32 peak
-        "roofline": {"bound": "mfma", "pipe": "fp32 valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
+        # this kernel); priced against MI355X's published 157.3 TFLOP/s FP32
+        # peak, which counts packed FP32 (this build emits none: 78.6 without)
+        "roofline": {"bound": "valu", "pipe": "fp32 valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "traffic_source": flops.RT_HBM_SOURCE if traffic else None,
                      "kernel": "rt_kernel<1>", "launch_ms": launch_ms, "launches_per_step": nlaunch,
                      "flop_per_column_lambda": fl},
     }
     if not args.no_cpu:
+        # the oracle as the checker of the timed cube (64 sampled columns, after the timed region)
+        cols = np.sort(np.random.default_rng(64).choice(ncol, 64, replace=False))
+        icols = torch.from_numpy(cols).cuda()
+        err = oracle_error(ls, z, gain, d_atm[:, :, icols].cpu().numpy(), stokes[icols].cpu().numpy(),
+                           jac[icols].cpu().numpy(), np.arange(64))
+        print("[bench] error vs oracle (64 columns): Stokes max %.2e Ic, Jacobian max %.2e"
+              % (err["stokes_max_abs_Ic"], err["jacobian_max_rel"]), file=sys.stderr, flush=True)
+        line["error"] = err
         base = None
         if args.cpu_kind == "reference":
-            print("[bench] CPU baseline (reference Fortran, 16 processes) ...", file=sys.stderr, flush=True)
+            print("[bench] CPU baseline (reference Fortran, one process per core) ...", file=sys.stderr, flush=True)
             base = cpu_baseline_reference(args.nz, seconds=args.cpu_seconds)
         if base is None:
             print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)

@@ -200,7 +200,7 @@ def parse(text: str) -> Control:
                         if key.endswith("TYPE:"):
                             c.pen_typ[name] = _nint(v)
                         else:
-                            c.pen_alp[name] = float(v)
+                            c.pen_alp[name] = float(np.float32(v))   # PEN_ALP is REAL(SP) (invert_param.f90:66)
         elif key == "MISC SETUP:":
             body, i = section(i)
             for row in body:

@@ -1530,6 +1530,14 @@ int fz_gain_hsra(fz_ctx* c, float* gain) {
   return FZ_OK;
 }
 
+// device allocation released when it goes out of scope (early error returns included)
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return FZ_OK;
   if (*p) (void)hipFree(*p);
@@ -1765,11 +1773,19 @@ int fz_autotune(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes,
   if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
   HIPCHK(c, hipSetDevice(c->device));
   const bool prof = c->prof;
+  const int mtile0 = c->mtile, tile0 = c->tile;
+  // on a failed trial the context keeps the caller's tiles and profiling switch
+  auto fail = [&](int code) {
+    c->mtile = mtile0;
+    c->tile = tile0;
+    c->prof = prof;
+    return code;
+  };
   int r = fz_set_profiling(c, 1);
   if (r) return r;
   const int nz = c->cfg.nz, numw = c->cfg.numw;
   size_t freeb = 0, totb = 0;
-  HIPCHK(c, hipMemGetInfo(&freeb, &totb));
+  if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return fail(set_err(c, FZ_EHIP, "hipMemGetInfo failed"));
   const size_t budget = freeb / 3;
   // microphysics tile: Saha factors, EOS and derivative buffers per column
   const size_t per_mcol = sizeof(double) * (size_t)nz * (FZ_NELEM * 4 * (c->hydro ? 2 : 1) + 2 + 8);
@@ -1784,10 +1800,10 @@ int fz_autotune(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes,
     if ((size_t)std::min(m, ncol) * per_mcol > budget) break;
     c->mtile = m;
     r = fz_forward_dev(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_status, stream);
-    if (r) return r;
+    if (r) return fail(r);
     float ms = 0.f;
     r = fz_last_timing(c, &ms, nullptr, nullptr);
-    if (r) return r;
+    if (r) return fail(r);
     if (ms < best) {
       best = ms;
       best_m = m;
@@ -1801,10 +1817,10 @@ int fz_autotune(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes,
       if ((size_t)std::min(t, ncol) * per_rcol > budget) break;
       c->tile = t;
       r = fz_forward_dev(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_status, stream);
-      if (r) return r;
+      if (r) return fail(r);
       float ms = 0.f;
       r = fz_last_timing(c, nullptr, &ms, nullptr);
-      if (r) return r;
+      if (r) return fail(r);
       if (ms < best) {
         best = ms;
         best_t = t;
@@ -1853,14 +1869,15 @@ int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rh
   const float* cubes[NATM] = {tem3d, pg3d, rho3d, pel3d, mw3d, bx3d, by3d, bz3d, vx3d, vy3d, vz3d, tau3d5};
   const size_t colbytes = sizeof(float) * nz;
   std::vector<float> h(colbytes / sizeof(float) * n);
-  float *d_atm = nullptr, *d_tmp = nullptr, *d_st = nullptr, *d_jac = nullptr;
-  int32_t* d_status = nullptr;
+  DevBuf b_atm, b_tmp, b_st, b_jac, b_status;
   HIPCHK(c, hipSetDevice(c->device));
-  HIPCHK(c, hipMalloc(&d_atm, colbytes * NATM * n));
-  HIPCHK(c, hipMalloc(&d_tmp, colbytes * n));
-  HIPCHK(c, hipMalloc(&d_st, sizeof(float) * 4 * numw * n));
-  HIPCHK(c, hipMalloc(&d_status, sizeof(int32_t) * n));
-  if (np) HIPCHK(c, hipMalloc(&d_jac, sizeof(float) * (size_t)np * nz * 4 * numw * n));
+  HIPCHK(c, hipMalloc(&b_atm.p, colbytes * NATM * n));
+  HIPCHK(c, hipMalloc(&b_tmp.p, colbytes * n));
+  HIPCHK(c, hipMalloc(&b_st.p, sizeof(float) * 4 * numw * n));
+  HIPCHK(c, hipMalloc(&b_status.p, sizeof(int32_t) * n));
+  if (np) HIPCHK(c, hipMalloc(&b_jac.p, sizeof(float) * (size_t)np * nz * 4 * numw * n));
+  float *d_atm = (float*)b_atm.p, *d_tmp = (float*)b_tmp.p, *d_st = (float*)b_st.p, *d_jac = (float*)b_jac.p;
+  int32_t* d_status = (int32_t*)b_status.p;
   const unsigned tb = 256, tg = (unsigned)(((long)nz * n + tb - 1) / tb);
   for (int p = 0; p < NATM; ++p) {
     for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * nz], cubes[p] + (size_t)cols[i] * nz, colbytes);
@@ -1899,11 +1916,6 @@ int fz_forward3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rh
       }
     }
   }
-  (void)hipFree(d_atm);
-  (void)hipFree(d_tmp);
-  (void)hipFree(d_st);
-  (void)hipFree(d_status);
-  if (d_jac) (void)hipFree(d_jac);
   return rc;
 }
 
@@ -2324,13 +2336,6 @@ static int njevals(int apar, int curic, int maxiter, int cycpow, int nz) {
   if (curic == maxiter) return nz;
   return std::min(nz, (int)std::pow((double)cycpow, (double)(curic - 1)));
 }
-
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
 
 int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
                     float* d_stokes, double* d_chi2, int* iterations, void* stream) {

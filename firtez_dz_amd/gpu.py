@@ -152,7 +152,7 @@ def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.
     for name, v in (pen_typ or {}).items():
         c.pen_typ[INV_PARAMS.index(name)] = int(v)
     for name, v in (pen_alp or {}).items():
-        c.pen_alp[INV_PARAMS.index(name)] = float(v)
+        c.pen_alp[INV_PARAMS.index(name)] = float(np.float32(v))   # REAL(SP) PEN_ALP
     return c
 
 
@@ -177,7 +177,7 @@ def inversion(nodes: dict, invstk=(1, 1, 1, 1), solver=1, node_expansion=1, maxs
     for name, v in (pen_typ or {}).items():
         c.pen_typ[INV_PARAMS.index(name)] = int(v)
     for name, v in (pen_alp or {}).items():
-        c.pen_alp[INV_PARAMS.index(name)] = float(v)
+        c.pen_alp[INV_PARAMS.index(name)] = float(np.float32(v))   # REAL(SP) PEN_ALP
     c.regul, c.solver, c.node_expansion, c.maxsteps = int(regul), solver, node_expansion, maxsteps
     c.svdtol, c.inu, c.mpert = svdtol, inu, mpert
     return c

@@ -68,7 +68,7 @@ def lm_cfg(nz, numw, atmpar, nodes, invstk=(1, 1, 1, 1), solver=1, node_expansio
     c.invstk[:] = list(invstk)
     c.nodes[:] = [n if a else 0 for n, a in zip(nodes, atmpar)]
     c.pen_typ[:] = list(pen_typ)
-    c.pen_alp[:] = list(pen_alp)
+    c.pen_alp[:] = [float(np.float32(a)) for a in pen_alp]   # PEN_ALP is REAL(SP)
     c.regul, c.solver, c.node_expansion, c.curic = int(regul), solver, node_expansion, curic
     c.svdtol = svdtol
     c.inu = float(sum(invstk) * numw) if inu is None else float(inu)

@@ -7,8 +7,10 @@ process per core like the reference's MPI workers.
 TEST INFRASTRUCTURE: only bench.py's cpu_baseline leg runs it, as a child
 process (this module never touches the GPU).
 
-  python -m oracle.refbench --nz 96 --seconds 10 --procs 16
+  python -m oracle.refbench --nz 96 --seconds 10
 prints one JSON object: column-wavelengths per second over all processes.
+The process count defaults to the cores this process may run on
+(os.sched_getaffinity), at most 16 (the host share of one GPU).
 """
 from __future__ import annotations
 
@@ -56,7 +58,15 @@ def _worker(args):
     return done, time.perf_counter() - t0, numw
 
 
-def run(nz: int, seconds: float, procs: int) -> dict:
+def host_cores() -> tuple:
+    """(cores to use, cores in this process's affinity mask)."""
+    n = len(os.sched_getaffinity(0))
+    return min(n, 16), n
+
+
+def run(nz: int, seconds: float, procs: int = 0) -> dict:
+    use, avail = host_cores()
+    procs = procs or use
     ref, numw, np = _setup(nz)
     gain = ref.gain()
     del ref
@@ -67,15 +77,16 @@ def run(nz: int, seconds: float, procs: int) -> dict:
     cols = sum(d for d, _, _ in res)
     return {"value": rate, "unit": "column-λ/s", "cores": procs, "kind": "reference",
             "sample": "%d columns (nz=%d, %d wavelengths, Jacobians of 7 parameters) of seeded 4x4 cubes, the "
-                      "reference's Fortran modules (amdflang -O2) driven by the harness's FORWARD1D loop, %d "
-                      "processes x %.0f s" % (cols, nz, numw, procs, seconds)}
+                      "reference's Fortran modules (amdflang -O3) driven by the harness's FORWARD1D loop, %d "
+                      "processes x %.0f s on %d of the %d cores in the affinity mask"
+                      % (cols, nz, numw, procs, seconds, procs, avail)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--nz", type=int, default=96)
     ap.add_argument("--seconds", type=float, default=10.0)
-    ap.add_argument("--procs", type=int, default=16)
+    ap.add_argument("--procs", type=int, default=0, help="0: the cores in the affinity mask, at most 16")
     a = ap.parse_args()
     print(json.dumps(run(a.nz, a.seconds, a.procs)), flush=True)
 

@@ -27,7 +27,7 @@ def report(name, err, tol):
     assert err <= tol, name
 
 
-def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7, solver=1):
+def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7, solver=1, regul=False, pen_typ=None, pen_alp=None):
     import torch
     assert torch.cuda.is_available()
     ls = lines.fe6301_6302()
@@ -49,14 +49,19 @@ def problem(gpu_ctx, cpu, nodes, nx=4, ny=4, nz=64, seed=7, solver=1):
     guess[0] += 250.0 * np.linspace(1.0, 0.2, nz, dtype=np.float32)[:, None]
     guess[5:8] *= np.float32(0.7)
     guess[10] += np.float32(1.0e5)
-    inv = gpu.inversion(nodes, solver=solver)
+    inv = gpu.inversion(nodes, solver=solver, regul=regul, pen_typ=pen_typ, pen_alp=pen_alp)
     ifreep, nfreq = gpu_ctx.set_inversion(inv)
     atmpar = [0] * 8
     nd = [0] * 8
     for k, v in nodes.items():
         i = gpu.INV_PARAMS.index(k)
         atmpar[i], nd[i] = 1, v
-    cfg = oracle.lm_cfg(nz, numw, atmpar, nd, solver=solver)
+    pt, pa = [0] * 8, [0.0] * 8
+    for k, v in (pen_typ or {}).items():
+        pt[gpu.INV_PARAMS.index(k)] = v
+    for k, v in (pen_alp or {}).items():
+        pa[gpu.INV_PARAMS.index(k)] = v
+    cfg = oracle.lm_cfg(nz, numw, atmpar, nd, solver=solver, regul=regul, pen_typ=pt, pen_alp=pa)
     coefs = cpu.coefs_block(cfg, {i: cpu.int_coefs(nz, nd[i]) for i in range(8) if atmpar[i]})
     return dict(z=z, gain=gain, true=true, obs=obs, guess=guess, cfg=cfg, coefs=coefs, ifreep=ifreep,
                 nfreq=nfreq, numw=numw, ncol=ncol, nz=nz, ls=ls)
@@ -144,11 +149,26 @@ NODES_BIG = {"T": 40, "Bz": 16, "vz": 24}
 # iterate path and stopping iteration follow the rounding of the dot
 # products, which the device forms in a tree); big: 80 free parameters,
 # the work arrays in device memory
-@pytest.mark.parametrize("nodes,solver", [(NODES_C1, 1), (NODES_C2, 1), (NODES_C1, 2), (NODES_BIG, 1)],
-                         ids=["cycle1", "cycle2", "cg", "big"])
-def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver):
+# regularised cases (REGULARIZATION: yes, GET_IT_PENALTIES): penalty type 0
+# (difference to zero) on some parameters and 1 (difference to the next node)
+# on others, with both solvers; big-cg: the conjugate-gradient solver on 80
+# free parameters with the work arrays in device memory
+PEN_A = ({"T": 1, "Bx": 0, "By": 0, "Bz": 1, "vz": 1}, {"T": 0.1, "Bx": 0.01, "By": 0.01, "Bz": 0.1, "vz": 0.3})
+PEN_B = ({"T": 0, "Bx": 1, "By": 1, "Bz": 0, "vz": 0}, {"T": 0.3, "Bx": 0.1, "By": 0.1, "Bz": 0.01, "vz": 0.1})
+PEN_BIG = ({"T": 1, "Bz": 0, "vz": 1}, {"T": 0.1, "Bz": 0.01, "vz": 0.1})
+LM_CASES = [(NODES_C1, 1, None), (NODES_C2, 1, None), (NODES_C1, 2, None), (NODES_BIG, 1, None),
+            (NODES_C2, 1, PEN_A), (NODES_C2, 1, PEN_B), (NODES_C1, 2, PEN_A), (NODES_C2, 2, PEN_B),
+            (NODES_BIG, 2, None), (NODES_BIG, 1, PEN_BIG), (NODES_BIG, 2, PEN_BIG)]
+LM_IDS = ["cycle1", "cycle2", "cg", "big", "regul-pen1", "regul-pen0", "regul-cg-pen1", "regul-cg-pen0",
+          "big-cg", "big-regul", "big-regul-cg"]
+
+
+@pytest.mark.parametrize("nodes,solver,pen", LM_CASES, ids=LM_IDS)
+def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver, pen):
     import torch
-    P = problem(gpu_ctx, cpu, nodes, solver=solver)
+    regul = pen is not None
+    P = problem(gpu_ctx, cpu, nodes, solver=solver, regul=regul, pen_typ=pen[0] if regul else None,
+                pen_alp=pen[1] if regul else None)
     ncol, nf, npr, numw = P["ncol"], P["nfreq"], P["ifreep"], P["numw"]
     d = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dt).cuda()
     atm = d(P["guess"], torch.float32)

@@ -9,7 +9,12 @@
   * one LM step on ragged rows against the oracle run per pixel with that
     pixel's own NFREQ (its chi^2 normalisation included);
   * INVERT3D with windows against the oracle's INVERT3D run per pixel on the
-    line setup whose regions are that pixel's windows.
+    line setup whose regions are that pixel's windows;
+  * the uneven-lines workload itself: the eight-line list (Fe I 6301.5,
+    6302.5, 6173.3, 15648.5, 15652.9, 15662.0, 5576.1, Si I 10827.1), 1-4
+    lines per pixel in windows of 60-300 wavelengths, Stokes and node
+    response functions against the oracle per pixel, and the LM step and
+    INVERT3D tests above on it.
 """
 import numpy as np
 import pytest
@@ -43,10 +48,27 @@ def windows_for(ls, ncol, seed):
     return w
 
 
-def setup_cube(gpu_ctx, nx, ny, nz, seed):
+def config3_windows(ls, ncol, seed):
+    """1-4 of the eight lines per pixel, windows of 60-300 wavelengths; the
+    first pixels are given every line between them, so that each line of the
+    list is observed whatever the draw."""
+    w = lines.uneven_windows(ls, ncol, seed, nlines=(1, 4), width=(60, 300))
+    nreg = len(ls.regions)
+    for c in range(min(ncol, 2)):
+        w[c] = 0
+        for r in range(c * nreg // 2, (c + 1) * nreg // 2):
+            cnt = 60 + 60 * (r % 5)
+            w[c, r] = ((ls.regions[r].nwave - cnt) // 2, cnt)
+    return w
+
+
+LINESETS = {"three": (three_regions, windows_for), "config3": (lines.uneven_list, config3_windows)}
+
+
+def setup_cube(gpu_ctx, nx, ny, nz, seed, lineset="three"):
     import torch
     assert torch.cuda.is_available()
-    ls = three_regions()
+    ls = LINESETS[lineset][0]()
     atm, z = atmos.seeded(nx, ny, nz, seed=seed)
     gpu_ctx.configure(ls, z)
     gain = gpu_ctx.gain_hsra()
@@ -136,14 +158,15 @@ def test_windows_eqv_matches_dense(gpu_ctx):
         gpu_ctx.set_windows(None)
 
 
-def test_windows_lm_step_vs_oracle(gpu_ctx, cpu):
+@pytest.mark.parametrize("lineset", ["three", "config3"])
+def test_windows_lm_step_vs_oracle(gpu_ctx, cpu, lineset):
     """Device and oracle fed the same ragged inputs; the oracle runs each
     pixel as its own problem of NFREQ = 4*numw_pixel."""
     import torch
     nx, ny, nz = 4, 2, 64
-    ls, z, gain, truth = setup_cube(gpu_ctx, nx, ny, nz, 13)
+    ls, z, gain, truth = setup_cube(gpu_ctx, nx, ny, nz, 13, lineset)
     ncol = nx * ny
-    w = windows_for(ls, ncol, 7)
+    w = LINESETS[lineset][1](ls, ncol, 7)
     st_true, _, _ = gpu_ctx.forward_torch(truth.clone(), gpu.RF_NONE)
     rng = np.random.default_rng(3)
     guess = truth.clone()
@@ -200,27 +223,31 @@ def test_windows_lm_step_vs_oracle(gpu_ctx, cpu):
             for row in (0, 5, 6, 7, 10):
                 werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc_o[row]) /
                                                           (np.abs(mc_o[row]) + 1e-3)))
-        report("ragged LM step chi2 vs oracle per pixel (relative)", werr["chi"], 1e-11)
-        report("ragged LM step node perturbation vs oracle", werr["delta"], 1e-6)
-        report("ragged LM step new model vs oracle (relative)", werr["model"], 2e-6)
+        report("%s: ragged LM step chi2 vs oracle per pixel (relative)" % lineset, werr["chi"], 1e-11)
+        report("%s: ragged LM step node perturbation vs oracle" % lineset, werr["delta"], 1e-6)
+        report("%s: ragged LM step new model vs oracle (relative)" % lineset, werr["model"], 2e-6)
     finally:
         gpu_ctx.set_windows(None)
 
 
-def test_windows_invert3d_vs_oracle(gpu_ctx, cpu):
+@pytest.mark.parametrize("lineset", ["three", "config3"])
+def test_windows_invert3d_vs_oracle(gpu_ctx, cpu, lineset):
     """INVERT3D (one node per parameter, no smoothing) on 4 pixels of
     different windows, against the oracle's INVERT3D of each pixel alone on
     the line setup made of its windows (same GAIN1D values)."""
     import torch
     from oracle import invert
     nx, ny, nz = 2, 2, 64
-    ls, z, gain, truth = setup_cube(gpu_ctx, nx, ny, nz, 21)
+    ls, z, gain, truth = setup_cube(gpu_ctx, nx, ny, nz, 21, lineset)
     ncol = nx * ny
-    w = np.zeros((ncol, 3, 2), np.int32)
-    w[0] = [(20, 72), (0, 0), (0, 0)]
-    w[1] = [(0, 0), (10, 60), (20, 56)]
-    w[2] = [(0, 112), (0, 0), (30, 40)]
-    w[3] = [(40, 30), (5, 70), (0, 0)]
+    if lineset == "three":
+        w = np.zeros((ncol, 3, 2), np.int32)
+        w[0] = [(20, 72), (0, 0), (0, 0)]
+        w[1] = [(0, 0), (10, 60), (20, 56)]
+        w[2] = [(0, 112), (0, 0), (30, 40)]
+        w[3] = [(40, 30), (5, 70), (0, 0)]
+    else:
+        w = config3_windows(ls, ncol, 21)
     true = truth.cpu().numpy()
     rng = np.random.default_rng(21)
     obs_parts, per_pixel = [], []
@@ -263,6 +290,61 @@ def test_windows_invert3d_vs_oracle(gpu_ctx, cpu):
         for row in (0, 5, 6, 7, 10):
             scale = max(1.0, float(np.abs(true[row]).max()))
             werr["model"] = max(werr["model"], float(np.max(np.abs(g_model[row, :, c] - o_model[row, :, 0]))) / scale)
-    report("windowed INVERT3D chi2 vs oracle per pixel (relative)", werr["chi"], 0.1)
-    report("windowed INVERT3D synthesis vs oracle", werr["syn"], 5e-3)
-    report("windowed INVERT3D model vs oracle (relative to max)", werr["model"], 2e-3)
+    report("%s: windowed INVERT3D chi2 vs oracle per pixel (relative)" % lineset, werr["chi"], 0.1)
+    report("%s: windowed INVERT3D synthesis vs oracle" % lineset, werr["syn"], 5e-3)
+    report("%s: windowed INVERT3D model vs oracle (relative to max)" % lineset, werr["model"], 2e-3)
+
+
+def test_config3_eight_lines_vs_oracle(gpu_ctx, cpu):
+    """The eight-line list with 60-300-point windows: ragged Stokes and node
+    response functions (T, B, v_z at 4 nodes) of every pixel against the
+    oracle run on that pixel's own windows."""
+    import torch
+    nx, ny, nz = 3, 3, 64
+    ls, z, gain, truth = setup_cube(gpu_ctx, nx, ny, nz, 41, "config3")
+    ncol = nx * ny
+    w = config3_windows(ls, ncol, 41)
+    assert np.all((w[:, :, 1] == 0) | ((w[:, :, 1] >= 60) & (w[:, :, 1] <= 300)))
+    assert np.all(np.any(w[:, :, 1] > 0, axis=0)), "every line observed"
+    guess = truth.clone()
+    guess[0] += 150.0
+    guess[5:8] *= 0.8
+    guess[10] += 5e4
+    ifreep, _ = gpu_ctx.set_inversion(gpu.inversion(NODES))
+    atmpar, nd = [0] * 8, [0] * 8
+    for k, v in NODES.items():
+        i = gpu.INV_PARAMS.index(k)
+        atmpar[i], nd[i] = 1, v
+    off = gpu_ctx.set_windows(w)
+    try:
+        a1 = guess.clone()
+        st, _, status = gpu_ctx.forward_torch(a1, gpu.RF_NONE)
+        a2 = guess.clone()
+        st2, eqv, status2 = gpu_ctx.forward_nodes_torch(a2)
+        torch.cuda.synchronize()
+        assert int(status.max()) == 0 and int(status2.max()) == 0
+        h_st, h_st2, h_eqv, h_atm = st.cpu().numpy(), st2.cpu().numpy(), eqv.cpu().numpy(), a1.cpu().numpy()
+    finally:
+        gpu_ctx.set_windows(None)
+    assert np.array_equal(h_st, h_st2)
+    worst_s = worst_j = 0.0
+    for c in range(ncol):
+        idx = lines.window_index(ls, w[c])
+        nw = idx.size
+        lsc = lines.window_setup(ls, w[c])
+        cpu.setup(lsc, resp=True, atmpar=CART)
+        cpu.set_gain(gain[:, idx])
+        _, syn, dsyn, _ = cpu.forward1d(atmos.column_model(h_atm, z, c))
+        worst_s = max(worst_s, float(np.max(np.abs(h_st[4 * off[c]:4 * off[c + 1]].reshape(4, nw) - syn))))
+        cfg = oracle.lm_cfg(nz, nw, atmpar, nd)
+        coefs = cpu.coefs_block(cfg, {i: cpu.int_coefs(nz, nd[i]) for i in range(8) if atmpar[i]})
+        ref = cpu.eqv_dsyn(cfg, coefs, dsyn)                       # (4*nw, ifreep)
+        blk = h_eqv[ifreep * 4 * off[c]:ifreep * 4 * off[c + 1]].reshape(ifreep, 4 * nw)
+        for j in range(ifreep):
+            sc = np.abs(ref[:, j]).max()
+            if sc > 0:
+                worst_j = max(worst_j, float(np.max(np.abs(blk[j] - ref[:, j])) / sc))
+        print("[windows] config3 pixel %d: regions %s, %d wavelengths"
+              % (c, list(np.nonzero(w[c, :, 1])[0]), nw), flush=True)
+    report("config3: eight-line ragged Stokes vs oracle", worst_s, 1e-4)
+    report("config3: eight-line ragged EQVDSYN vs oracle", worst_j, 2e-3)
