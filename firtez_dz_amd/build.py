@@ -40,14 +40,25 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FLANG = "/opt/rocm/bin/amdflang"
+CALLER_SRC = os.path.join(ROOT, "tests", "fortran", "fz_caller.f90")
+CALLER = os.path.join(ROOT, "tests", "fortran", "fz_caller")
+
+
 def build_fortran(verbose: bool = False) -> str:
-    """Compile the iso_c_binding module (checks that the Fortran boundary builds)."""
-    src = os.path.join(HERE, "fortran", "firtez_gpu_mod.f90")
-    out = os.path.join(HERE, "fortran", "firtez_gpu_mod.o")
-    cmd = ["/opt/rocm/bin/amdflang", "-c", "-O2", "-fPIC", "-J", os.path.join(HERE, "fortran"), src, "-o", out]
-    if verbose:
-        print(" ".join(cmd))
-    subprocess.check_call(cmd)
+    """Compile the iso_c_binding module and link the test caller
+    (tests/fortran/fz_caller.f90) against it and libfirtez_gpu.so."""
+    mdir = os.path.join(HERE, "fortran")
+    src = os.path.join(mdir, "firtez_gpu_mod.f90")
+    out = os.path.join(mdir, "firtez_gpu_mod.o")
+    cmds = [[FLANG, "-c", "-O2", "-fPIC", "-J", mdir, src, "-o", out]]
+    if os.path.exists(CALLER_SRC):
+        cmds.append([FLANG, "-O2", "-J", mdir, CALLER_SRC, out, "-L", HERE, "-lfirtez_gpu",
+                     "-Wl,-rpath,$ORIGIN/../../firtez_dz_amd", "-o", CALLER])
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.check_call(cmd)
     return out
 
 

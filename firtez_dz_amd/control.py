@@ -23,10 +23,9 @@ SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
 CONTINUUM NORMALIZATION, COUPLED INVERSION, LINE OPTICAL DEPTH, NLTE
-DEPARTURE COEFFICIENTS, LINE SPREAD FUNCTION -- the library's fz_set_lsf
-takes LSF_KERNEL directly --, AUTO Stokes weights, the ASSIST_*
-initialisers, FULL_STOKES) raise NotImplementedError instead of being
-ignored.
+DEPARTURE COEFFICIENTS, AUTO Stokes weights, the ASSIST_* initialisers,
+FULL_STOKES) raise NotImplementedError instead of being ignored.  LINE
+SPREAD FUNCTION (Gaussian rows or LSF files) sets the library's LSF.
 """
 from __future__ import annotations
 
@@ -45,7 +44,7 @@ PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
 _UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "COUPLED INVERSION:", "LINE OPTICAL DEPTH:",
-                "NLTE DEPARTURE COEFFICIENTS:", "LINE SPREAD FUNCTION:")
+                "NLTE DEPARTURE COEFFICIENTS:")
 
 
 @dataclass
@@ -73,6 +72,9 @@ class Control:
     svdtol: float = 0.0                     # 0: the library's default (RSVDTOL 1e-4)
     solver: int = 1
     node_expansion: int = 1
+    # LINE SPREAD FUNCTION: per region (sigma, w0) in mA, or an LSF file name
+    # (DATAPATH-relative), or None for EMPTY; empty list: no LSF
+    lsf: List[object] = field(default_factory=list)
 
 
 def _number(tok: str) -> float:
@@ -201,6 +203,21 @@ def parse(text: str) -> Control:
                             c.pen_typ[name] = _nint(v)
                         else:
                             c.pen_alp[name] = float(np.float32(v))   # PEN_ALP is REAL(SP) (invert_param.f90:66)
+        elif key == "LINE SPREAD FUNCTION:":
+            # NSET_LSF (input_file.f90:1719-1865): one row per region, either
+            # "sigma w0" (Gaussian, sigma /= 0) or an LSF file name, or EMPTY
+            body, i = section(i)
+            for row in body:
+                cols = row.split()
+                if len(cols) == 2:
+                    sig, w0 = _number(cols[0]), _number(cols[1])
+                    if sig == 0:
+                        raise ValueError("LSF standard deviation of spectral region %d is 0" % (len(c.lsf) + 1))
+                    c.lsf.append((sig, w0))
+                elif len(cols) == 1:
+                    c.lsf.append(None if cols[0].upper() == "EMPTY" else cols[0])
+                else:
+                    raise ValueError('wrong input format in "LINE SPREAD FUNCTION": %r' % row)
         elif key == "MISC SETUP:":
             body, i = section(i)
             for row in body:
@@ -238,6 +255,8 @@ def parse(text: str) -> Control:
             raise ValueError('"STOKES SETUP:" is mandatory in inversion mode')
         if not c.params:
             raise ValueError('"INVERSION SETUP:" must name at least one parameter in inversion mode')
+    if c.lsf and len(c.lsf) != len(c.lines):
+        raise ValueError("the number of LSF (%d) and LINE (%d) do not match" % (len(c.lsf), len(c.lines)))
     if c.mode in ("synthesis", "inversion"):
         if not c.lines:
             raise ValueError('missing spectral lines ("LINES:")')
@@ -249,6 +268,23 @@ def parse(text: str) -> Control:
 def parse_file(path: str) -> Control:
     with open(path) as fh:
         return parse(fh.read())
+
+
+def lsf_arrays(c: Control, ls: _lines.LineSetup, base: str = "."):
+    """LSF_KERNEL and LSF_VALID of LSF_INIT (pre_post_duties.f90:770-898)
+    from the LINE SPREAD FUNCTION rows: Gaussians where (sigma, w0) is
+    given, the samples of DATAPATH/<file> (READ_LSF_FILE) where a file is
+    named, nothing (an invalid region) for EMPTY."""
+    sig = [e[0] if isinstance(e, tuple) else -1.0 for e in c.lsf]
+    w0 = [e[1] if isinstance(e, tuple) else -1.0 for e in c.lsf]
+    taps = {}
+    for r, e in enumerate(c.lsf):
+        if isinstance(e, str):
+            path = os.path.join(base, c.datapath, e)
+            if not os.path.exists(path):
+                raise FileNotFoundError("cannot find LSF file %s in directory %s" % (e, c.datapath))
+            taps[r] = modelio.read_lsf(path, ls.regions[r].nwave)
+    return _lines.lsf_kernel(ls, sig, w0, taps)
 
 
 def line_setup(c: Control) -> _lines.LineSetup:
@@ -297,6 +333,8 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
         ls = line_setup(c)
         ctx.configure(ls, z)
         ctx.gain_hsra()
+        if c.lsf:
+            ctx.set_lsf(*lsf_arrays(c, ls, workdir))
         ctx.set_hydrostatic(c.hydrostatic)
         if c.hydrostatic and not c.hydro_ders and c.mode == "inversion":
             raise NotImplementedError("HYDROSTATIC DERIVATIVES: no is not supported by this path")

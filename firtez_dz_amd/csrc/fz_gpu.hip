@@ -482,7 +482,7 @@ struct RtArgs {
   const float* atm;
   const float* micro;
   const float* gain; // [numw][4]
-  const int32_t* status;
+  int32_t* status;
   float* stokes;     // [ncol][4][numw]
   float* jac;        // [ncol - jcol0][np][nz][4][numw]
   int jcol0;         // column of jac[0]
@@ -687,9 +687,15 @@ __device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
 // O(nz) instead of the reference's O(nz^2).  In that mode the reference
 // evaluates the rho, B and v responses with the layer's outgoing Stokes
 // vector (STKIN is reset to SYN1D after the hydrostatic block); reproduced.
-template <int RF, bool HE>
+//
+// PACK (one region, no per-pixel windows): the lanes of a launch run over
+// the flattened (column, wavelength) pairs of the tile, 64 to a wave, so no
+// lane idles when the wavelength count is not a multiple of 64 (112 = 64 +
+// 48 would leave 16 of 128 lanes idle per column); a wave then spans up to
+// two columns and reads their per-height quantities with vector loads.
+template <int RF, bool HE, bool PACK>
 __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ atm, const float* __restrict__ micro,
-                                                     const float* __restrict__ gain, const int32_t* __restrict__ status,
+                                                     const float* __restrict__ gain, int32_t* __restrict__ status,
                                                      float* __restrict__ stokes, float* __restrict__ jac,
                                                      float* __restrict__ scratch, int ncol, int col0, int ntile,
                                                      int jcol0, const RtItem* __restrict__ items,
@@ -720,7 +726,19 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   int snumw, lo;
   long t, nthr;      // lane slot and lane count of the tile (scratch rows)
   long jbase;        // the column's response functions: jac[jbase + ((p*nz + k)*4 + s)*snumw]
-  if (items) {
+  if constexpr (PACK) {
+    const int g = (int)blockIdx.x * RT_WAVE + (int)threadIdx.x;
+    if (g >= a.ntile * numw) return;
+    ct = g / numw;
+    l = g - ct * numw;
+    r = 0;
+    sbase = (long)(col0 + ct) * 4 * numw;
+    snumw = numw;
+    lo = l;
+    t = g;
+    nthr = (long)a.ntile * numw;
+    jbase = (long)(col0 + ct - a.jcol0) * NP * nz * 4 * numw + l;
+  } else if (items) {
     // per-pixel windows: scratch and response functions are ragged too
     const RtItem it = items[blockIdx.x];
     if ((int)threadIdx.x >= it.n) return;
@@ -814,6 +832,12 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
     gn[s] = a.gain[l * 4 + s];
     a.stokes[sbase + (long)s * snumw + lo] = I[s] * gn[s];
   }
+  // a non-finite Stokes vector comes from a non-finite profile, propagation
+  // matrix or evolution operator, where the reference stops
+  // (voigt.f:91-95, absorption_matrix.f90:1882-1889, rtesolver.f90:346-352)
+  if (a.status && !(__builtin_isfinite(I[0]) && __builtin_isfinite(I[1]) && __builtin_isfinite(I[2]) &&
+                    __builtin_isfinite(I[3])))
+    atomicMax(&a.status[col], (int32_t)ERR_NAN);
   if constexpr (RF != 0) {
     float acc[4][4];
 #pragma unroll
@@ -1026,6 +1050,7 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 }  // namespace fz
 
 #include "fz_lm.hpp"
+#include "fz_pert.hpp"
 #include "fz_invert.hpp"
 #include "fz_mlp.hpp"
 
@@ -1179,6 +1204,12 @@ struct fz_ctx {
   int64_t* d_poff = nullptr;
   int32_t* d_win = nullptr;         // [ncol][nreg][3]: first, count, position in the pixel's spectrum
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
+  // GET_PERTURBATION / HESSIAN: normal matrix and solver work arrays, and
+  // the device copies of the host-array entry points' inputs
+  float* d_pert = nullptr;
+  size_t pert_cap = 0;
+  float* d_pstage = nullptr;
+  size_t pstage_cap = 0;
   MlpState mlp;
 };
 
@@ -1216,6 +1247,14 @@ static int upload_cfg(fz_ctx* c) {
   if (c->device < 64) g_owner[c->device] = c;
   return FZ_OK;
 }
+
+// device allocation released when it goes out of scope (early error returns included)
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
 
 extern "C" {
 
@@ -1261,6 +1300,8 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_poff) (void)hipFree(c->d_poff);
   if (c->d_win) (void)hipFree(c->d_win);
   if (c->d_lsf) (void)hipFree(c->d_lsf);
+  if (c->d_pert) (void)hipFree(c->d_pert);
+  if (c->d_pstage) (void)hipFree(c->d_pstage);
   c->mlp.release();
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -1514,29 +1555,23 @@ int fz_gain_hsra(fz_ctx* c, float* gain) {
   HIPCHK(c, hipSetDevice(c->device));
   int ru = upload_cfg(c);
   if (ru) return ru;
-  double* kc = nullptr;
-  int32_t* derr = nullptr;
-  HIPCHK(c, hipMalloc(&kc, sizeof(double) * FZ_HSRA_LEN * c->cfg.nreg));
-  HIPCHK(c, hipMalloc(&derr, sizeof(int32_t)));
+  DevBuf b_kc, b_err;
+  HIPCHK(c, hipMalloc(&b_kc.p, sizeof(double) * FZ_HSRA_LEN * c->cfg.nreg));
+  HIPCHK(c, hipMalloc(&b_err.p, sizeof(int32_t)));
+  double* kc = (double*)b_kc.p;
+  int32_t* derr = (int32_t*)b_err.p;
   hipLaunchKernelGGL(hsra_kernel, dim3(1), dim3(64), 0, 0, kc, derr);
-  hipLaunchKernelGGL(gain_kernel, dim3((c->cfg.numw + 63) / 64), dim3(64), 0, 0, kc, c->d_gain);
   int32_t e = 0;
   HIPCHK(c, hipMemcpy(&e, derr, sizeof(int32_t), hipMemcpyDeviceToHost));
-  (void)hipFree(kc);
-  (void)hipFree(derr);
-  if (e) return set_err(c, e, "HSRA equation of state failed");
+  if (e == ERR_LAMBDA) return set_err(c, e, "HSRA normalisation: wavelength below 1800 A");
+  if (e) return set_err(c, e, "HSRA normalisation: equation of state or opacity failed");
+  hipLaunchKernelGGL(gain_kernel, dim3((c->cfg.numw + 63) / 64), dim3(64), 0, 0, kc, c->d_gain);
+  HIPCHK(c, hipGetLastError());
   if (gain) HIPCHK(c, hipMemcpy(gain, c->d_gain, sizeof(float) * 4 * c->cfg.numw, hipMemcpyDeviceToHost));
   c->have_gain = true;
   return FZ_OK;
 }
 
-// device allocation released when it goes out of scope (early error returns included)
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-};
 
 static int ensure(fz_ctx* c, float** p, size_t* cap, size_t bytes) {
   if (*cap >= bytes) return FZ_OK;
@@ -1565,25 +1600,38 @@ static int rf_nparam(int rf_mode) {
 }
 
 static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, int nitems = 0) {
-  const dim3 grid(a.items ? (unsigned)nitems : (unsigned)a.ntile * (unsigned)c->cfg.nbpc), blk(RT_WAVE);
   if (a.items && nitems == 0) return FZ_OK;
+  // packed lanes for one region without windows (the HYDROSTATIC and P0
+  // instances keep one region-uniform wave per 64 wavelengths)
+  const bool pack = !a.items && c->cfg.nreg == 1 && !c->hydro && rf_mode != FZ_RF_CARTESIAN_P0;
+  const unsigned npack = (unsigned)(((long)a.ntile * c->cfg.numw + RT_WAVE - 1) / RT_WAVE);
+  const dim3 grid(a.items ? (unsigned)nitems : pack ? npack : (unsigned)a.ntile * (unsigned)c->cfg.nbpc),
+      blk(RT_WAVE);
 #define FZ_RT_ARGS                                                                                               \
   a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0, a.items, a.poff
-  if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  if (pack) {
+    if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
+    else if (rf_mode == FZ_RF_SPHERICAL)
+      hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
+    else if (rf_mode == RF_CART_TBV)
+      hipLaunchKernelGGL((rt_kernel<RF_CART_TBV, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
+    else hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
+  }
+  else if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro && rf_mode == FZ_RF_SPHERICAL)
-    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false>), grid, blk, 0, st, FZ_RT_ARGS);
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, false, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (!c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
-    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, false>), grid, blk, 0, st, FZ_RT_ARGS);
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, false, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (rf_mode == RF_CART_TBV) {
     if (c->hydro) return set_err(c, FZ_ECONFIG, "reduced response set without HYDROSTATIC only");
-    hipLaunchKernelGGL((rt_kernel<RF_CART_TBV, false>), grid, blk, 0, st, FZ_RT_ARGS);
+    hipLaunchKernelGGL((rt_kernel<RF_CART_TBV, false, false>), grid, blk, 0, st, FZ_RT_ARGS);
   }
   else if (c->hydro && rf_mode == FZ_RF_CARTESIAN_P0)
-    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, true>), grid, blk, 0, st, FZ_RT_ARGS);
-  else if (!c->hydro) hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false>), grid, blk, 0, st, FZ_RT_ARGS);
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN_P0, true, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else if (!c->hydro) hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, false, false>), grid, blk, 0, st, FZ_RT_ARGS);
   else if (rf_mode == FZ_RF_SPHERICAL)
-    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, true>), grid, blk, 0, st, FZ_RT_ARGS);
-  else hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, true>), grid, blk, 0, st, FZ_RT_ARGS);
+    hipLaunchKernelGGL((rt_kernel<FZ_RF_SPHERICAL, true, false>), grid, blk, 0, st, FZ_RT_ARGS);
+  else hipLaunchKernelGGL((rt_kernel<FZ_RF_CARTESIAN, true, false>), grid, blk, 0, st, FZ_RT_ARGS);
 #undef FZ_RT_ARGS
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
@@ -2297,6 +2345,264 @@ int fz_lm_step_dev(fz_ctx* c, int ncol, const double* d_obs, const float* d_stok
   a.pix_of = nullptr;
   set_lm_windows(c, a);
   return launch_lm(c, a, ncol, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ GET_PERTURBATION / HESSIAN
+
+// doubles of device memory that run_pert needs beyond its inputs and outputs:
+// the normal matrix with the gradient column, and the solver's work arrays
+// when they do not fit the LDS
+static size_t pert_scratch_doubles(int np) {
+  size_t n = (size_t)np * (np + 1);
+  if (np > LM_MAXNP) n += pert_work_doubles(np, LM_BIG_THREADS);
+  return (n + 31) / 32 * 32;
+}
+
+static int pert_check(fz_ctx* c, int np, int nf, int dtype) {
+  if (np < 1 || nf < 1) return set_err(c, FZ_EARG, "GET_PERTURBATION: NP and NF must be positive");
+  if (np > LM_MAXNP_BIG) return set_err(c, FZ_EARG, "GET_PERTURBATION: more than 2048 parameters");
+  // lm.f90:194-201: any other DTYPE stops the reference
+  if (dtype != 1 && dtype != 2) return set_err(c, FZ_EARG, "LM does not support DTYPE " + std::to_string(dtype));
+  return FZ_OK;
+}
+
+// the normal matrix of X (np columns at x[p*sp + f*sf]) and, with y, the
+// gradient X^T y into gram[np][np+1] (upper triangle)
+static int launch_gram(fz_ctx* c, int np, int nf, const double* x, long sp, long sf, const double* y, double* gram,
+                       hipStream_t st) {
+  GramArgs g;
+  g.np = np;
+  g.nf = nf;
+  g.x = x;
+  g.sp = sp;
+  g.sf = sf;
+  g.y = y;
+  g.gram = gram;
+  const unsigned nti = (unsigned)((np + GR_TILE - 1) / GR_TILE), ntj = (unsigned)((np + 1 + GR_TILE - 1) / GR_TILE);
+  hipLaunchKernelGGL(gram_kernel, dim3(ntj, nti), dim3(GR_THREADS), 0, st, g);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+// GET_PERTURBATION on device arrays with the caller's scratch (at least
+// pert_scratch_doubles(np) doubles)
+static int run_pert(fz_ctx* c, int np, int nf, const double* d_ijac, const double* d_diff, double lambda,
+                    double* d_delta, const double* d_pen, const double* d_hss, int dtype, double svdtol,
+                    double* scratch, hipStream_t st) {
+  int r = launch_gram(c, np, nf, d_ijac, nf, 1, d_diff, scratch, st);
+  if (r) return r;
+  PertArgs a;
+  a.np = np;
+  a.dtype = dtype;
+  a.lambda = lambda;
+  a.svdtol = svdtol;
+  a.gram = scratch;
+  a.reg_pen = d_pen;
+  a.reg_hss = d_hss;
+  a.delta = d_delta;
+  if (np <= LM_MAXNP) {
+    const size_t lds = pert_work_doubles(np, LM_THREADS) * sizeof(double);
+    HIPCHK(c, hipFuncSetAttribute((const void*)pert_solve_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds));
+    a.work = nullptr;
+    hipLaunchKernelGGL(pert_solve_kernel<false>, dim3(1), dim3(LM_THREADS), lds, st, a);
+  } else {
+    a.work = scratch + (size_t)np * (np + 1);
+    hipLaunchKernelGGL(pert_solve_kernel<true>, dim3(1), dim3(LM_BIG_THREADS), 0, st, a);
+  }
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+extern "C" {
+
+int fz_get_perturbation_dev(fz_ctx* c, int np, int nf, const double* d_ijacobian, const double* d_diff,
+                            double lambdap, double* d_delta, const double* d_reg_pen, const double* d_reg_hss,
+                            int dtype, double svdtol, void* stream) {
+  if (!c || !d_ijacobian || !d_diff || !d_delta) return FZ_EARG;
+  int r = pert_check(c, np, nf, dtype);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  r = ensure(c, &c->d_pert, &c->pert_cap, sizeof(double) * pert_scratch_doubles(np));
+  if (r) return r;
+  return run_pert(c, np, nf, d_ijacobian, d_diff, lambdap, d_delta, d_reg_pen, d_reg_hss, dtype, svdtol,
+                  (double*)c->d_pert, (hipStream_t)stream);
+}
+
+int fz_get_perturbation(fz_ctx* c, int np, int nf, const double* ijacobian, const double* diff, double lambdap,
+                        double* delta, const double* reg_pen, const double* reg_hss, int dtype, double svdtol) {
+  if (!c || !ijacobian || !diff || !delta) return FZ_EARG;
+  int r = pert_check(c, np, nf, dtype);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nj = (size_t)np * nf, nh = reg_hss ? (size_t)np * np : 0, npn = reg_pen ? (size_t)np : 0;
+  r = ensure(c, &c->d_pstage, &c->pstage_cap, sizeof(double) * (nj + nf + npn + nh + np));
+  if (r) return r;
+  r = ensure(c, &c->d_pert, &c->pert_cap, sizeof(double) * pert_scratch_doubles(np));
+  if (r) return r;
+  double* dj = (double*)c->d_pstage;
+  double* dd = dj + nj;
+  double* dp = dd + nf;
+  double* dh = dp + npn;
+  double* dl = dh + nh;
+  HIPCHK(c, hipMemcpy(dj, ijacobian, sizeof(double) * nj, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(dd, diff, sizeof(double) * nf, hipMemcpyHostToDevice));
+  if (npn) HIPCHK(c, hipMemcpy(dp, reg_pen, sizeof(double) * npn, hipMemcpyHostToDevice));
+  if (nh) HIPCHK(c, hipMemcpy(dh, reg_hss, sizeof(double) * nh, hipMemcpyHostToDevice));
+  r = run_pert(c, np, nf, dj, dd, lambdap, dl, npn ? dp : nullptr, nh ? dh : nullptr, dtype, svdtol,
+               (double*)c->d_pert, nullptr);
+  if (r) return r;
+  HIPCHK(c, hipMemcpy(delta, dl, sizeof(double) * np, hipMemcpyDeviceToHost));
+  return FZ_OK;
+}
+
+int fz_hessian(fz_ctx* c, int np, int nf, const double* jacobian, double* hss) {
+  if (!c || !jacobian || !hss) return FZ_EARG;
+  if (np < 1 || nf < 1 || np > LM_MAXNP_BIG) return set_err(c, FZ_EARG, "HESSIAN: NP in 1..2048 and NF >= 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t nj = (size_t)np * nf, ng = (size_t)np * (np + 1);
+  int r = ensure(c, &c->d_pstage, &c->pstage_cap, sizeof(double) * (nj + ng));
+  if (r) return r;
+  double* dj = (double*)c->d_pstage;
+  double* dg = dj + nj;
+  HIPCHK(c, hipMemcpy(dj, jacobian, sizeof(double) * nj, hipMemcpyHostToDevice));
+  // JACOBIAN(NP,NF): parameter p of spectral row f at jacobian[f*np + p]
+  r = launch_gram(c, np, nf, dj, 1, np, nullptr, dg, nullptr);
+  if (r) return r;
+  std::vector<double> g(ng);
+  HIPCHK(c, hipMemcpy(g.data(), dg, sizeof(double) * ng, hipMemcpyDeviceToHost));
+  for (int j = 0; j < np; ++j)
+    for (int i = 0; i < np; ++i) hss[(size_t)j * np + i] = g[(size_t)std::min(i, j) * (np + 1) + std::max(i, j)];
+  return FZ_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int fz_get_dmodel3ds(fz_ctx* c, int nx, int ny, const double* obs3d, const double* syn3d, double* best_syn,
+                     double* inv_mas, int ld_inv_mas, const float* dsyn, float* best_dsyn, const double* isigmap3d,
+                     float* const* model, float* const* best_model, int32_t* am_i_done) {
+  if (!c || nx < 1 || ny < 1 || !obs3d || !syn3d || !best_syn || !inv_mas || ld_inv_mas < 9 || !dsyn ||
+      !best_dsyn || !isigmap3d || !model || !best_model || !am_i_done)
+    return FZ_EARG;
+  for (int p = 0; p < NATM; ++p)
+    if (!model[p] || !best_model[p]) return FZ_EARG;
+  if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  if (c->win_on) return set_err(c, FZ_EARG, "per-pixel windows: use fz_lm_step_dev");
+  const int nz = c->cfg.nz, nf = c->lm.nfreq, np = c->lm.np;
+  // pixels with AM_I_DONE == 0, in the reference's (y fastest) order
+  std::vector<int> cols;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      if (am_i_done[y + (size_t)ny * x] == 0) cols.push_back(y + ny * x);
+  const int n = (int)cols.size();
+  if (n == 0) return FZ_OK;
+  // one weight vector when ISIGMAP3D is the same in every pixel
+  bool shared = true;
+  for (int i = 1; i < n && shared; ++i)
+    shared = memcmp(isigmap3d + (size_t)cols[i] * nf, isigmap3d + (size_t)cols[0] * nf, sizeof(double) * nf) == 0;
+  const size_t nfn = (size_t)nf * n, ne = (size_t)np * nf;
+  DevBuf b_obs, b_syn, b_bsyn, b_sla, b_eqv, b_eqvb, b_isig, b_atm, b_best, b_tmp, b_done;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMalloc(&b_obs.p, sizeof(double) * nfn));
+  HIPCHK(c, hipMalloc(&b_syn.p, sizeof(double) * nfn));
+  HIPCHK(c, hipMalloc(&b_bsyn.p, sizeof(double) * nfn));
+  HIPCHK(c, hipMalloc(&b_sla.p, sizeof(double) * 9 * n));
+  HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * ne * n));
+  HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * ne * n));
+  HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * (shared ? (size_t)nf : nfn)));
+  HIPCHK(c, hipMalloc(&b_atm.p, sizeof(float) * NATM * nz * n));
+  HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * NATM * nz * n));
+  HIPCHK(c, hipMalloc(&b_tmp.p, sizeof(float) * nz * n));
+  HIPCHK(c, hipMalloc(&b_done.p, sizeof(int32_t) * n));
+  // gather the active pixels' rows (GET_DMODEL3DS sends the same arrays per pixel)
+  auto gather_d = [&](const double* src, size_t len, void* dst) -> int {
+    std::vector<double> h(len * n);
+    for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * len], src + (size_t)cols[i] * len, sizeof(double) * len);
+    HIPCHK(c, hipMemcpy(dst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    return FZ_OK;
+  };
+  auto gather_f = [&](const float* src, size_t len, void* dst) -> int {
+    std::vector<float> h(len * n);
+    for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * len], src + (size_t)cols[i] * len, sizeof(float) * len);
+    HIPCHK(c, hipMemcpy(dst, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+    return FZ_OK;
+  };
+  int r;
+  if ((r = gather_d(obs3d, nf, b_obs.p)) || (r = gather_d(syn3d, nf, b_syn.p)) ||
+      (r = gather_d(best_syn, nf, b_bsyn.p)) || (r = gather_f(dsyn, ne, b_eqv.p)) ||
+      (r = gather_f(best_dsyn, ne, b_eqvb.p)))
+    return r;
+  if (shared) HIPCHK(c, hipMemcpy(b_isig.p, isigmap3d + (size_t)cols[0] * nf, sizeof(double) * nf, hipMemcpyHostToDevice));
+  else if ((r = gather_d(isigmap3d, nf, b_isig.p))) return r;
+  {
+    std::vector<double> h((size_t)9 * n);
+    for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * 9], inv_mas + (size_t)cols[i] * ld_inv_mas, sizeof(double) * 9);
+    HIPCHK(c, hipMemcpy(b_sla.p, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  }
+  HIPCHK(c, hipMemset(b_done.p, 0, sizeof(int32_t) * n));
+  const unsigned tb = 256, tg = (unsigned)(((long)nz * n + tb - 1) / tb);
+  const size_t colbytes = sizeof(float) * nz;
+  std::vector<float> h((size_t)nz * n);
+  float* d_tmp = (float*)b_tmp.p;
+  for (int which = 0; which < 2; ++which) {
+    float* const* cubes = which ? best_model : model;
+    float* d = (float*)(which ? b_best.p : b_atm.p);
+    for (int p = 0; p < NATM; ++p) {
+      for (int i = 0; i < n; ++i) memcpy(&h[(size_t)i * nz], cubes[p] + (size_t)cols[i] * nz, colbytes);
+      HIPCHK(c, hipMemcpy(d_tmp, h.data(), colbytes * n, hipMemcpyHostToDevice));
+      hipLaunchKernelGGL(cube_to_soa, dim3(tg), dim3(tb), 0, 0, d_tmp, d + (size_t)p * nz * n, nz, n);
+    }
+  }
+  LmArgs a = c->lm;
+  a.ncol = n;
+  a.obs = (const double*)b_obs.p;
+  a.syn_d = (const double*)b_syn.p;
+  a.stokes = (const float*)b_syn.p;   // not read: the FP64 synthesis replaces it
+  a.best_syn = (double*)b_bsyn.p;
+  a.sla = (double*)b_sla.p;
+  a.eqv = (const float*)b_eqv.p;
+  a.eqv_best = (float*)b_eqvb.p;
+  a.isigmap = (const double*)b_isig.p;
+  a.isig_stride = shared ? 0 : nf;
+  a.atm = (float*)b_atm.p;
+  a.atm_best = (float*)b_best.p;
+  a.done = (int32_t*)b_done.p;
+  a.delta = nullptr;
+  a.pix_of = nullptr;
+  set_lm_windows(c, a);
+  r = launch_lm(c, a, n, nullptr);
+  if (r) return r;
+  HIPCHK(c, hipDeviceSynchronize());
+  // scatter back: models, INV_MAS(1:9), BEST_DSYN, BEST_SYN, AM_I_DONE
+  for (int which = 0; which < 2; ++which) {
+    float* const* cubes = which ? best_model : model;
+    const float* d = (const float*)(which ? b_best.p : b_atm.p);
+    for (int p = 0; p < NATM; ++p) {
+      hipLaunchKernelGGL(soa_to_cube, dim3(tg), dim3(tb), 0, 0, d + (size_t)p * nz * n, d_tmp, nz, n);
+      HIPCHK(c, hipMemcpy(h.data(), d_tmp, colbytes * n, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; ++i) memcpy(cubes[p] + (size_t)cols[i] * nz, &h[(size_t)i * nz], colbytes);
+    }
+  }
+  {
+    std::vector<double> hs((size_t)9 * n), hb(nfn);
+    HIPCHK(c, hipMemcpy(hs.data(), b_sla.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hb.data(), b_bsyn.p, sizeof(double) * nfn, hipMemcpyDeviceToHost));
+    std::vector<int32_t> hd(n);
+    HIPCHK(c, hipMemcpy(hd.data(), b_done.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      memcpy(inv_mas + (size_t)cols[i] * ld_inv_mas, &hs[(size_t)i * 9], sizeof(double) * 9);
+      memcpy(best_syn + (size_t)cols[i] * nf, &hb[(size_t)i * nf], sizeof(double) * nf);
+      if (hd[i]) am_i_done[cols[i]] = 1;
+    }
+    std::vector<float> he(ne * n);
+    HIPCHK(c, hipMemcpy(he.data(), b_eqvb.p, sizeof(float) * he.size(), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) memcpy(best_dsyn + (size_t)cols[i] * ne, &he[(size_t)i * ne], sizeof(float) * ne);
+  }
+  return FZ_OK;
 }
 
 // fz_lm_step_dev over the na active pixels pix_of[0..na): stokes and eqv are

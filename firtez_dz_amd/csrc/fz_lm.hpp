@@ -49,11 +49,15 @@ struct LmArgs {
   // per pixel
   const double* obs;           // [ncol][nfreq]
   const float* stokes;         // [ncol][4][numw] current synthesis (times GAIN1D)
+  const double* syn_d;
   double* best_syn;            // [ncol][nfreq]
   double* sla;                 // [ncol][9]
   const float* eqv;            // [ncol][np][nfreq] current equivalent response functions
   float* eqv_best;             // [ncol][np][nfreq]
-  const double* isigmap;       // [nfreq]
+  const double* isigmap;       // [nfreq], or [ncol][nfreq] with isig_stride = nfreq
+  long isig_stride;            // 0: one weight vector for every pixel (ISIGMAP3D equal over the map)
+  // synthesis of the inverted Stokes parameters in FP64, [ncol][nfreq] like
+  // obs (SYN3D of the host-array entry point), or null: the float Stokes rows
   float* atm;                  // [12][nz][ncol] current model (in: model of eqv; out: new trial)
   float* atm_best;             // [12][nz][ncol]
   int32_t* done;               // [ncol] AM_I_DONE
@@ -250,6 +254,191 @@ __device__ void lm_penalties(const LmArgs& a, const float* mod, double* pen_res,
   }
 }
 
+// Damped solve of GET_PERTURBATION (lm.f90:137-204) on a symmetric matrix
+// A[np][ld] (ld = np + 1; rows up to np + (np & 1) zero-padded) and the
+// right-hand side g, by all NT threads of the workgroup; V must hold the
+// identity.  solver 1: parallel cyclic Jacobi (tournament order) and the
+// truncated pseudo-inverse of GET_INVERSE (lm.f90:61-133: values below
+// svdtol * max dropped); solver 2: the CG iteration of
+// NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453).  Work vectors: dv[np],
+// red[NT], rot[np + 1], nod[4 np].  Result in del[np]; A and V are destroyed.
+template <int NT>
+__device__ void damped_solve(int solver, double svdtol, int np, double* A, double* V, const double* g,
+                             double* dv, double* del, double* red, double* rot, double* nod) {
+  const int t = threadIdx.x;
+  const int ld = np + 1;
+  const int npe = np + (np & 1);
+  if (solver == 1) {
+    const int half = npe / 2;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+      double o = 0.0, tt = 0.0;
+      for (int e = t; e < np * np; e += NT) {
+        const int i = e / np, j = e - i * np;
+        const double x = A[i * ld + j];
+        tt += x * x;
+        if (i != j) o += x * x;
+      }
+      const double off = block_sum<NT>(o, red), tot = block_sum<NT>(tt, red);
+      if (off <= 1e-30 * tot || off == 0.0) break;
+      for (int rnd = 0; rnd < npe - 1; ++rnd) {
+        // pair i: players L[i], L[npe-1-i] with L = [0, (rnd+k)%(npe-1)+1 ...]
+        for (int i = t; i < half; i += NT) {
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          double cs = 1.0, sn = 0.0;
+          if (q < np) {
+            const double apq = A[p * ld + q];
+            if (apq != 0.0) {
+              const double th = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
+              const double tq = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
+              cs = 1.0 / sqrt(1.0 + tq * tq);
+              sn = tq * cs;
+            }
+          }
+          rot[2 * i] = cs;
+          rot[2 * i + 1] = sn;
+        }
+        __syncthreads();
+        // columns of A and V: A <- A J, V <- V J
+        for (int e = t; e < half * npe; e += NT) {
+          const int i = e / npe, k = e - i * npe;
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          const double cs = rot[2 * i], sn = rot[2 * i + 1];
+          if (sn == 0.0 && cs == 1.0) continue;
+          const double akp = A[k * ld + p], akq = A[k * ld + q];
+          A[k * ld + p] = cs * akp - sn * akq;
+          A[k * ld + q] = sn * akp + cs * akq;
+          const double vkp = V[k * ld + p], vkq = V[k * ld + q];
+          V[k * ld + p] = cs * vkp - sn * vkq;
+          V[k * ld + q] = sn * vkp + cs * vkq;
+        }
+        __syncthreads();
+        // rows of A: A <- J^T A
+        for (int e = t; e < half * npe; e += NT) {
+          const int i = e / npe, k = e - i * npe;
+          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
+          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
+          const int p = min(p0, q0), q = max(p0, q0);
+          const double cs = rot[2 * i], sn = rot[2 * i + 1];
+          if (sn == 0.0 && cs == 1.0) continue;
+          const double apk = A[p * ld + k], aqk = A[q * ld + k];
+          A[p * ld + k] = cs * apk - sn * aqk;
+          A[q * ld + k] = sn * apk + cs * aqk;
+        }
+        __syncthreads();
+      }
+    }
+    // truncated pseudo-inverse: keep |d| >= svdtol * max|d|
+    double mx = 0.0;
+    for (int i = 0; i < np; ++i) mx = fmax(mx, fabs(A[i * ld + i]));
+    const double thr = mx * (svdtol > 0 ? svdtol : 10.0e-4);
+    for (int e = t; e < np; e += NT) {
+      const double d = A[e * ld + e];
+      double pr = 0.0;
+      if (!(fabs(d) < thr)) {
+        for (int i = 0; i < np; ++i) pr += V[i * ld + e] * g[i];
+        pr /= d;
+      }
+      dv[e] = pr;
+    }
+    __syncthreads();
+    for (int i = t; i < np; i += NT) {
+      double s = 0.0;
+      for (int e = 0; e < np; ++e) s += V[i * ld + e] * dv[e];
+      del[i] = s;
+    }
+  } else {
+    // NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453): the matrix-vector and
+    // dot products spread over the workgroup; every thread holds the same
+    // scalars, so the iteration and its stopping test are uniform
+    double* xv = nod;
+    double* pv = nod + np;
+    double* rv = nod + 2 * np;
+    double* qv = nod + 3 * np;
+    auto dot = [&](const double* x, const double* y) {
+      double s = 0.0;
+      for (int i = t; i < np; i += NT) s += x[i] * y[i];
+      return block_sum<NT>(s, red);
+    };
+    // A is exactly symmetric here (its lower triangle mirrored, a symmetric
+    // penalty Hessian, the diagonal scaled): row i is read as column i, so
+    // the lanes of a wave read consecutive entries
+    auto matvec = [&](const double* x, double* y) {
+      for (int i = t; i < np; i += NT) {
+        double s = 0.0;
+        for (int j = 0; j < np; ++j) s += A[j * ld + i] * x[j];
+        y[i] = s;
+      }
+      __syncthreads();
+    };
+    for (int i = t; i < np; i += NT) {
+      xv[i] = 0.0;
+      pv[i] = g[i];
+      rv[i] = g[i];   // bet - alp x with x = 0
+    }
+    __syncthreads();
+    double crho = dot(rv, rv);
+    matvec(pv, qv);
+    double calpha = crho / dot(pv, qv);
+    for (int i = t; i < np; i += NT) {
+      xv[i] = xv[i] + calpha * pv[i];
+      rv[i] = rv[i] - calpha * qv[i];
+    }
+    __syncthreads();
+    double pcond = 1.0e29, dcond = -1.0e29;
+    bool conv = false;
+    // each iteration forms the next direction before its stopping test, so
+    // one pass over A gives both A x (the test) and A p (the next step);
+    // each of the two sums keeps the order of a separate product
+    {
+      const double crho0 = crho;
+      crho = dot(rv, rv);
+      const double cbeta = crho / crho0;
+      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
+      __syncthreads();
+      matvec(pv, qv);
+    }
+    for (int it = 1; it <= 100000; ++it) {
+      calpha = crho / dot(pv, qv);
+      for (int i = t; i < np; i += NT) {
+        xv[i] = xv[i] + calpha * pv[i];
+        rv[i] = rv[i] - calpha * qv[i];
+      }
+      __syncthreads();
+      const double crho0 = crho;
+      crho = dot(rv, rv);
+      const double cbeta = crho / crho0;
+      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
+      __syncthreads();
+      double cs = 0.0;
+      for (int i = t; i < np; i += NT) {
+        double s = 0.0, sq = 0.0;
+        for (int j = 0; j < np; ++j) {
+          const double aij = A[j * ld + i];
+          s += aij * xv[j];
+          sq += aij * pv[j];
+        }
+        qv[i] = sq;
+        const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
+        cs += e * e;
+      }
+      const double cond = block_sum<NT>(cs, red) / (double)np;
+      if (cond < 1.0e-3 || dcond > 1.0e30) {
+        for (int i = t; i < np; i += NT) del[i] = xv[i];
+        conv = true;
+        break;
+      }
+      dcond = cond - pcond;
+      pcond = cond;
+    }
+    if (!conv)
+      for (int i = t; i < np; i += NT) del[i] = xv[i] / 10.0;
+  }
+}
+
 template <bool BIG>
 __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(LmArgs a) {
   constexpr int NT = BIG ? LM_BIG_THREADS : LM_THREADS;
@@ -274,8 +463,9 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     if (a.inu_pix) inu = (double)nf;
   }
   // inverse noise weight of row f (one value per Stokes block with windows)
+  const double* isg = a.isigmap + (long)pix * a.isig_stride;
   auto isig = [&](int f) -> double {
-    return a.poff ? a.isigmap[(long)(f / numw) * a.numw] : a.isigmap[f];
+    return a.poff ? isg[(long)(f / numw) * a.numw] : isg[f];
   };
   const int ld = np + 1;                       // LDS row stride of A and V
   const int npe = np + (np & 1);               // even size for the tournament ordering
@@ -317,10 +507,16 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
   // ---- NEW_CALCULATE_CHI2 (get_dmodel.f90:960-1032)
   const double* obs = a.obs + o_obs;
   const float* stk = a.stokes + o_stk;
+  const double* sd = a.syn_d ? a.syn_d + o_obs : nullptr;
+  // current synthesis of row f
+  auto syn = [&](int f) -> double {
+    if (sd) return sd[f];
+    const int b = f / numw, l = f - b * numw;
+    return (double)stk[a.rowstk[b] * numw + l];
+  };
   double part = 0.0;
   for (int f = t; f < nf; f += NT) {
-    const int b = f / numw, l = f - b * numw;
-    const double d = obs[f] - (double)stk[a.rowstk[b] * numw + l];
+    const double d = obs[f] - syn(f);
     part += d * d * isig(f);
   }
   double chi = block_sum<NT>(part, red) / inu;
@@ -340,10 +536,7 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     // best := current (model, equivalent response functions, synthesis)
     for (int i = t; i < 12 * nz; i += NT) a.atm_best[(long)i * ncol + pix] = mod[i];
     for (long i = t; i < (long)np * nf; i += NT) ebest[i] = izero[i / nf] ? 0.f : ecur[i];
-    for (int f = t; f < nf; f += NT) {
-      const int b = f / numw, l = f - b * numw;
-      bsyn[f] = (double)stk[a.rowstk[b] * numw + l];
-    }
+    for (int f = t; f < nf; f += NT) bsyn[f] = syn(f);
   } else {
     // current := best
     for (int i = t; i < 12 * nz; i += NT) mod[i] = a.atm_best[(long)i * ncol + pix];
@@ -455,176 +648,8 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
   }
   __syncthreads();
 
-  // ---- symmetric eigen-decomposition, parallel cyclic Jacobi (tournament order)
-  if (a.solver == 1) {
-    const int half = npe / 2;
-    for (int sweep = 0; sweep < 60; ++sweep) {
-      double o = 0.0, tt = 0.0;
-      for (int e = t; e < np * np; e += NT) {
-        const int i = e / np, j = e - i * np;
-        const double x = A[i * ld + j];
-        tt += x * x;
-        if (i != j) o += x * x;
-      }
-      const double off = block_sum<NT>(o, red), tot = block_sum<NT>(tt, red);
-      if (off <= 1e-30 * tot || off == 0.0) break;
-      for (int rnd = 0; rnd < npe - 1; ++rnd) {
-        // pair i: players L[i], L[npe-1-i] with L = [0, (rnd+k)%(npe-1)+1 ...]
-        for (int i = t; i < half; i += NT) {
-          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
-          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
-          const int p = min(p0, q0), q = max(p0, q0);
-          double cs = 1.0, sn = 0.0;
-          if (q < np) {
-            const double apq = A[p * ld + q];
-            if (apq != 0.0) {
-              const double th = (A[q * ld + q] - A[p * ld + p]) / (2.0 * apq);
-              const double tq = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(1.0 + th * th));
-              cs = 1.0 / sqrt(1.0 + tq * tq);
-              sn = tq * cs;
-            }
-          }
-          rot[2 * i] = cs;
-          rot[2 * i + 1] = sn;
-        }
-        __syncthreads();
-        // columns of A and V: A <- A J, V <- V J
-        for (int e = t; e < half * npe; e += NT) {
-          const int i = e / npe, k = e - i * npe;
-          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
-          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
-          const int p = min(p0, q0), q = max(p0, q0);
-          const double cs = rot[2 * i], sn = rot[2 * i + 1];
-          if (sn == 0.0 && cs == 1.0) continue;
-          const double akp = A[k * ld + p], akq = A[k * ld + q];
-          A[k * ld + p] = cs * akp - sn * akq;
-          A[k * ld + q] = sn * akp + cs * akq;
-          const double vkp = V[k * ld + p], vkq = V[k * ld + q];
-          V[k * ld + p] = cs * vkp - sn * vkq;
-          V[k * ld + q] = sn * vkp + cs * vkq;
-        }
-        __syncthreads();
-        // rows of A: A <- J^T A
-        for (int e = t; e < half * npe; e += NT) {
-          const int i = e / npe, k = e - i * npe;
-          const int p0 = (i == 0) ? 0 : ((rnd + i - 1) % (npe - 1)) + 1;
-          const int q0 = ((rnd + (npe - 1 - i) - 1) % (npe - 1)) + 1;
-          const int p = min(p0, q0), q = max(p0, q0);
-          const double cs = rot[2 * i], sn = rot[2 * i + 1];
-          if (sn == 0.0 && cs == 1.0) continue;
-          const double apk = A[p * ld + k], aqk = A[q * ld + k];
-          A[p * ld + k] = cs * apk - sn * aqk;
-          A[q * ld + k] = sn * apk + cs * aqk;
-        }
-        __syncthreads();
-      }
-    }
-    // truncated pseudo-inverse: keep |d| >= svdtol * max|d|
-    double mx = 0.0;
-    for (int i = 0; i < np; ++i) mx = fmax(mx, fabs(A[i * ld + i]));
-    const double thr = mx * (a.svdtol > 0 ? a.svdtol : 10.0e-4);
-    for (int e = t; e < np; e += NT) {
-      const double d = A[e * ld + e];
-      double pr = 0.0;
-      if (!(fabs(d) < thr)) {
-        for (int i = 0; i < np; ++i) pr += V[i * ld + e] * g[i];
-        pr /= d;
-      }
-      dv[e] = pr;
-    }
-    __syncthreads();
-    for (int i = t; i < np; i += NT) {
-      double s = 0.0;
-      for (int e = 0; e < np; ++e) s += V[i * ld + e] * dv[e];
-      del[i] = s;
-    }
-  } else {
-    // NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453): the matrix-vector and
-    // dot products spread over the workgroup; every thread holds the same
-    // scalars, so the iteration and its stopping test are uniform
-    double* xv = nod;
-    double* pv = nod + np;
-    double* rv = nod + 2 * np;
-    double* qv = nod + 3 * np;
-    auto dot = [&](const double* x, const double* y) {
-      double s = 0.0;
-      for (int i = t; i < np; i += NT) s += x[i] * y[i];
-      return block_sum<NT>(s, red);
-    };
-    // A is exactly symmetric here (its lower triangle mirrored, a symmetric
-    // penalty Hessian, the diagonal scaled): row i is read as column i, so
-    // the lanes of a wave read consecutive entries
-    auto matvec = [&](const double* x, double* y) {
-      for (int i = t; i < np; i += NT) {
-        double s = 0.0;
-        for (int j = 0; j < np; ++j) s += A[j * ld + i] * x[j];
-        y[i] = s;
-      }
-      __syncthreads();
-    };
-    for (int i = t; i < np; i += NT) {
-      xv[i] = 0.0;
-      pv[i] = g[i];
-      rv[i] = g[i];   // bet - alp x with x = 0
-    }
-    __syncthreads();
-    double crho = dot(rv, rv);
-    matvec(pv, qv);
-    double calpha = crho / dot(pv, qv);
-    for (int i = t; i < np; i += NT) {
-      xv[i] = xv[i] + calpha * pv[i];
-      rv[i] = rv[i] - calpha * qv[i];
-    }
-    __syncthreads();
-    double pcond = 1.0e29, dcond = -1.0e29;
-    bool conv = false;
-    // each iteration forms the next direction before its stopping test, so
-    // one pass over A gives both A x (the test) and A p (the next step);
-    // each of the two sums keeps the order of a separate product
-    {
-      const double crho0 = crho;
-      crho = dot(rv, rv);
-      const double cbeta = crho / crho0;
-      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
-      __syncthreads();
-      matvec(pv, qv);
-    }
-    for (int it = 1; it <= 100000; ++it) {
-      calpha = crho / dot(pv, qv);
-      for (int i = t; i < np; i += NT) {
-        xv[i] = xv[i] + calpha * pv[i];
-        rv[i] = rv[i] - calpha * qv[i];
-      }
-      __syncthreads();
-      const double crho0 = crho;
-      crho = dot(rv, rv);
-      const double cbeta = crho / crho0;
-      for (int i = t; i < np; i += NT) pv[i] = rv[i] + cbeta * pv[i];
-      __syncthreads();
-      double cs = 0.0;
-      for (int i = t; i < np; i += NT) {
-        double s = 0.0, sq = 0.0;
-        for (int j = 0; j < np; ++j) {
-          const double aij = A[j * ld + i];
-          s += aij * xv[j];
-          sq += aij * pv[j];
-        }
-        qv[i] = sq;
-        const double e = (s - g[i]) / (fabs(g[i]) + 1.0e-5);
-        cs += e * e;
-      }
-      const double cond = block_sum<NT>(cs, red) / (double)np;
-      if (cond < 1.0e-3 || dcond > 1.0e30) {
-        for (int i = t; i < np; i += NT) del[i] = xv[i];
-        conv = true;
-        break;
-      }
-      dcond = cond - pcond;
-      pcond = cond;
-    }
-    if (!conv)
-      for (int i = t; i < np; i += NT) del[i] = xv[i] / 10.0;
-  }
+  // ---- damped solve (symmetric eigen-decomposition or CG)
+  damped_solve<NT>(a.solver, a.svdtol, np, A, V, g, dv, del, red, rot, nod);
   __syncthreads();
   // NaN filter (get_dmodel.f90:806-822)
   if (t == 0) {

@@ -117,6 +117,11 @@ def library():
     L.fz_mlp_gradient_dev.argtypes = [vp, ip, vp, vp, vp, vp, vp, vp]
     L.fz_mlp_train_step_dev.argtypes = [vp, ip, vp, vp, vp, ctypes.c_float, vp, vp]
     L.fz_learned_init_dev.argtypes = [vp, ip, vp, vp, vp]
+    L.fz_get_perturbation.argtypes = [vp, ip, ip, dp, dp, ctypes.c_double, dp, dp, dp, ip, ctypes.c_double]
+    L.fz_get_perturbation_dev.argtypes = [vp, ip, ip, vp, vp, ctypes.c_double, vp, vp, vp, ip, ctypes.c_double, vp]
+    L.fz_hessian.argtypes = [vp, ip, ip, dp, dp]
+    L.fz_get_dmodel3ds.argtypes = [vp, ip, ip, dp, dp, dp, dp, ip, fp, fp, dp, ctypes.POINTER(fp),
+                                   ctypes.POINTER(fp), ctypes.POINTER(ctypes.c_int32)]
     _lib = L
     return L
 
@@ -131,7 +136,7 @@ def exported_symbols():
             "fz_invert3d",
             "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
             "fz_mlp_targets_dev", "fz_mlp_forward_dev", "fz_mlp_gradient_dev", "fz_mlp_train_step_dev",
-            "fz_learned_init_dev"]
+            "fz_learned_init_dev", "fz_get_perturbation", "fz_get_perturbation_dev", "fz_hessian", "fz_get_dmodel3ds"]
 
 
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
@@ -471,6 +476,76 @@ class Context:
         [ncol][4*numw] (float64) into atm [12][nz][ncol]."""
         self._check(self.lib.fz_learned_init_dev(self.h, obs.shape[0], obs.data_ptr(), atm.data_ptr(),
                                                  self._stream(atm)), "fz_learned_init_dev")
+
+    def get_perturbation(self, jac, diff, lam, reg_pen=None, reg_hss=None, dtype=1, svdtol=1e-4):
+        """GET_PERTURBATION (lm.f90:137-204) on host arrays: jac (nf, np)
+        = IJACOBIAN, diff (nf), reg_pen (np), reg_hss (np, np); returns
+        DELTA (np)."""
+        jac = np.asarray(jac, dtype=np.float64)
+        nf, np_ = jac.shape
+        jt = np.ascontiguousarray(jac.T)                  # IJACOBIAN(NF,NP) column-major
+        df = np.ascontiguousarray(diff, dtype=np.float64)
+        delta = np.zeros(np_)
+        dp = ctypes.POINTER(ctypes.c_double)
+        rp = None if reg_pen is None else np.ascontiguousarray(reg_pen, dtype=np.float64)
+        rh = None if reg_hss is None else np.ascontiguousarray(np.asarray(reg_hss, np.float64).T)
+        self._check(self.lib.fz_get_perturbation(self.h, np_, nf, jt.ctypes.data_as(dp), df.ctypes.data_as(dp),
+                                                 float(lam), delta.ctypes.data_as(dp),
+                                                 None if rp is None else rp.ctypes.data_as(dp),
+                                                 None if rh is None else rh.ctypes.data_as(dp), int(dtype),
+                                                 float(svdtol)), "fz_get_perturbation")
+        return delta
+
+    def get_perturbation_torch(self, jt, diff, lam, delta, reg_pen=None, reg_hss=None, dtype=1, svdtol=1e-4):
+        """fz_get_perturbation_dev on device tensors: jt (np, nf) (the
+        IJACOBIAN(NF,NP) layout), diff (nf), delta (np) out, reg_hss (np, np)
+        with REG_HSS(i, j) at [j, i]."""
+        np_, nf = jt.shape
+        ptr = lambda t: None if t is None else t.data_ptr()
+        self._check(self.lib.fz_get_perturbation_dev(self.h, np_, nf, jt.data_ptr(), diff.data_ptr(), float(lam),
+                                                     delta.data_ptr(), ptr(reg_pen), ptr(reg_hss), int(dtype),
+                                                     float(svdtol), self._stream(jt)), "fz_get_perturbation_dev")
+
+    def get_dmodel3ds(self, obs3d, syn3d, best_syn, inv_mas, dsyn, best_dsyn, isigmap3d, model: dict,
+                      best_model: dict, am_i_done):
+        """GET_DMODEL3DS (get_dmodel.f90:1327-1577) on host arrays in the
+        reference's column-major layout, given here as numpy arrays indexed
+        [x, y, ...] (C order of the reversed Fortran shape): obs3d, syn3d,
+        best_syn, isigmap3d (nx, ny, NFREQ); inv_mas (nx, ny, ld >= 9);
+        dsyn, best_dsyn (nx, ny, IFREEP, NFREQ) float32; model / best_model
+        {name: (nx, ny, nz) float32} with the names of ATM; am_i_done
+        (nx, ny) int32.  best_syn, inv_mas, best_dsyn, the models and
+        am_i_done are updated in place."""
+        nx, ny = obs3d.shape[:2]
+        dp, fp = ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_float)
+        for arr, dt in ((obs3d, np.float64), (syn3d, np.float64), (best_syn, np.float64), (inv_mas, np.float64),
+                        (dsyn, np.float32), (best_dsyn, np.float32), (isigmap3d, np.float64), (am_i_done, np.int32)):
+            if arr.dtype != dt or not arr.flags.c_contiguous:
+                raise ValueError("get_dmodel3ds: contiguous %s arrays expected" % np.dtype(dt).name)
+        cubes = [model[n] for n in ATM]
+        bcubes = [best_model[n] for n in ATM]
+        for cb in cubes + bcubes:
+            if cb.dtype != np.float32 or not cb.flags.c_contiguous:
+                raise ValueError("get_dmodel3ds: contiguous float32 model cubes expected")
+        mp = (fp * 12)(*[cb.ctypes.data_as(fp) for cb in cubes])
+        bp = (fp * 12)(*[cb.ctypes.data_as(fp) for cb in bcubes])
+        self._check(self.lib.fz_get_dmodel3ds(self.h, nx, ny, obs3d.ctypes.data_as(dp), syn3d.ctypes.data_as(dp),
+                                              best_syn.ctypes.data_as(dp), inv_mas.ctypes.data_as(dp),
+                                              inv_mas.shape[2], dsyn.ctypes.data_as(fp), best_dsyn.ctypes.data_as(fp),
+                                              isigmap3d.ctypes.data_as(dp), mp, bp,
+                                              am_i_done.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+                    "fz_get_dmodel3ds")
+
+    def hessian(self, jacobian):
+        """HESSIAN (lm.f90:30-39): jacobian (np, nf) -> HSS (np, np)."""
+        jacobian = np.asarray(jacobian, dtype=np.float64)
+        np_, nf = jacobian.shape
+        jf = np.asfortranarray(jacobian)                   # JACOBIAN(NP,NF) column-major
+        hss = np.zeros((np_, np_), order="F")
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self.lib.fz_hessian(self.h, np_, nf, jf.ctypes.data_as(dp), hss.ctypes.data_as(dp)),
+                    "fz_hessian")
+        return np.ascontiguousarray(hss)
 
     def invert3d_torch(self, atm, obs, control: CInvertControl, nx: int, ny: int, stokes=None, chi2=None):
         """INVERT3D on device tensors: atm [12, nz, nx*ny] float32 (initial

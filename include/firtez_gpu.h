@@ -228,6 +228,56 @@ int fz_lm_step_dev(fz_ctx* ctx, int ncol, const double* d_obs, const float* d_st
                    double* d_sla, const float* d_eqv, float* d_eqv_best, const double* d_isigmap,
                    float* d_atm, float* d_atm_best, int32_t* d_done, double* d_delta, void* stream);
 
+/* GET_DMODEL3DS() (get_dmodel.f90:1327-1577) on host arrays in the
+ * reference's module layout: one LM step (GET_PERT_1D, :1201-1244) for every
+ * pixel with am_i_done == 0, after fz_set_inversion.  All arrays
+ * column-major as the reference declares them, NFREQ as fz_set_inversion
+ * returns it (the inverted Stokes blocks), IFREEP the free parameters:
+ *   obs3d      OBS3D(NFREQ,NY,NX)      double in
+ *   syn3d      SYN3D(NFREQ,NY,NX)      double in (current synthesis)
+ *   best_syn   BEST_SYN(NFREQ,NY,NX)   double in/out
+ *   inv_mas    INV_MAS(ld_inv_mas,NY,NX) double in/out, rows 1..9 (steps,
+ *              chi2, previous chi2, PRECYC, ..., lambda)
+ *   dsyn       DSYN(NFREQ,IFREEP,NY,NX) float in (current EQVDSYN)
+ *   best_dsyn  BEST_DSYN(NFREQ,IFREEP,NY,NX) float in/out
+ *   isigmap3d  ISIGMAP3D(NFREQ,NY,NX)  double in
+ *   model[12]  TEM3D, PG3D, RHO3D, PEL3D, MW3D, BX3D, BY3D, BZ3D, VX3D,
+ *              VY3D, VZ3D, TAU3D5 (NZ,NY,NX) float in/out: current -> next trial
+ *   best_model[12] BEST_TEM3D ... BEST_TAU3D5 float in/out
+ *   am_i_done  AM_I_DONE(NY,NX) in/out: set where INV_MAS(1) > MAXSTEPS
+ * Synchronous.  Not combined with per-pixel windows (fz_lm_step_dev). */
+int fz_get_dmodel3ds(fz_ctx* ctx, int nx, int ny, const double* obs3d, const double* syn3d, double* best_syn,
+                     double* inv_mas, int ld_inv_mas, const float* dsyn, float* best_dsyn, const double* isigmap3d,
+                     float* const* model, float* const* best_model, int32_t* am_i_done);
+
+/* GET_PERTURBATION(NP,NF,IJACOBIAN,DIFF,LAMBDAP,DELTA,REG_PEN,REG_HSS,DTYPE,SVDTOL)
+ * (lm.f90:137-204) on host arrays in lm.f90's layout, REAL*8 column-major:
+ * ijacobian IJACOBIAN(NF,NP) (ijacobian[p*nf + f]), diff DIFF(NF), reg_pen
+ * REG_PEN(NP) or NULL (zero), reg_hss REG_HSS(NP,NP) or NULL (zero), delta
+ * DELTA(NP) out.  TERM1 = J^T J + REG_HSS with its diagonal times
+ * (1 + lambdap), TERM2 = J^T DIFF - REG_PEN; dtype 1: DELTA = TERM1^+ TERM2
+ * through the truncated pseudo-inverse of GET_INVERSE (lm.f90:61-133:
+ * singular values below svdtol times the largest dropped; svdtol <= 0 means
+ * the reference's default 1e-3), dtype 2: the CG iteration of
+ * NEW_CALCULATE_DELTA_UBICGSTAB (lm.f90:375-453).  Any other dtype returns
+ * FZ_EARG (the reference stops, lm.f90:194-201).  NP up to 2048.  J^T J and
+ * J^T DIFF are formed on the FP64 matrix cores.  Synchronous. */
+int fz_get_perturbation(fz_ctx* ctx, int np, int nf, const double* ijacobian, const double* diff, double lambdap,
+                        double* delta, const double* reg_pen, const double* reg_hss, int dtype, double svdtol);
+
+/* fz_get_perturbation on device arrays of the same layout, asynchronous on
+ * stream (NULL = default stream).  The normal matrix and the solver's work
+ * arrays live in device memory that the context allocates on first use and
+ * keeps. */
+int fz_get_perturbation_dev(fz_ctx* ctx, int np, int nf, const double* d_ijacobian, const double* d_diff,
+                            double lambdap, double* d_delta, const double* d_reg_pen, const double* d_reg_hss,
+                            int dtype, double svdtol, void* stream);
+
+/* HESSIAN(NP,NF,JACOBIAN,HSS) (lm.f90:30-39): HSS = JACOBIAN JACOBIAN^T on
+ * host arrays, jacobian JACOBIAN(NP,NF) column-major (jacobian[f*np + p]),
+ * hss HSS(NP,NP) out (symmetric).  FP64 matrix cores.  Synchronous. */
+int fz_hessian(fz_ctx* ctx, int np, int nf, const double* jacobian, double* hss);
+
 /* Control of a whole inversion (the INVERSION SETUP, STOKES SETUP and MISC
  * SETUP sections of the control file, input_file.f90:1064-1135, 1280-1303,
  * 170-190). */

@@ -40,3 +40,23 @@ def test_binding_fails_loudly_without_library(monkeypatch):
     monkeypatch.setattr(gpu, "_lib", None)
     with pytest.raises(gpu.FzError):
         gpu.library()
+
+
+def test_fortran_module_and_caller_link():
+    """The FIRTEZ_GPU module compiles with amdflang and a Fortran host
+    (tests/fortran/fz_caller.f90) links against it and the library, binding
+    the C symbols of FORWARD3D, INVERT3D, GET_PERTURBATION, HESSIAN."""
+    import shutil
+    import subprocess
+    from firtez_dz_amd import build, gpu
+    if not os.path.exists(build.FLANG) or not os.path.exists(gpu.LIB_PATH):
+        pytest.skip("amdflang or the library not available")
+    build.build_fortran()
+    assert os.path.exists(build.CALLER)
+    if shutil.which("nm"):
+        und = subprocess.run(["nm", "-u", build.CALLER], capture_output=True, text=True).stdout
+        for sym in ("fz_forward3d", "fz_invert3d", "fz_get_perturbation", "fz_hessian", "fz_create"):
+            assert sym in und, sym
+    if shutil.which("readelf"):
+        dyn = subprocess.run(["readelf", "-d", build.CALLER], capture_output=True, text=True).stdout
+        assert "libfirtez_gpu.so" in dyn

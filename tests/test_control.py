@@ -127,3 +127,27 @@ def test_reader_errors():
         control.parse("BOX:\n1 1\n1 1\n1 1\nFILEMODEL:\nm\nMODE:\ntau\n")
     with pytest.raises(NotImplementedError):
         control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nTEMP\n")
+
+
+def test_line_spread_function_section(tmp_path):
+    """NSET_LSF (input_file.f90:1719-1865): one row per region -- "sigma w0",
+    an LSF file name or EMPTY; sigma 0 and a row count different from the
+    LINES count are errors."""
+    from firtez_dz_amd import control, lines, modelio
+    base = ("LINES:\n4,5 100 -300.0 10.0\n4 50 -200.0 10.0\n5 40 -100.0 10.0\nFILEPROFILE:\np.bin\n"
+            "FILEMODEL:\nm.bin\nMODE:\nsynthesis\nMISC SETUP:\nDATAPATH ./\n")
+    c = control.parse(base + "LINE SPREAD FUNCTION:\n25.0 3.0\nlsf_b.bin\nEMPTY\nEND:\n")
+    assert c.lsf == [(25.0, 3.0), "lsf_b.bin", None]
+    taps = np.exp(-0.5 * ((np.arange(50) - 25) / 3.0) ** 2)
+    taps = np.roll(taps / taps.sum(), -25)
+    modelio.write_lsf(str(tmp_path / "lsf_b.bin"), taps)
+    ls = control.line_setup(c)
+    kern, valid = control.lsf_arrays(c, ls, str(tmp_path))
+    assert list(valid) == [1, 1, 0]
+    k0, _ = lines.lsf_kernel(ls, [25.0, -1, -1], [3.0, -1, -1])
+    assert np.array_equal(kern[:100], k0[:100])
+    assert np.allclose(kern[100:150] * 50, taps / taps.sum(), rtol=1e-6) and np.all(kern[150:] == 0)
+    with pytest.raises(ValueError):
+        control.parse(base + "LINE SPREAD FUNCTION:\n0.0 3.0\n1 1\n1 1\nEND:\n")
+    with pytest.raises(ValueError):
+        control.parse(base + "LINE SPREAD FUNCTION:\n20.0 3.0\nEND:\n")

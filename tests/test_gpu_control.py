@@ -128,3 +128,27 @@ def test_inversion_after_synthesis(gpu_ctx, tree):
     assert np.isfinite(chi) and e1 < 0.5 * e0
     m = modelio.read_model(os.path.join(wd, "out_test_atmos.bin"))
     assert np.all(np.isfinite(m["T"])) and np.all(m["T"] >= 2500.0)
+
+
+def test_syn1_with_line_spread_function(gpu_ctx, golden, tree):
+    """test_syn1 with a LINE SPREAD FUNCTION: section (a Gaussian of sigma
+    30 mA centred on 0, NSET_LSF / LSF_INIT): the written profiles equal the
+    profiles without it circularly convolved with the LSF taps, the FFT
+    product of forward.f90:810-844 written with numpy's DFT."""
+    from firtez_dz_amd import lines
+    wd = tree / "tests" / "test_syn1"
+    _, _, out = run(tree, "test_syn1", gpu_ctx)
+    st0, _, _ = modelio.read_profiles(os.path.join(out, "syn_profile.bin"))
+    txt = (wd / "control_file.dat").read_text().replace("END:", "LINE SPREAD FUNCTION:\n30.0 0.0\nEND:")
+    (wd / "control_file.dat").write_text(txt)
+    c, _, out = run(tree, "test_syn1", gpu_ctx)
+    assert c.lsf == [(30.0, 0.0)]
+    st1, _, _ = modelio.read_profiles(os.path.join(out, "syn_profile.bin"))
+    kern, valid = lines.lsf_kernel(control.line_setup(c), [30.0], [0.0])
+    assert valid[0] == 1
+    n = kern.size
+    taps = kern * n
+    ref = np.stack([np.fft.irfft(np.fft.rfft(st0[s, :, 0, 0]) * np.fft.rfft(taps), n) for s in range(4)])
+    report("test_syn1 + LSF vs convolved profiles", float(np.max(np.abs(st1[:, :, 0, 0] - ref))), 1e-5)
+    assert float(np.max(np.abs(st1 - st0))) > 1e-3          # the LSF changed the profiles
+    gpu_ctx.set_lsf(None)

@@ -130,14 +130,45 @@ def test_eqvdsyn_lsf_vs_oracle(gpu_ctx, cpu):
         cpu.set_lsf(None)
 
 
-def _oracle_step(cpu, P, obs, syn, bsyn, sla, eqv, ebest, isig, cur, best):
+def _oracle_step(cpu, P, obs, syn, bsyn, sla, eqv, ebest, isig, cur, best, perm=False):
+    """The oracle's GET_PERT_1D per pixel.  perm: the spectral rows of every
+    input in a seeded random order -- the same step with every sum over
+    rows taken in another order."""
     out = []
     for c in range(P["ncol"]):
         mc = atmos.column_model(cur, P["z"], c)
         mb = atmos.column_model(best, P["z"], c)
-        r = cpu.lm_step(P["cfg"], P["coefs"], obs[c], syn[c], bsyn[c], sla[c], eqv[c].T, ebest[c].T, isig, mc, mb)
+        p = np.random.default_rng(c).permutation(len(obs[c])) if perm else slice(None)
+        r = cpu.lm_step(P["cfg"], P["coefs"], obs[c][p], syn[c][p], bsyn[c][p], sla[c], eqv[c].T[p], ebest[c].T[p],
+                        isig[p], mc, mb)
         out.append(r)
     return out
+
+
+def _delta_err(a, b):
+    return np.max(np.abs(a - b) / (np.abs(b) + 1e-6 * np.abs(b).max()))
+
+
+def _model_err(g_cur, c, mc):
+    return max(np.max(np.abs(g_cur[row, :, c] - mc[row]) / (np.abs(mc[row]) + 1e-3)) for row in (0, 5, 6, 7, 10))
+
+
+def _tolerances(solver, ref, ref_perm, ncol):
+    """Jacobi path: fixed tolerances.  CG (LINEAR_SOLVER 2): the iteration
+    stops at a mean squared relative residual of 1e-3, and its iterate
+    follows the rounding of every sum (the oracle moves by 3e-4 on six free
+    parameters and by O(1) on twenty, with a penalty, when only the order of
+    its sums over the spectral rows changes); the device, whose dot products
+    are a tree of partial sums, is held to four times the oracle's own
+    spread, at least 1e-4."""
+    if solver == 1:
+        return 1e-6, 2e-6, 0.0
+    sens_d = max(_delta_err(ref_perm[c][5], ref[c][5]) for c in range(ncol))
+    sens_m = max(max(np.max(np.abs(ref_perm[c][0][row] - ref[c][0][row]) / (np.abs(ref[c][0][row]) + 1e-3))
+                     for row in (0, 5, 6, 7, 10)) for c in range(ncol))
+    print("[parity] oracle self-spread under reordered sums: perturbation %.3e, model %.3e" % (sens_d, sens_m),
+          flush=True)
+    return max(1e-4, 4 * sens_d), max(1e-4, 4 * sens_m), sens_d
 
 
 NODES_BIG = {"T": 40, "Bz": 16, "vz": 24}
@@ -187,8 +218,10 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver, pen):
     h_st = st.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
     h_eqv = eqv.cpu().numpy()
     h_cur, h_best = atm.cpu().numpy(), best.cpu().numpy()
-    ref = _oracle_step(cpu, P, P["obs"], h_st, np.zeros((ncol, nf)), sla0, h_eqv, np.zeros_like(h_eqv),
-                       np.full(nf, 1.0 / NOISE ** 2), h_cur, h_best)
+    args1 = (P["obs"], h_st, np.zeros((ncol, nf)), sla0, h_eqv, np.zeros_like(h_eqv), np.full(nf, 1.0 / NOISE ** 2),
+             h_cur, h_best)
+    ref = _oracle_step(cpu, P, *args1)
+    ref_p = _oracle_step(cpu, P, *args1, perm=True) if solver == 2 else None
     gpu_ctx.lm_step_torch(obs, st, bsyn, sla, eqv, ebest, isig, atm, best, done, delta)
     torch.cuda.synchronize()
     g_sla, g_cur, g_best, g_del = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy(), delta.cpu().numpy()
@@ -200,14 +233,10 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver, pen):
         werr["chi"] = max(werr["chi"], abs(g_sla[c, 1] - sl[1]) / sl[1])
         assert np.array_equal(g_best[:, :, c], h_cur[:, :, c])         # accepted: best := current
         assert np.array_equal(g_eb[c], h_eqv[c]) and np.array_equal(g_bs[c], h_st[c])
-        werr["delta"] = max(werr["delta"], np.max(np.abs(g_del[c] - dl) / (np.abs(dl) + 1e-6 * np.abs(dl).max())))
-        for row in (0, 5, 6, 7, 10):
-            werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
-                                                      (np.abs(mc[row]) + 1e-3)))
+        werr["delta"] = max(werr["delta"], _delta_err(g_del[c], dl))
+        werr["model"] = max(werr["model"], _model_err(g_cur, c, mc))
     report("LM step (accept) chi2 vs oracle (relative)", werr["chi"], 1e-11)
-    # the CG iteration compounds the rounding of its dot products (a tree of
-    # partial sums on the device, a running sum in the oracle) over its steps
-    dtol, mtol = (1e-6, 2e-6) if solver == 1 else (1e-4, 1e-4)
+    dtol, mtol, _ = _tolerances(solver, ref, ref_p, ncol)
     report("LM step (accept) node perturbation vs oracle", werr["delta"], dtol)
     report("LM step (accept) new model vs oracle (relative)", werr["model"], mtol)
 
@@ -216,8 +245,9 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver, pen):
     h_worse = worse.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
     h_sla, h_cur2, h_best2 = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy()
     h_bs2, h_eb2, h_eqv2 = bsyn.cpu().numpy(), ebest.cpu().numpy(), eqv.cpu().numpy()
-    ref2 = _oracle_step(cpu, P, P["obs"], h_worse, h_bs2, h_sla, h_eqv2, h_eb2, np.full(nf, 1.0 / NOISE ** 2),
-                        h_cur2, h_best2)
+    args2 = (P["obs"], h_worse, h_bs2, h_sla, h_eqv2, h_eb2, np.full(nf, 1.0 / NOISE ** 2), h_cur2, h_best2)
+    ref2 = _oracle_step(cpu, P, *args2)
+    ref2_p = _oracle_step(cpu, P, *args2, perm=True) if solver == 2 else None
     gpu_ctx.lm_step_torch(obs, worse, bsyn, sla, eqv, ebest, isig, atm, best, done, delta)
     torch.cuda.synchronize()
     g_sla, g_cur, g_best, g_del = sla.cpu().numpy(), atm.cpu().numpy(), best.cpu().numpy(), delta.cpu().numpy()
@@ -227,10 +257,9 @@ def test_lm_step_vs_oracle(gpu_ctx, cpu, nodes, solver, pen):
         assert g_sla[c, 0] == sl[0] == h_sla[c, 0] + 1
         assert g_sla[c, 8] == pytest.approx(sl[8], rel=1e-14) and sl[8] == pytest.approx(10 * h_sla[c, 8])
         assert np.array_equal(g_best[:, :, c], h_best2[:, :, c])      # rejected: best kept
-        werr["delta"] = max(werr["delta"], np.max(np.abs(g_del[c] - dl) / (np.abs(dl) + 1e-6 * np.abs(dl).max())))
-        for row in (0, 5, 6, 7, 10):
-            werr["model"] = max(werr["model"], np.max(np.abs(g_cur[row, :, c] - mc[row]) /
-                                                      (np.abs(mc[row]) + 1e-3)))
+        werr["delta"] = max(werr["delta"], _delta_err(g_del[c], dl))
+        werr["model"] = max(werr["model"], _model_err(g_cur, c, mc))
+    dtol, mtol, _ = _tolerances(solver, ref2, ref2_p, ncol)
     report("LM step (reject) node perturbation vs oracle", werr["delta"], dtol)
     report("LM step (reject) new model vs oracle (relative)", werr["model"], mtol)
 
@@ -471,3 +500,60 @@ def test_p0_inversion_step_vs_oracle(gpu_ctx, cpu):
     finally:
         gpu_ctx.set_hydrostatic(False)
         cpu.set_hydrostatic(False)
+
+
+def test_get_dmodel3ds_host_arrays_vs_oracle(gpu_ctx, cpu):
+    """GET_DMODEL3DS on host arrays in the reference's module layout
+    (fz_get_dmodel3ds): OBS3D, SYN3D (FP64), BEST_SYN, INV_MAS, DSYN,
+    BEST_DSYN, a per-pixel ISIGMAP3D, the current and best model cubes and
+    AM_I_DONE; one accepting and one rejecting step against the oracle's
+    GET_PERT_1D per pixel.  Pixel 5 is marked done and must stay untouched."""
+    import torch
+    P = problem(gpu_ctx, cpu, NODES_C2)
+    nx = ny = 4
+    ncol, nf, npr, numw, nz = P["ncol"], P["nfreq"], P["ifreep"], P["numw"], P["nz"]
+    atm = torch.from_numpy(np.ascontiguousarray(P["guess"])).cuda()
+    st, eqv, status = gpu_ctx.forward_nodes_torch(atm)
+    torch.cuda.synchronize()
+    h_st = st.cpu().numpy().reshape(ncol, 4 * numw).astype(np.float64)
+    h_eqv = eqv.cpu().numpy()
+    cur = atm.cpu().numpy()
+    noise = NOISE * (1.0 + 0.1 * np.arange(ncol))             # a different ISIGMAP3D per pixel
+    isig = np.repeat((1.0 / noise ** 2)[:, None], nf, axis=1)
+    cube = lambda m: {n: np.ascontiguousarray(m[i].T.reshape(nx, ny, nz)) for i, n in enumerate(gpu.ATM)}
+    sh = lambda a: np.ascontiguousarray(a.reshape((nx, ny) + a.shape[1:]))
+    obs3d, syn3d, isig3d = sh(P["obs"]), sh(h_st), sh(isig)
+    best_syn = np.zeros((nx, ny, nf))
+    inv_mas = np.zeros((nx, ny, 12))
+    inv_mas[:, :, :9] = [0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]
+    dsyn, best_dsyn = sh(h_eqv), np.zeros((nx, ny, npr, nf), np.float32)
+    model, best = cube(cur), cube(cur)
+    done = np.zeros((nx, ny), np.int32)
+    done[1, 1] = 1                                               # pixel 5 = (x 1, y 1)
+    keep = {n: model[n][1, 1].copy() for n in gpu.ATM}
+    for step, syn in enumerate((h_st, h_st + 0.05)):
+        syn3d = sh(syn)
+        m_cur = np.stack([model[n].reshape(ncol, nz).T for n in gpu.ATM])
+        m_best = np.stack([best[n].reshape(ncol, nz).T for n in gpu.ATM])
+        ref = [cpu.lm_step(P["cfg"], P["coefs"], P["obs"][c], syn[c], best_syn.reshape(ncol, nf)[c],
+                           inv_mas.reshape(ncol, 12)[c, :9], h_eqv[c].T, best_dsyn.reshape(ncol, npr, nf)[c].T,
+                           isig[c], atmos.column_model(m_cur, P["z"], c), atmos.column_model(m_best, P["z"], c))
+               for c in range(ncol)]
+        gpu_ctx.get_dmodel3ds(obs3d, syn3d, best_syn, inv_mas, dsyn, best_dsyn, isig3d, model, best, done)
+        chi, mod = 0.0, 0.0
+        for c in range(ncol):
+            x, y = divmod(c, ny)
+            if c == 5:
+                assert inv_mas[x, y, 0] == 0 and np.all(best_syn[x, y] == 0)
+                assert all(np.array_equal(model[n][x, y], keep[n]) for n in gpu.ATM)
+                continue
+            mc, mb, sl, bs, jb, _ = ref[c]
+            assert inv_mas[x, y, 0] == sl[0] and inv_mas[x, y, 8] == pytest.approx(sl[8], rel=1e-14)
+            chi = max(chi, abs(inv_mas[x, y, 1] - sl[1]) / sl[1])
+            assert np.array_equal(best_syn[x, y], bs) and np.array_equal(best_dsyn[x, y], jb.T)
+            for i in (0, 5, 6, 7, 10):
+                n = gpu.ATM[i]
+                mod = max(mod, np.max(np.abs(model[n][x, y] - mc[i]) / (np.abs(mc[i]) + 1e-3)))
+                assert np.array_equal(best[n][x, y], mb[i])
+        report("GET_DMODEL3DS step %d chi2 vs oracle (relative)" % step, chi, 1e-11)
+        report("GET_DMODEL3DS step %d new model vs oracle (relative)" % step, mod, 2e-6)
