@@ -126,6 +126,109 @@ def cpu_baseline(ls, nz, gain, seconds=12.0, threads=0):
                       "FP64 oracle, %d OpenMP threads, %.1f s" % (done, nz, numw, threads, dt)}
 
 
+def lm_metric(n=128, nz=64, train_cubes=16, epochs=6, max_passes=3, log=None):
+    """The inversion half of the metric on a smaller map: LM-converged pixels
+    per second of plain INVERT3D and of the learned pipeline (network
+    proposal + INVERT3D + plain-LM fallback), same GPU, same observations.
+
+    Observations: the device forward of a seeded n x n x nz truth + N(0, 1e-3)
+    noise (Fe I 6301.5+6302.5, 112 wavelengths).  Reference chi^2 of a pixel:
+    the chi^2 of the truth's own 4-node projection (T, Bx, By, Bz, v_z each
+    replaced by its least-squares fit on the hat basis of 4 nodes, the model
+    the inversion can represent) against the same noisy observations.  A
+    pixel is converged when its final chi^2 is at most twice its reference;
+    the learned pipeline hands a pixel to plain LM from the default model
+    when its chi^2 exceeds that bound (the fallback rule of the learned
+    initial model).  The network is trained beforehand (not timed) on
+    train_cubes seeded n x n cubes of other seeds."""
+    import torch
+    from firtez_dz_amd import atmos, gpu, learned, lines
+
+    say = log or (lambda m: None)
+    ncol = n * n
+    ls = lines.fe6301_6302(nwave=112, start=-624.0, step=20.0)
+    atm, z = atmos.seeded(n, n, nz, dz_km=15.0, seed=7)
+    ctx = gpu.Context(0)
+    try:
+        numw = ctx.configure(ls, z)
+        ctx.gain_hsra()
+        truth = torch.from_numpy(atm).cuda()
+        assert int(ctx.hydrostatic_torch(truth).max().item()) == 0
+        st, _, status = ctx.forward_torch(truth, gpu.RF_NONE)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(11)
+        obs = (st.double().reshape(ncol, 4 * numw) + 1e-3 * torch.randn((ncol, 4 * numw), generator=gen,
+                                                                        device="cuda", dtype=torch.float64))
+        obs = obs.contiguous()
+        # reference chi^2: the truth projected on the 4-node basis
+        C = torch.from_numpy(gpu.int_coefs(nz, 4)).cuda()
+        Pz = (C @ torch.linalg.pinv(C)).float()
+        proj = truth.clone()
+        for r in (0, 5, 6, 7, 10):
+            proj[r] = Pz @ truth[r]
+        pst, _, pstat = ctx.forward_torch(proj, gpu.RF_NONE)
+        torch.cuda.synchronize()
+        ref_chi2 = ((pst.double().reshape(ncol, -1) - obs) ** 2).mean(dim=1) / 1e-6
+        bound = 2.0 * ref_chi2
+        ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3, smoothing=False)
+        init = plain_init(ctx, truth)
+        say("plain INVERT3D of %d pixels" % ncol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m_plain, _, c_plain, rp = learned.invert(ctx, init, obs, ctl, n, n, None, learned=False)
+        torch.cuda.synchronize()
+        t_plain = time.perf_counter() - t0
+        # the network (training not timed)
+        ctx.mlp_create(4 * numw, 512, 3, 4, seed=1)
+        X, T = learned.training_set(ctx, train_cubes, n, n, nz, seed0=1000, noise=1e-3)
+        rep = learned.train(ctx, X, T, epochs=epochs, batch=1024, lr=1e-3)
+        del X, T
+        say("learned pipeline (network trained in %.1f s)" % rep.seconds)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m_lrn, _, c_lrn, rl = learned.invert(ctx, init, obs, ctl, n, n, bound, learned=True)
+        torch.cuda.synchronize()
+        t_lrn = time.perf_counter() - t0
+        conv_p = int((c_plain <= bound).sum().item())
+        conv_l = int((c_lrn <= bound).sum().item())
+        # plain LM restarted from its own result towards the learned median chi^2
+        target = float(c_lrn.median())
+        passes, t_more, cur, med = 1, t_plain, m_plain, float(c_plain.median())
+        conv_more = conv_p
+        while med > target and passes < max_passes:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            cur, _, c2, _ = learned.invert(ctx, cur, obs, ctl, n, n, None, learned=False)
+            torch.cuda.synchronize()
+            t_more += time.perf_counter() - t0
+            passes += 1
+            med = float(c2.median())
+            conv_more = int((c2 <= bound).sum().item())
+        out = {
+            "workload": "INVERT3D %dx%d pixels, nz=%d, Fe I 6301.5+6302.5, %d wavelengths, T/Bx/By/Bz/vz up to 4 "
+                        "nodes (2 cycles), obs = forward(seeded truth) + N(0, 1e-3 Ic)" % (n, n, nz, numw),
+            "converged_rule": "final chi2 <= 2 x chi2 of the truth's 4-node projection (per pixel)",
+            "reference_chi2_median": float(ref_chi2.median()),
+            "plain": {"seconds": t_plain, "pixels_per_s": ncol / t_plain, "converged_pixels": conv_p,
+                      "converged_pixels_per_s": conv_p / t_plain, "median_chi2": float(c_plain.median()),
+                      "lm_iterations": rp.iterations},
+            "learned": {"seconds": t_lrn, "pixels_per_s": ncol / t_lrn, "converged_pixels": conv_l,
+                        "converged_pixels_per_s": conv_l / t_lrn, "median_chi2": target,
+                        "lm_iterations": rl.iterations, "fallback_pixels": rl.fallback_pixels,
+                        "seconds_network": rl.seconds_init, "seconds_lm": rl.seconds_lm,
+                        "seconds_fallback": rl.seconds_fallback},
+            "network": {"shape": "%d-512-512-512-20" % (4 * numw), "pairs": train_cubes * ncol, "epochs": epochs,
+                        "train_seconds": rep.seconds, "valid_loss": rep.valid_loss[-1] if rep.valid_loss else None},
+            "plain_to_learned_median_chi2": {"passes": passes, "seconds": t_more, "median_chi2": med,
+                                             "reached": med <= target, "converged_pixels": conv_more},
+            "speedup_converged_pixels_per_s": (conv_l / t_lrn) / (conv_p / t_plain) if conv_p else None,
+            "speedup_at_learned_median_chi2": (t_more / t_lrn) if med <= target else None,
+        }
+        return out
+    finally:
+        ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +241,8 @@ def main():
     # workload (--autotune re-runs it during warm-up)
     ap.add_argument("--tile", type=int, default=32768)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-lm", action="store_true", help="skip the inversion half of the metric")
+    ap.add_argument("--lm-n", type=int, default=128, help="pixels per side of the inversion half")
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--cpu-kind", choices=("reference", "port"), default="reference",
@@ -258,6 +363,17 @@ def main():
             print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
             base = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds)
         line["cpu_baseline"] = base
+    if not args.no_lm:
+        # the second half of the metric: LM-converged pixels/s, learned vs plain
+        del jac, stokes
+        torch.cuda.empty_cache()
+        print("[bench] inversion half: %dx%d pixels" % (args.lm_n, args.lm_n), file=sys.stderr, flush=True)
+        line["lm"] = lm_metric(n=args.lm_n, log=lambda m: print("[bench-lm] " + m, file=sys.stderr, flush=True))
+        lm = line["lm"]
+        print("[bench] LM-converged pixels/s: plain %.3g (%d of %d), learned %.3g (%d); median chi2 %.3g vs %.3g"
+              % (lm["plain"]["converged_pixels_per_s"], lm["plain"]["converged_pixels"], args.lm_n ** 2,
+                 lm["learned"]["converged_pixels_per_s"], lm["learned"]["converged_pixels"],
+                 lm["plain"]["median_chi2"], lm["learned"]["median_chi2"]), file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

@@ -22,10 +22,12 @@ RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
-CONTINUUM NORMALIZATION, COUPLED INVERSION, LINE OPTICAL DEPTH, NLTE
-DEPARTURE COEFFICIENTS, AUTO Stokes weights, the ASSIST_* initialisers,
-FULL_STOKES) raise NotImplementedError instead of being ignored.  LINE
-SPREAD FUNCTION (Gaussian rows or LSF files) sets the library's LSF.
+CONTINUUM NORMALIZATION, COUPLED INVERSION, LINE OPTICAL DEPTH, AUTO
+Stokes weights, FULL_STOKES) raise NotImplementedError instead of being
+ignored.  MISC SETUP TEMP / PGAS / BVEC / VLOS switch on SET_TAU's assist
+initialisers.  LINE SPREAD FUNCTION
+(Gaussian rows or LSF files) sets the library's LSF; NLTE DEPARTURE
+COEFFICIENTS (per-line files of BETAL, BETAU per height) sets POPL/POPU.
 """
 from __future__ import annotations
 
@@ -43,8 +45,7 @@ from . import modelio
 PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
-_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "COUPLED INVERSION:", "LINE OPTICAL DEPTH:",
-                "NLTE DEPARTURE COEFFICIENTS:")
+_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "COUPLED INVERSION:", "LINE OPTICAL DEPTH:")
 
 
 @dataclass
@@ -75,6 +76,10 @@ class Control:
     # LINE SPREAD FUNCTION: per region (sigma, w0) in mA, or an LSF file name
     # (DATAPATH-relative), or None for EMPTY; empty list: no LSF
     lsf: List[object] = field(default_factory=list)
+    # NLTE DEPARTURE COEFFICIENTS: (line index, file name or LTE) rows
+    nlte: List[Tuple[int, str]] = field(default_factory=list)
+    # MISC SETUP TEMP / PGAS / BVEC / VLOS: ASSIST_* bits
+    assist: int = 0
 
 
 def _number(tok: str) -> float:
@@ -203,6 +208,14 @@ def parse(text: str) -> Control:
                             c.pen_typ[name] = _nint(v)
                         else:
                             c.pen_alp[name] = float(np.float32(v))   # PEN_ALP is REAL(SP) (invert_param.f90:66)
+        elif key == "NLTE DEPARTURE COEFFICIENTS:":
+            # NSET_NLTE (input_file.f90:1979-2144): "<line index> <file | LTE>" rows
+            body, i = section(i)
+            for row in body:
+                cols = row.split()
+                if len(cols) != 2:
+                    raise ValueError('wrong input format in "NLTE DEPARTURE COEFFICIENTS": %r' % row)
+                c.nlte.append((int(_number(cols[0])), cols[1]))
         elif key == "LINE SPREAD FUNCTION:":
             # NSET_LSF (input_file.f90:1719-1865): one row per region, either
             # "sigma w0" (Gaussian, sigma /= 0) or an LSF file name, or EMPTY
@@ -221,9 +234,13 @@ def parse(text: str) -> Control:
         elif key == "MISC SETUP:":
             body, i = section(i)
             for row in body:
-                for name in ("TEMP", "PGAS", "BVEC", "VLOS", "FULL_STOKES"):
+                # ASSIST_T/P/B/V (input_file.f90:170-176): SET_TAU's initialisers
+                for name, bit in (("TEMP", _gpu.ASSIST_T), ("PGAS", _gpu.ASSIST_P), ("BVEC", _gpu.ASSIST_B),
+                                  ("VLOS", _gpu.ASSIST_V)):
                     if _field(row, name, 0)[0]:
-                        raise NotImplementedError("MISC SETUP %s is not supported by this path" % name)
+                        c.assist |= bit
+                if _field(row, "FULL_STOKES", 0)[0]:
+                    raise NotImplementedError("MISC SETUP FULL_STOKES is not supported by this path")
                 ok, v = _field(row, "SVDTOL", 0)
                 if ok:
                     c.svdtol = float(v)
@@ -287,6 +304,32 @@ def lsf_arrays(c: Control, ls: _lines.LineSetup, base: str = "."):
     return _lines.lsf_kernel(ls, sig, w0, taps)
 
 
+def departure_arrays(c: Control, nz: int, base: str = "."):
+    """POPL, POPU (nregions, nz) from the NLTE DEPARTURE COEFFICIENTS rows,
+    with NSET_NLTE's rules (input_file.f90:1979-2144): a line with a file
+    must be the main line of a LINES row and must not share its region with
+    another line; its region takes the file's coefficients
+    (READ_DEPARTURE_COEFFICIENTS, :2146-2166, DATAPATH-relative); every
+    other region stays at 1 (LTE)."""
+    popl = np.ones((len(c.lines), nz))
+    popu = np.ones((len(c.lines), nz))
+    for lid, fname in c.nlte:
+        if fname.upper() == "LTE":
+            continue
+        for ids, *_ in c.lines:
+            if lid in ids and len(ids) > 1:
+                raise ValueError("NLTE lines cannot be blended (line %d)" % lid)
+        regs = [r for r, (ids, *_) in enumerate(c.lines) if ids[0] == lid]
+        if not regs:
+            raise ValueError("departure coefficients for line %d, which is not under LINES" % lid)
+        path = os.path.join(base, c.datapath, fname)
+        if not os.path.exists(path):
+            raise FileNotFoundError("cannot find NLTE file %s in directory %s" % (fname, c.datapath))
+        bl, bu = modelio.read_departure(path, nz)
+        popl[regs[0]], popu[regs[0]] = bl, bu
+    return popl, popu
+
+
 def line_setup(c: Control) -> _lines.LineSetup:
     """The configured regions; a mode without LINES (tau, hydrostatic) gets
     one wavelength of Fe I 6301.5, which leaves the model outputs unchanged
@@ -310,7 +353,7 @@ def control_for_inversion(c: Control, nz: int) -> _gpu.CInvertControl:
                                cycpow=c.cycpow, maxsteps=c.maxsteps, solver=c.solver,
                                node_expansion=c.node_expansion, svdtol=c.svdtol, smoothing=True,
                                regul=c.vregul, pen_typ={PARAM_NAMES[k]: v for k, v in c.pen_typ.items()},
-                               pen_alp={PARAM_NAMES[k]: v for k, v in c.pen_alp.items()})
+                               pen_alp={PARAM_NAMES[k]: v for k, v in c.pen_alp.items()}, assist=c.assist)
 
 
 def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> dict:
@@ -335,6 +378,8 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
         ctx.gain_hsra()
         if c.lsf:
             ctx.set_lsf(*lsf_arrays(c, ls, workdir))
+        if any(f.upper() != "LTE" for _, f in c.nlte):
+            ctx.set_departure(*departure_arrays(c, nz, workdir))
         ctx.set_hydrostatic(c.hydrostatic)
         if c.hydrostatic and not c.hydro_ders and c.mode == "inversion":
             raise NotImplementedError("HYDROSTATIC DERIVATIVES: no is not supported by this path")

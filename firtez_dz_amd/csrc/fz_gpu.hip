@@ -104,6 +104,8 @@ struct MicroTile {
   double* eos;     // [2][nz][ntile]: n_e, N_H
   double* deriv;   // [8][nz][ntile]: the DERIVVAR scalars (with response functions)
   int32_t* status; // [ncol]
+  // NLTE departure coefficients [2][nz][nreg] (POPL, POPU), or null (LTE)
+  const double* pop;
 };
 
 __global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
@@ -326,7 +328,10 @@ __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
         const int sl = c_cfg.reg_slot0[r] + b;
         const LineConst& L = c_cfg.line[c_cfg.slot_line[sl]];
         double kl[4] = {0, 0, 0, 0}, dm[4] = {0, 0, 0, 0};
-        err = opac_line(L, T, ne, nh, s, d, a.der != 0, kl);
+        // the region's departure coefficients for every line of the region (forward.f90:949)
+        const double bl = a.pop ? a.pop[((long)0 * nz + k) * c_cfg.nreg + r] : 1.0;
+        const double bu = a.pop ? a.pop[((long)nz + k) * c_cfg.nreg + r] : 1.0;
+        err = opac_line(L, T, ne, nh, s, d, a.der != 0, kl, bl, bu);
         damping(L, T, nh, d, a.der != 0, dm);
         M(fslot(sl, 0)) = (float)kl[0];
         M(fslot(sl, 1)) = (float)dm[0];
@@ -490,6 +495,7 @@ struct RtArgs {
   // per-pixel windows (items null: every column uses every wavelength)
   const RtItem* items;   // this launch's workgroups
   const int64_t* poff;   // [ncol+1] prefix of the pixels' wavelength counts: Stokes at 4*poff[col]
+  const double* pop;     // NLTE departure coefficients [2][nz][nreg], or null (LTE)
 };
 
 enum { D_T = 0, D_P, D_R, D_V, D_B, D_G, D_F, NDIR };
@@ -699,7 +705,8 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
                                                      float* __restrict__ stokes, float* __restrict__ jac,
                                                      float* __restrict__ scratch, int ncol, int col0, int ntile,
                                                      int jcol0, const RtItem* __restrict__ items,
-                                                     const int64_t* __restrict__ poff) {
+                                                     const int64_t* __restrict__ poff,
+                                                     const double* __restrict__ pop) {
   RtArgs a;
   a.ncol = ncol;
   a.col0 = col0;
@@ -714,6 +721,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   a.scratch = scratch;
   a.items = items;
   a.poff = poff;
+  a.pop = pop;
   // parameters: 5 spherical, 7 Cartesian, 8 with P0 (the top pressure)
   constexpr int NP = (RF == FZ_RF_SPHERICAL || RF == RF_CART_TBV)
                          ? 5
@@ -782,9 +790,14 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   float Ahe[4] = {0.f, 0.f, 0.f, 0.f};
   float* scr = a.scratch + t;
   const long sstr = nthr;
+  // POPL/POPU of the wavelength's region at height k (SOURCE_FUNCTION,
+  // rtesolver.f90:1030-1033): S = c1 / ((POPL/POPU) exp(c2/T) - 1)
+  auto depart = [&](int k) -> float {
+    return a.pop ? (float)(a.pop[(long)k * c_cfg.nreg + r] / a.pop[((long)nz + k) * c_cfg.nreg + r]) : 1.0f;
+  };
   for (int k = 0; k < nz; ++k) {
     const Layer ly = load_layer(a, col, k);
-    const float S = c1 / (expf(c2 / ly.T) - 1.0f);
+    const float S = c1 / (depart(k) * expf(c2 / ly.T) - 1.0f);
     if (RF) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) scr[(long)(k * 4 + s) * sstr] = I[s];
@@ -859,7 +872,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
           for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
       } else {
-        const float ex = expf(c2 / ly.T);
+        const float ex = depart(k) * expf(c2 / ly.T);
         const float S = c1 / (ex - 1.0f);
         const float dS = c1 * c2 * ex / (ly.T * ly.T * (ex - 1.f) * (ex - 1.f));
         float x7[7], dx[NDIR][7];
@@ -1204,6 +1217,8 @@ struct fz_ctx {
   int64_t* d_poff = nullptr;
   int32_t* d_win = nullptr;         // [ncol][nreg][3]: first, count, position in the pixel's spectrum
   float* d_lsf = nullptr;     // LSF taps n * LSF_KERNEL [numw]
+  double* d_pop = nullptr;    // NLTE departure coefficients [2][nz][nreg]
+  bool pop_on = false;
   // GET_PERTURBATION / HESSIAN: normal matrix and solver work arrays, and
   // the device copies of the host-array entry points' inputs
   float* d_pert = nullptr;
@@ -1300,6 +1315,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_poff) (void)hipFree(c->d_poff);
   if (c->d_win) (void)hipFree(c->d_win);
   if (c->d_lsf) (void)hipFree(c->d_lsf);
+  if (c->d_pop) (void)hipFree(c->d_pop);
   if (c->d_pert) (void)hipFree(c->d_pert);
   if (c->d_pstage) (void)hipFree(c->d_pstage);
   c->mlp.release();
@@ -1469,10 +1485,36 @@ int fz_set_gain(fz_ctx* c, const float* gain) {
   return FZ_OK;
 }
 
+int fz_set_departure(fz_ctx* c, const double* popl, const double* popu) {
+  if (!c || !c->configured) return FZ_EARG;
+  if (!popl || !popu) {
+    c->pop_on = false;
+    return FZ_OK;
+  }
+  const int nz = c->cfg.nz, nreg = c->cfg.nreg;
+  const size_t n = (size_t)nz * nreg;
+  std::vector<double> h(2 * n);
+  for (size_t i = 0; i < n; ++i) {
+    // POPL(r, k) at r + k*nreg, the device layout [2][nz][nreg]
+    if (!(popl[i] > 0.0) || !(popu[i] > 0.0) || !std::isfinite(popl[i]) || !std::isfinite(popu[i]))
+      return set_err(c, FZ_EARG, "departure coefficients must be finite and positive");
+    h[i] = popl[i];
+    h[n + i] = popu[i];
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->d_pop) (void)hipFree(c->d_pop);
+  c->d_pop = nullptr;
+  HIPCHK(c, hipMalloc(&c->d_pop, sizeof(double) * 2 * n));
+  HIPCHK(c, hipMemcpy(c->d_pop, h.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+  c->pop_on = true;
+  return FZ_OK;
+}
+
 int fz_set_lsf(fz_ctx* c, const double* lsf_kernel, const int32_t* lsf_valid) {
   if (!c || !c->configured) return FZ_EARG;
   Cfg& g = c->cfg;
   HIPCHK(c, hipSetDevice(c->device));
+  c->pop_on = false;   // LTE until fz_set_departure (input_file.f90:504-508)
   g.lsf_on = 0;
   for (int r = 0; r < g.nreg; ++r) g.reg_lsf[r] = 0;
   if (lsf_kernel) {
@@ -1608,7 +1650,8 @@ static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, in
   const dim3 grid(a.items ? (unsigned)nitems : pack ? npack : (unsigned)a.ntile * (unsigned)c->cfg.nbpc),
       blk(RT_WAVE);
 #define FZ_RT_ARGS                                                                                               \
-  a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0, a.items, a.poff
+  a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0, a.items, a.poff, \
+      c->pop_on ? c->d_pop : nullptr
   if (pack) {
     if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
     else if (rf_mode == FZ_RF_SPHERICAL)
@@ -1692,6 +1735,7 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
     a.eos = physics ? (double*)c->d_eosb : nullptr;
     a.deriv = (der && physics) ? (double*)c->d_derb : nullptr;
     a.status = stv;
+    a.pop = c->pop_on ? c->d_pop : nullptr;
     const long nkt = (long)nz * a.ntile;
     hipLaunchKernelGGL(saha_kernel, dim3((unsigned)((nkt + 255) / 256)), dim3(256), 0, st, a);
     if (hydro) {
@@ -2643,6 +2687,53 @@ static int njevals(int apar, int curic, int maxiter, int cycpow, int nz) {
   return std::min(nz, (int)std::pow((double)cycpow, (double)(curic - 1)));
 }
 
+// SET_TAU's assists on device data (fz_assist_dev, the first cycle of fz_invert3d_dev)
+static int run_assist(fz_ctx* c, int ncol, const double* d_obs, int flags, float* d_atm, hipStream_t st) {
+  const Cfg& g = c->cfg;
+  AssistArgs a;
+  a.ncol = ncol;
+  a.nz = g.nz;
+  a.nfreq = c->lm.nfreq;
+  a.numw = g.numw;
+  a.itz = g.reg_nwave[0];
+  a.nstk = c->lm.nfreq / g.numw;
+  a.flags = flags;
+  a.kz = (int)std::ceil((double)(float)g.nz / 3.0) - 1;   // CEILING(REAL(NZ)/3.)
+  a.lk_hc = (g.reg_l0main[0] * 1.0e-8 * KBOL) / (HPLA * LIGHT);
+  a.obs = d_obs;
+  a.atm = d_atm;
+  DevBuf b_vec, b_pix, b_mx;
+  HIPCHK(c, hipMalloc(&b_vec.p, sizeof(double) * (3 * (size_t)a.itz + 1)));
+  HIPCHK(c, hipMalloc(&b_pix.p, sizeof(double) * 5 * (size_t)ncol));
+  HIPCHK(c, hipMalloc(&b_mx.p, sizeof(double) * 4));
+  a.vec = (double*)b_vec.p;
+  a.pix = (double*)b_pix.p;
+  a.mx = (double*)b_mx.p;
+  const unsigned gp = (unsigned)((ncol + 255) / 256);
+  if (flags & (ASSIST_B | ASSIST_V)) {
+    hipLaunchKernelGGL(assist_sum_kernel, dim3((unsigned)a.itz), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(assist_vec_kernel, dim3(1), dim3(64), 0, st, a);
+  } else {
+    HIPCHK(c, hipMemsetAsync(a.vec, 0, sizeof(double) * (3 * (size_t)a.itz + 1), st));
+  }
+  hipLaunchKernelGGL(assist_pix_kernel, dim3(gp), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(assist_max_kernel, dim3(1), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(assist_apply_kernel, dim3(gp), dim3(256), 0, st, a);
+  HIPCHK(c, hipGetLastError());
+  // the scratch is released at return: wait for the kernels that read it
+  HIPCHK(c, hipStreamSynchronize(st));
+  return FZ_OK;
+}
+
+extern "C" int fz_assist_dev(fz_ctx* c, int ncol, const double* d_obs, int flags, float* d_atm, void* stream) {
+  if (!c || ncol < 1 || !d_obs || !d_atm || (flags & ~15)) return FZ_EARG;
+  if (!c->inv_set) return set_err(c, FZ_ECONFIG, "fz_set_inversion first");
+  if (c->win_on) return set_err(c, FZ_EARG, "per-pixel windows: the assists read a common first region");
+  if (flags == 0) return FZ_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  return run_assist(c, ncol, d_obs, flags, d_atm, (hipStream_t)stream);
+}
+
 int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
                     float* d_stokes, double* d_chi2, int* iterations, void* stream) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
@@ -2743,8 +2834,14 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
     int r = fz_set_inversion(c, &inv, nullptr, nullptr);
     if (r) return r;
     hipLaunchKernelGGL(inv_init_kernel, dim3((ncol + 255) / 256), dim3(256), 0, st, ncol, sla, done);
-    // SET_TAU (:492-727): a synthesis updates log tau5 when Pg and P0 are not inverted
+    // SET_TAU (:492-727): a synthesis updates log tau5 when Pg and P0 are not
+    // inverted; in the first cycle the assist initialisers change the start
+    // model before it
     if (!ctl->atmpar[1] && !ctl->atmpar[7]) {
+      if (cyc == cyc0 && ctl->assist && !c->win_on) {
+        r = run_assist(c, ncol, d_obs, ctl->assist & 15, d_atm, st);
+        if (r) return r;
+      }
       r = fz_forward_dev(c, ncol, d_atm, FZ_RF_NONE, d_stokes, nullptr, nullptr, stream);
       if (r) return r;
     }

@@ -37,7 +37,7 @@ struct GramArgs {
 // v_mfma_f64_16x16x4_f64 operand and result layout (64 lanes, l = lane):
 //   A (16x4): lane l holds A[l % 16][l / 16]
 //   B (4x16): lane l holds B[l / 16][l % 16]
-//   D (16x16): lane l holds D[4 (l / 16) + r][l % 16] in element r = 0..3
+//   D (16x16): lane l holds D[4 r + l / 16][l % 16] in element r = 0..3
 __global__ void __launch_bounds__(GR_THREADS) gram_kernel(GramArgs a) {
   const int ti = blockIdx.y, tj = blockIdx.x;
   if (tj < ti) return;   // lower triangle: mirrored by the reader
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(GR_THREADS) gram_kernel(GramArgs a) {
   const int j = j0 + wj + (l & 15);
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int i = i0 + wi + 4 * (l >> 4) + r;
+    const int i = i0 + wi + 4 * r + (l >> 4);
     if (i < a.np && j < ncolx && j >= i) a.gram[(long)i * (a.np + 1) + j] = acc[r];
   }
 }

@@ -53,8 +53,11 @@ class CInvertControl(ctypes.Structure):
                 ("maxsteps", ctypes.c_int32), ("solver", ctypes.c_int32), ("node_expansion", ctypes.c_int32),
                 ("regul", ctypes.c_int32), ("pen_typ", ctypes.c_int32 * 8), ("pen_alp", ctypes.c_double * 8),
                 ("svdtol", ctypes.c_double), ("max_iterations", ctypes.c_int32), ("smoothing", ctypes.c_int32),
-                ("first_cycle", ctypes.c_int32)]
+                ("first_cycle", ctypes.c_int32), ("assist", ctypes.c_int32)]
 
+
+# SET_TAU assist initialisers (fz_invert_control.assist, fz_assist_dev)
+ASSIST_B, ASSIST_V, ASSIST_T, ASSIST_P = 1, 2, 4, 8
 
 # inverted physical parameters, in the reference's INV_ATMPAR order
 INV_PARAMS = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "P0")
@@ -90,6 +93,8 @@ def library():
     L.fz_set_tile.argtypes = [vp, ip, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_lsf.argtypes = [vp, vp, vp]
+    L.fz_set_departure.argtypes = [vp, vp, vp]
+    L.fz_assist_dev.argtypes = [vp, ip, vp, ip, vp, vp]
     L.fz_set_windows.argtypes = [vp, ip, vp]
     L.fz_window_offsets.argtypes = [vp, vp]
     L.fz_synchronize.argtypes = [vp]
@@ -131,7 +136,7 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
-            "fz_set_hydrostatic", "fz_set_lsf", "fz_set_windows", "fz_window_offsets",
+            "fz_set_hydrostatic", "fz_set_lsf", "fz_set_departure", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d",
             "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
@@ -142,8 +147,9 @@ def exported_symbols():
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
                    maxsteps=5, solver=1, node_expansion=1, svdtol=0.0, max_iterations=0, smoothing=True,
                    regul=False, pen_typ: Optional[dict] = None, pen_alp: Optional[dict] = None,
-                   first_cycle=1) -> CInvertControl:
-    """fz_invert_control from {parameter name: maximum nodes}, e.g. {"T": 4, "vz": 4}."""
+                   first_cycle=1, assist=0) -> CInvertControl:
+    """fz_invert_control from {parameter name: maximum nodes}, e.g. {"T": 4, "vz": 4};
+    assist: ASSIST_* bits (SET_TAU's initialisers of the first cycle)."""
     c = CInvertControl()
     for name, n in max_nodes.items():
         i = INV_PARAMS.index(name)
@@ -154,6 +160,7 @@ def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.
     c.noise, c.cycpow, c.maxsteps, c.solver, c.node_expansion = noise, cycpow, maxsteps, solver, node_expansion
     c.regul, c.svdtol, c.max_iterations, c.smoothing = int(regul), svdtol, max_iterations, int(smoothing)
     c.first_cycle = int(first_cycle)
+    c.assist = int(assist)
     for name, v in (pen_typ or {}).items():
         c.pen_typ[INV_PARAMS.index(name)] = int(v)
     for name, v in (pen_alp or {}).items():
@@ -262,6 +269,19 @@ class Context:
         v = None if valid is None else np.ascontiguousarray(valid, dtype=np.int32)
         self._check(self.lib.fz_set_lsf(self.h, k.ctypes.data, None if v is None else v.ctypes.data), "fz_set_lsf")
         self._lsf = (k, v)
+
+    def set_departure(self, popl=None, popu=None):
+        """NLTE departure coefficients POPL, POPU as arrays (nreg, nz)
+        (input_file.f90:2146-2166); None: LTE."""
+        if popl is None:
+            self._check(self.lib.fz_set_departure(self.h, None, None), "fz_set_departure")
+            return
+        self._pop = (np.asfortranarray(popl, dtype=np.float64), np.asfortranarray(popu, dtype=np.float64))
+        if self._pop[0].shape != (len(self.lineset.regions), self.nz) or self._pop[1].shape != self._pop[0].shape:
+            raise FzError("departure coefficients must be (nregions, nz) = (%d, %d)"
+                          % (len(self.lineset.regions), self.nz))
+        self._check(self.lib.fz_set_departure(self.h, self._pop[0].ctypes.data, self._pop[1].ctypes.data),
+                    "fz_set_departure")
 
     def set_windows(self, windows=None):
         """Per-pixel spectral windows: windows int [ncol][nreg][2] = (first
@@ -470,6 +490,12 @@ class Context:
         self._check(self.lib.fz_mlp_train_step_dev(self.h, idx.shape[0], x.data_ptr(), t.data_ptr(), idx.data_ptr(),
                                                    float(lr), loss.data_ptr(), self._stream(x)),
                     "fz_mlp_train_step_dev")
+
+    def assist_torch(self, obs, atm, flags):
+        """SET_TAU's assist initialisers (fz_assist_dev) on device tensors:
+        obs [ncol][NFREQ] float64 (after set_inversion), atm [12][nz][ncol]."""
+        self._check(self.lib.fz_assist_dev(self.h, obs.shape[0], obs.data_ptr(), int(flags), atm.data_ptr(),
+                                           self._stream(atm)), "fz_assist_dev")
 
     def learned_init_torch(self, obs, atm):
         """Write the network's T, Bx, By, Bz, vz for observations obs

@@ -200,3 +200,33 @@ def read_lsf(path, nwave=None):
     if data.size != nw:
         raise ValueError("%s: %d samples, header says %d" % (path, data.size, nw))
     return data.astype(np.float64)
+
+
+def write_departure(path, popl, popu):
+    """NLTE departure-coefficient file of one line: one Fortran unformatted
+    record (BETAL, BETAU) of two REAL*8 per height, bottom first, as
+    READ_DEPARTURE_COEFFICIENTS reads it (input_file.f90:2146-2166)."""
+    popl = np.asarray(popl, np.float64).reshape(-1)
+    popu = np.asarray(popu, np.float64).reshape(-1)
+    with open(path, "wb") as f:
+        n = np.array([16], np.int32).tobytes()
+        for bl, bu in zip(popl, popu):
+            f.write(n + np.array([bl, bu], np.float64).tobytes() + n)
+
+
+def read_departure(path, nz):
+    """(BETAL[nz], BETAU[nz]) of a departure-coefficient file; the file must
+    hold at least nz records of two REAL*8 (the reference reads NZ of them)."""
+    raw = open(path, "rb").read()
+    bl, bu, pos = [], [], 0
+    for k in range(nz):
+        if pos + 4 > len(raw):
+            raise ValueError("%s: %d records, the model has %d heights" % (path, k, nz))
+        n = int(np.frombuffer(raw, np.int32, 1, pos)[0])
+        if n < 16:
+            raise ValueError("%s: record %d holds %d bytes, two REAL*8 expected" % (path, k + 1, n))
+        v = np.frombuffer(raw, np.float64, 2, pos + 4)
+        bl.append(float(v[0]))
+        bu.append(float(v[1]))
+        pos += 4 + n + 4
+    return np.array(bl), np.array(bu)

@@ -85,6 +85,18 @@ int fz_set_gain(fz_ctx* ctx, const float* gain);
  * fz_configure resets it. */
 int fz_set_lsf(fz_ctx* ctx, const double* lsf_kernel, const int32_t* lsf_valid);
 
+/* NLTE departure coefficients (POPL, POPU of FORWARD_PARAM,
+ * forward_param.f90:47, as READ_DEPARTURE_COEFFICIENTS fills them,
+ * input_file.f90:2146-2166): popl POPL(NUML,NZ), popu POPU(NUML,NZ)
+ * column-major double (element (r,k) at r + k*nreg) for the configured
+ * regions and heights.  Every later forward uses them in the line opacity of
+ * every line of region r (OPAC_LINE(POPL(r,k), POPU(r,k), ...),
+ * forward.f90:949, line_opacity.f90:101-106) and in the source function of
+ * the region's wavelengths (rtesolver.f90:1030-1064).  NULL: LTE (the
+ * reference's default of 1, input_file.f90:504-508); fz_configure resets to
+ * LTE.  Values must be finite and positive. */
+int fz_set_departure(fz_ctx* ctx, const double* popl, const double* popu);
+
 /* Per-pixel spectral windows: each of the ncol pixels observes its own
  * sub-range of every configured region.  The reference reads one set of
  * regions for the whole map (NREAD_LINES, input_file.f90:253-367: NUML
@@ -299,7 +311,22 @@ typedef struct {
   int32_t smoothing;     /* SMOOTHING_B at every cycle start (the reference always applies it) */
   int32_t first_cycle;   /* first cycle to run, 0 -> 1 (a start model already on the final node
                             basis can skip the coarse cycles); no counterpart in the reference */
+  int32_t assist;        /* SET_TAU assist initialisers of the first cycle (MISC SETUP TEMP, PGAS,
+                            BVEC, VLOS: ASSIST_T/P/B/V, input_file.f90:170-176): FZ_ASSIST_* bits */
 } fz_invert_control;
+
+enum { FZ_ASSIST_B = 1, FZ_ASSIST_V = 2, FZ_ASSIST_T = 4, FZ_ASSIST_P = 8 };
+
+/* SET_TAU's assist initialisers (invert3d.f90:528-700) on device data, for
+ * hosts that keep their own cycle loop: from the observations d_obs
+ * [ncol][NFREQ] (the inverted Stokes blocks, as fz_invert3d_dev takes them;
+ * the first region's wavelengths are read) they overwrite rows of the start
+ * model d_atm [12][nz][ncol]: FZ_ASSIST_B Bx, By, Bz (needs I, Q, U, V
+ * inverted), FZ_ASSIST_V v_z, FZ_ASSIST_T T (shifted to the brightness
+ * temperature at height ceiling(nz/3)), FZ_ASSIST_P Pg (scaled to 1.333e5 there);
+ * T and P also reset log tau5 as the reference does before its synthesis.
+ * After fz_set_inversion (for the inverted Stokes parameters). */
+int fz_assist_dev(fz_ctx* ctx, int ncol, const double* d_obs, int flags, float* d_atm, void* stream);
 
 /* INVERT3D (invert3d.f90:57-178) on device data: the cycle schedule (nodes
  * per cycle from GET_NJEVALS, get_dmodel.f90:375-407), SET_TAU, the 1-2-1

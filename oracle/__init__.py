@@ -99,6 +99,7 @@ class Cpu:
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
         L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
         L.fzc_set_lsf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.fzc_set_departure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
         L.fzc_int_coefs.argtypes = [ctypes.c_int, ctypes.c_int, _dp]
         L.fzc_njevals.argtypes = [ctypes.c_int] * 6
@@ -185,6 +186,16 @@ class Cpu:
     def set_gain(self, g):
         g = np.ascontiguousarray(np.asarray(g, dtype=np.float32).T)
         self.lib.fzc_set_gain(_ptr(g, _fp))
+
+    def set_departure(self, popl=None, popu=None):
+        """NLTE departure coefficients POPL, POPU as arrays (nreg, nz)
+        (input_file.f90:2146-2166); None: LTE."""
+        if popl is None:
+            self.lib.fzc_set_departure(0, 0, None, None)
+            return
+        pl = np.asfortranarray(popl, dtype=np.float64)
+        pu = np.asfortranarray(popu, dtype=np.float64)
+        self.lib.fzc_set_departure(pl.shape[0], pl.shape[1], pl.ctypes.data, pu.ctypes.data)
 
     def set_hydrostatic(self, on=True):
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside forward1d
@@ -439,6 +450,18 @@ class Ref:
         f = self.lib.fzref_set_hydro
         f.argtypes = [ctypes.c_int]
         f(int(on))
+
+    def set_departure(self, popl=None, popu=None):
+        """POPL, POPU (nreg, nz) for the reference's FORWARD_PARAM arrays; None: LTE."""
+        f = self.lib.fzref_set_departure
+        f.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        if popl is None:
+            one = np.ones(1)
+            f(0, 0, one.ctypes.data, one.ctypes.data)
+            return
+        self._pl = np.asfortranarray(popl, dtype=np.float64)
+        self._pu = np.asfortranarray(popu, dtype=np.float64)
+        f(self._pl.shape[0], self._pl.shape[1], self._pl.ctypes.data, self._pu.ctypes.data)
 
     def forward1d(self, model):
         m = np.asfortranarray(np.asarray(model, dtype=np.float32).T.copy())   # (nz,13) Fortran

@@ -32,6 +32,7 @@ static double DENO_SA, DENO_SAM;
 static double ABFAC[FZ_NELEM]; /* 10**(ABUND-12) */
 
 enum { E_OK = 0, E_EOS = 1, E_NAN = 2, E_ION = 3, E_LAMBDA = 4, E_CONFIG = 5 };
+int fzc_set_departure(int nreg, int nz, const double* popl, const double* popu);
 
 /* ------------------------------------------------ configuration (read-only) */
 static fzc_line LN[FZC_MAXLINES];
@@ -782,6 +783,7 @@ int fzc_zeeman(int line, double* shift, double* strength, int32_t* type) {
 /* ------------------------------------------------ setup */
 int fzc_setup(const fzc_line* lines, int nlines, const fzc_region* regs, int nreg, int resp,
               const int32_t* atmpar) {
+  fzc_set_departure(0, 0, NULL, NULL);   /* LTE until set (input_file.f90:504-508) */
   init_constants();
   if (nlines > FZC_MAXLINES || nreg > FZC_MAXREG || nlines < 1 || nreg < 1) return -E_CONFIG;
   NLN = nlines;
@@ -1291,22 +1293,56 @@ int fzc_evol(const double* x7, double dz_km, const double* dx, double* evol, dou
   return e;
 }
 
+/* ------------------------------------------------ NLTE departure coefficients
+ * POPL(NUML,NZ), POPU(NUML,NZ) (forward_param.f90:47), 1 unless set
+ * (input_file.f90:504-508; READ_DEPARTURE_COEFFICIENTS :2146-2166): per
+ * spectral region and height, used by OPAC_LINE for every line of the region
+ * (forward.f90:949) and by SOURCE_FUNCTION(_DER) (rtesolver.f90:1030-1064). */
+static double* POPL_A = NULL;
+static double* POPU_A = NULL;
+static int POP_NZ = 0, POP_NREG = 0;
+
+int fzc_set_departure(int nreg, int nz, const double* popl, const double* popu) {
+  free(POPL_A);
+  free(POPU_A);
+  POPL_A = POPU_A = NULL;
+  POP_NZ = POP_NREG = 0;
+  if (!popl || !popu) return 0;
+  POPL_A = (double*)malloc(sizeof(double) * nreg * nz);
+  POPU_A = (double*)malloc(sizeof(double) * nreg * nz);
+  memcpy(POPL_A, popl, sizeof(double) * nreg * nz);
+  memcpy(POPU_A, popu, sizeof(double) * nreg * nz);
+  POP_NZ = nz;
+  POP_NREG = nreg;
+  return 0;
+}
+
+static double popl_at(int r, int k) { return POPL_A ? POPL_A[r + (size_t)k * POP_NREG] : 1.0; }
+static double popu_at(int r, int k) { return POPU_A ? POPU_A[r + (size_t)k * POP_NREG] : 1.0; }
+
 /* ------------------------------------------------ source function (rtesolver.f90:1008-1066) */
+static int region_of(int l) {
+  for (int m = 0; m < NRG; ++m)
+    if (l >= PIX0[m] && l < PIX1[m]) return m;
+  return 0;
+}
+
 static double region_lambda_cm(int l) {
   for (int m = 0; m < NRG; ++m)
     if (l >= PIX0[m] && l < PIX1[m]) return (LN[RG[m].line[0]].lambda0 + WAVE[l] / 1.0e3) * 1.0e-8;
   return 0.0;
 }
 
-static double source_fn(double lam, float Tf) {
+static double source_fn(double lam, float Tf, double bl, double bu) {
   return 2.0 * HPLA * pow(LIGHT, 2.0) / pow(lam, 5.0) *
-         (1.0 / ((1.0 / 1.0) * exp(HPLA * LIGHT / (lam * KBOL * (double)Tf)) - 1.0));
+         (1.0 / ((bl / bu) * exp(HPLA * LIGHT / (lam * KBOL * (double)Tf)) - 1.0));
 }
 
-static double source_fn_der(double lam, float Tf) {
+static double source_fn_der(double lam, float Tf, double bl, double bu) {
   const double FACTOR = 8.5685641e-6;
   double T = (double)Tf, x = exp(HPLA * LIGHT / (lam * KBOL * T));
-  return (double)(float)((2.0 * FACTOR / (pow(lam, 6.0) * pow(T, 2.0))) * ((1.0 * x) / pow(1.0 * x - 1.0, 2.0)));
+  return (double)(float)((2.0 * FACTOR / (pow(lam, 6.0) * pow(T, 2.0))) *
+                         (((bl / bu) * x) / pow((bl / bu) * x - 1.0, 2.0)));
 }
 
 /* ------------------------------------------------ GAIN1D from HSRA (normalization.f90:696-754) */
@@ -1366,6 +1402,7 @@ int fzc_gain(float* gain) {
 
 /* ------------------------------------------------ one column (forward.f90:857-1029) */
 int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, double* kc5_out) {
+  if (POPL_A && (POP_NZ != nz || POP_NREG != NRG)) return E_CONFIG;
   float *TEM = model, *PG = model + nz, *RHO = model + 2 * nz, *PEL = model + 3 * nz, *MW = model + 4 * nz;
   float *BX = model + 5 * nz, *BY = model + 6 * nz, *BZ = model + 7 * nz, *VZ = model + 10 * nz;
   float* TAU5 = model + 11 * nz;
@@ -1414,7 +1451,7 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
         for (int b = 0; b < RG[r].nblend && !err; ++b) {
           double kl;
           derivvar_t dvb = dvr;
-          err = opac_line(1.0, 1.0, RG[r].line[b], TEM[k], ne, nh, &kl, &dvb, s);
+          err = opac_line(popl_at(r, k), popu_at(r, k), RG[r].line[b], TEM[k], ne, nh, &kl, &dvb, s);
           if (err) break;
           double damp = damping(RG[r].line[b], TEM[k], nh, &dvb);
           err = abs_mat_block(k, r, b, TEM[k], &BX[k], &BY[k], &BZ[k], &VZ[k], kc, kl, damp, &dvb, &A);
@@ -1424,7 +1461,8 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
     }
     for (int l = 0; l < NUMW; ++l) {
       double lam = region_lambda_cm(l);
-      double sf = source_fn(lam, TEM[k]);
+      const int rg = region_of(l);
+      double sf = source_fn(lam, TEM[k], popl_at(rg, k), popu_at(rg, k));
       double* I = &syn[4 * l];
       if (k == 0 || kc5[k] >= THR) { /* SET_BOUNDARY (rtesolver.f90:1134-1147) */
         I[0] = sf;
@@ -1454,7 +1492,7 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
       if (!RESP) continue;
       for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) evg[(((size_t)k * NUMW + l) * 4 + i) * 4 + j] = E.evol[i][j];
-      double dsf = source_fn_der(lam, TEM[k]);
+      double dsf = source_fn_der(lam, TEM[k], popl_at(rg, k), popu_at(rg, k));
       /* map evolution-operator derivatives to the DSYN1D parameter slots:
        * 1 T|Pg, 2 Pg|T, 3 rho|T, 4 Bx, 5 By, 6 Bz, 7 vlos, 8 P0 */
       const int slot_src[7] = {P_TPG, P_PGT, P_RHOT, P_BX, P_BY, P_BZ, P_VLOS};

@@ -66,6 +66,9 @@ int fzc_set_lsf(const double* lsf_kernel, const int32_t* valid);
  * syn[4*numw]  = SYN1D*GAIN1D;
  * dsyn[4*8*numw*nz] = propagated DSYN1D*GAIN1D (only if resp);
  * kc5[nz]. Returns 0 or an error code. */
+/* NLTE departure coefficients POPL(NUML,NZ), POPU(NUML,NZ) column-major
+ * (element (r,k) at r + k*nreg); NULL: LTE.  fzc_setup resets them. */
+int fzc_set_departure(int nreg, int nz, const double* popl, const double* popu);
 int fzc_forward1d(int nz, float* model, double* syn, double* dsyn, double* kc5);
 
 /* ncol independent columns, OpenMP over columns (nthreads<=0: default). */

@@ -150,3 +150,64 @@ def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 
         cur = best.copy()
         syn, _ = forward_all(cur, False)
     return cur, syn, sla[:, 2].copy(), total
+
+
+# SET_TAU's assist initialisers (invert3d.f90:528-700)
+ASSIST_B, ASSIST_V, ASSIST_T, ASSIST_P = 1, 2, 4, 8
+KBOL, HPLA, LIGHT = 1.38056503e-16, 6.62606957e-27, 2.99792458e10   # cons_param.f90:18-25
+
+
+def assist(obs, atm, flags, numw, itz, nstk, lambda0):
+    """The assists on a start model: obs (ncol, nfreq) float64 (the inverted
+    Stokes blocks of NUMW each), atm (12, nz, ncol) float32 (a modified copy
+    is returned); itz wavelengths of the first region, its main line at
+    lambda0 (A).  Steps and REAL(4) roundings of invert3d.f90:528-700."""
+    atm = np.array(atm, dtype=np.float32, copy=True)
+    nz, ncol = atm.shape[1], atm.shape[2]
+    f32 = np.float32
+    first = obs[:, :itz]
+
+    def norm(v):
+        return v / np.sum(np.abs(v))
+
+    def cdiff(v):
+        d = np.empty_like(v)
+        d[1:-1] = (v[2:] - v[:-2]) / 2.0
+        d[0], d[-1] = d[1], d[-2]
+        return d
+
+    if flags & (ASSIST_B | ASSIST_V):
+        v0 = norm(first.sum(axis=0))
+        v1 = norm(cdiff(v0))
+        v2 = norm(cdiff(v1))
+    kz = int(math.ceil(float(f32(nz)) / 3.0)) - 1          # CEILING(REAL(NZ)/3.)
+    if flags & ASSIST_B and nstk == 4:
+        t3 = obs[:, 3 * numw:3 * numw + itz] @ v1
+        t1 = obs[:, numw:numw + itz] @ v2
+        t2 = obs[:, 2 * numw:2 * numw + itz] @ v2
+        t3 = t3 / np.max(np.abs(t3))
+        t4 = np.sqrt(t1 ** 2 + t2 ** 2)
+        t4 = t4 / np.max(t4)
+        atn2 = np.arctan2(t2, t1) / 2.0
+        bx = ((-np.sin(atn2).astype(f32) * f32(1000.0)).astype(np.float64) * t4).astype(f32)
+        by = ((np.cos(atn2).astype(f32) * f32(1000.0)).astype(np.float64) * t4).astype(f32)
+        bz = t3.astype(f32) * f32(-1000.0)
+        atm[5], atm[6], atm[7] = bx[None, :], by[None, :], bz[None, :]
+    if flags & ASSIST_V:
+        kol = float(v1 @ v0)
+        tv = first @ v1 - (first @ v0) * kol
+        tv = tv / np.max(np.abs(tv))
+        atm[10] = (tv.astype(f32) * f32(-2.0e5))[None, :]
+    if flags & (ASSIST_T | ASSIST_P):
+        atm[11] = 1.0
+        atm[11, kz] = 0.0
+    if flags & ASSIST_T:
+        lk = (lambda0 * 1.0e-8 * KBOL) / (HPLA * LIGHT)
+        mean = first.sum(axis=1) / float(itz)
+        tb = 1.0 / (lk * np.log(1.0 + 1.0 / mean * (np.exp(1.0 / 6.39e3 / lk) - 1.0)))
+        dt = tb - atm[0, kz].astype(np.float64)
+        atm[0] = (atm[0].astype(np.float64) + dt[None, :]).astype(f32)
+    if flags & ASSIST_P:
+        pz = atm[1, kz].copy()
+        atm[1] = atm[1] * f32(1.333e5) / pz[None, :]
+    return atm

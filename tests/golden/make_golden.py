@@ -16,6 +16,10 @@ Fixtures (tests/golden/*.npz, data only):
   seeded_c1.npz    a 4x4 patch of the seeded 16x16x64 cube (2 lines, 112
                    wavelengths): hydrostatic Pg, Stokes for 16 columns and
                    the full response functions of 2 columns
+  nlte.npz         NLTE departure coefficients: two unblended regions
+                   (Fe I 6301.5 with height-dependent POPL/POPU, Fe I 6302.5
+                   in LTE) on a seeded hydrostatic column: Stokes and the
+                   seven Cartesian response functions (every 4th height)
   he.npz           HYDROSTATIC: yes (tests/test_he, test_syn2) on
                    default/test_atmos.bin, lines 4,5 at 125 wavelengths: the
                    re-integrated model, Stokes and all eight response
@@ -166,15 +170,50 @@ def he(r):
                         wave_step=20.0)
 
 
+def departure(nz):
+    """Smooth NLTE departure coefficients of two regions: region 0 departs
+    (lower level over-, upper level under-populated at the top), region 1 is
+    LTE."""
+    x = np.linspace(0.0, 1.0, nz)
+    popl = np.ones((2, nz))
+    popu = np.ones((2, nz))
+    popl[0] = 1.0 + 0.25 * x ** 2
+    popu[0] = 1.0 - 0.35 * x ** 1.5
+    return popl, popu
+
+
+def nlte(r):
+    ls = lines.setup([((4,), 56, -280.0, 10.0), ((5,), 56, -280.0, 10.0)])
+    r.setup(ls, resp=True, atmpar=CART)
+    g = r.gain()
+    r.set_gain(g)
+    atm, z = atmos.seeded(2, 2, 64, seed=5)
+    m = atmos.column_model(atm, z, 3)
+    pg, rho, pel, mw, nh = r.he(m[0], float(m[1, -1]), z)
+    m[1], m[2], m[3], m[4] = pg, rho, pel, mw
+    m[5] += 400.0
+    m[7] += 600.0
+    popl, popu = departure(m.shape[1])
+    r.set_departure(popl, popu)
+    try:
+        mo, syn, dsyn, kc5 = r.forward1d(m)
+    finally:
+        r.set_departure(None)
+    keep = np.arange(0, m.shape[1], 4)
+    np.savez_compressed(os.path.join(HERE, "nlte.npz"), model_in=m, model_out=mo, gain=g, stokes=syn,
+                        rf_heights=keep, rf=dsyn[keep][:, :, :7, :].astype(np.float32), popl=popl, popu=popu,
+                        nwave=56, wave_start=-280.0, wave_step=10.0)
+
+
+CASES = {"units": units, "syn1": syn1, "rf": rf, "seeded": seeded, "he": he, "nlte": nlte}
+
+
 def main():
     if not oracle.ref_available():
         oracle.build_ref()
     r = oracle.Ref()
-    units(r)
-    syn1(r)
-    rf(r)
-    seeded(r)
-    he(r)
+    for name in (sys.argv[1:] or CASES):
+        CASES[name](r)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

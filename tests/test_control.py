@@ -126,7 +126,7 @@ def test_reader_errors():
     with pytest.raises(NotImplementedError):
         control.parse("BOX:\n1 1\n1 1\n1 1\nFILEMODEL:\nm\nMODE:\ntau\n")
     with pytest.raises(NotImplementedError):
-        control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nTEMP\n")
+        control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nFULL_STOKES\n")
 
 
 def test_line_spread_function_section(tmp_path):
@@ -151,3 +151,44 @@ def test_line_spread_function_section(tmp_path):
         control.parse(base + "LINE SPREAD FUNCTION:\n0.0 3.0\n1 1\n1 1\nEND:\n")
     with pytest.raises(ValueError):
         control.parse(base + "LINE SPREAD FUNCTION:\n20.0 3.0\nEND:\n")
+
+
+def test_nlte_departure_section(tmp_path):
+    """NSET_NLTE / READ_DEPARTURE_COEFFICIENTS (input_file.f90:1979-2166):
+    "<line> <file|LTE>" rows; the file's (BETAL, BETAU) per height go to the
+    region whose LINES row names that line alone; a blended NLTE line and a
+    line not under LINES are errors."""
+    from firtez_dz_amd import control, modelio
+    base = ("LINES:\n4 60 -300.0 10.0\n5 40 -200.0 10.0\nFILEPROFILE:\np.bin\nFILEMODEL:\nm.bin\n"
+            "MODE:\nsynthesis\nMISC SETUP:\nDATAPATH ./\n")
+    nz = 16
+    bl, bu = np.linspace(0.8, 1.3, nz), np.linspace(1.1, 0.7, nz)
+    modelio.write_departure(str(tmp_path / "b5.bin"), bl, bu)
+    c = control.parse(base + "NLTE DEPARTURE COEFFICIENTS:\n4 LTE\n5 b5.bin\nEND:\n")
+    assert c.nlte == [(4, "LTE"), (5, "b5.bin")]
+    popl, popu = control.departure_arrays(c, nz, str(tmp_path))
+    assert np.all(popl[0] == 1) and np.all(popu[0] == 1)
+    assert np.array_equal(popl[1], bl) and np.array_equal(popu[1], bu)
+    blended = base.replace("4 60 -300.0 10.0\n5 40", "4,5 60 -300.0 10.0\n5 40")
+    with pytest.raises(ValueError, match="blended"):
+        control.departure_arrays(control.parse(blended + "NLTE DEPARTURE COEFFICIENTS:\n5 b5.bin\nEND:\n"),
+                                 nz, str(tmp_path))
+    with pytest.raises(ValueError, match="not under LINES"):
+        control.departure_arrays(control.parse(base + "NLTE DEPARTURE COEFFICIENTS:\n2 b5.bin\nEND:\n"),
+                                 nz, str(tmp_path))
+    with pytest.raises(ValueError):
+        modelio.read_departure(str(tmp_path / "b5.bin"), nz + 1)
+
+
+def test_misc_setup_assists():
+    """MISC SETUP TEMP / PGAS / BVEC / VLOS set ASSIST_T/P/B/V
+    (input_file.f90:170-176); FULL_STOKES is still refused."""
+    from firtez_dz_amd import control, gpu
+    base = ("LINES:\n4,5 112 -624.0 20.0\nFILEPROFILE:\np.bin\nFILEMODEL:\nm.bin\nMODE:\ninversion\n"
+            "STOKES SETUP:\nSTKI\nSTKQ\nSTKU\nSTKV\nINVERSION SETUP:\nTEM 4\nBZ 1\n")
+    c = control.parse(base + "MISC SETUP:\nBVEC\nVLOS\nTEMP\nEND:\n")
+    assert c.assist == gpu.ASSIST_B | gpu.ASSIST_V | gpu.ASSIST_T
+    assert control.control_for_inversion(c, 64).assist == c.assist
+    assert control.parse(base + "MISC SETUP:\nPGAS\nEND:\n").assist == gpu.ASSIST_P
+    with pytest.raises(NotImplementedError):
+        control.parse(base + "MISC SETUP:\nFULL_STOKES\nEND:\n")
