@@ -140,12 +140,21 @@ __global__ void __launch_bounds__(256) saha_kernel(MicroTile a) {
 // cannot fill the SIMDs.
 constexpr int RHO_G = 8;
 constexpr int RHO_NS = (FZ_NELEM + RHO_G - 1) / RHO_G;
+// 32 columns per workgroup: the LDS rows of the groups (48.5 KB) fit three
+// workgroups per CU
+constexpr int RHO_BLOCK = 32 * RHO_G;
 
 // sh: this group's LDS rows, nii of element j at sh[j], niii at sh[RHO_LD + j],
 // ni of element 0 at sh[FZ_NELEM].  The lanes of a group belong to one wave,
 // whose LDS operations complete in program order, so the fold reads what the
 // stage pass wrote without a workgroup barrier.
 constexpr int RHO_LD = 96;
+// Row stride of the groups: the 64 / RHO_G groups of a wave read the same
+// element index of their own rows at once (every lane of a group the same
+// address, a broadcast); a stride of 2 RHO_LD + RHO_G / 4 doubles puts those
+// rows 32 / (64 / RHO_G) banks apart, so the reads of a wave hit distinct
+// banks instead of all one.
+constexpr int RHO_ROW = 2 * RHO_LD + RHO_G / 4;
 
 __device__ inline int get_rho_group(const double (&a1)[RHO_NS], const double (&a2)[RHO_NS], int q, double* sh,
                                     double T, double PG, double& ne, double& nh, double& pelec, double& dens,
@@ -222,8 +231,8 @@ __device__ __noinline__ RhoOut rho_group_at(const double* __restrict__ base, lon
   return o;
 }
 
-__global__ void __launch_bounds__(256) rho_group_kernel(MicroTile a) {
-  __shared__ double shall[256 / RHO_G][2 * RHO_LD];
+__global__ void __launch_bounds__(RHO_BLOCK) rho_group_kernel(MicroTile a) {
+  __shared__ double shall[RHO_BLOCK / RHO_G][RHO_ROW];
   const int lane = threadIdx.x & 63;
   const int q = lane & (RHO_G - 1);
   double* sh = shall[threadIdx.x / RHO_G];
@@ -408,8 +417,8 @@ __device__ __forceinline__ double he_step_rhs_group(const double* base, long str
 // integration scalars are identical in the lanes of a group.  Writes Pg, rho,
 // Pe, mu of every height and, for the forward, (n_e, N_H) with n_e = Pe/(k T)
 // from the REAL(4) Pe and T (forward.f90:904).
-__global__ void __launch_bounds__(256) he_rk4_group_kernel(MicroTile a, const double* __restrict__ smid) {
-  __shared__ double shall[256 / RHO_G][2 * RHO_LD];
+__global__ void __launch_bounds__(RHO_BLOCK) he_rk4_group_kernel(MicroTile a, const double* __restrict__ smid) {
+  __shared__ double shall[RHO_BLOCK / RHO_G][RHO_ROW];
   const int lane = threadIdx.x & 63;
   const int q = lane & (RHO_G - 1);
   double* sh = shall[threadIdx.x / RHO_G];
@@ -1742,10 +1751,12 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
       const long nmt = (long)(nz - 1) * a.ntile;
       hipLaunchKernelGGL(saha_mid_kernel, dim3((unsigned)((nmt + 255) / 256)), dim3(256), 0, st, a,
                          (double*)c->d_smid);
-      hipLaunchKernelGGL(he_rk4_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + 255) / 256)), dim3(256), 0,
+      hipLaunchKernelGGL(he_rk4_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + RHO_BLOCK - 1) / RHO_BLOCK)),
+                         dim3(RHO_BLOCK), 0,
                          st, a, (const double*)c->d_smid);
     } else {
-      hipLaunchKernelGGL(rho_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + 255) / 256)), dim3(256), 0, st,
+      hipLaunchKernelGGL(rho_group_kernel, dim3((unsigned)(((long)a.ntile * RHO_G + RHO_BLOCK - 1) / RHO_BLOCK)),
+                         dim3(RHO_BLOCK), 0, st,
                          a);
     }
     if (physics) {

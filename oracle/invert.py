@@ -73,10 +73,12 @@ def smooth(a, nx, ny):
 
 def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 1, 1, 1), noise=4e-3, cycpow=2,
              maxsteps=5, solver=1, node_expansion=1, svdtol=1e-4, smoothing=True, max_iterations=1000,
-             first_cycle=1):
-    """cpu: oracle.Cpu set up with resp=True and all Cartesian slots.
-    max_nodes: {physical index 0..6: NSLB_MAX}.  Returns (model, stokes
-    [ncol,4,numw], chi2 [ncol], iterations)."""
+             first_cycle=1, regul=False, pen_typ=None, pen_alp=None):
+    """cpu: oracle.Cpu set up with resp=True and all Cartesian slots (and
+    set_hydrostatic for HYDROSTATIC: yes).  max_nodes: {physical index 0..6:
+    NSLB_MAX}; pen_typ / pen_alp: {physical index: PEN_TYP / PEN_ALP} of the
+    Tikhonov terms when regul.  Returns (model, stokes [ncol,4,numw], chi2
+    [ncol], iterations)."""
     cur = np.array(atm, np.float32, copy=True)
     nz, ncol = cur.shape[1], cur.shape[2]
     numw = cpu.numw
@@ -108,8 +110,13 @@ def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 
         nodes = [0] * 8
         for p in vmax:
             nodes[p] = min(njevals(p + 1, cyc, maxiter, cycpow, nz), vmax[p])
+        ptyp, palp = [0] * 8, [0.0] * 8
+        for p, v in (pen_typ or {}).items():
+            ptyp[p] = int(v)
+        for p, v in (pen_alp or {}).items():
+            palp[p] = float(v)
         cfg = lm_cfg(nz, numw, atmpar, nodes, invstk=invstk, solver=solver, node_expansion=node_expansion,
-                     svdtol=svdtol, inu=nfreq)
+                     svdtol=svdtol, inu=nfreq, regul=regul, pen_typ=ptyp, pen_alp=palp, curic=cyc)
         coefs = cpu.coefs_block(cfg, {p: cpu.int_coefs(nz, nodes[p]) for p in vmax})
         np_ = coefs.shape[0]
         sla = np.tile(np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0]), (ncol, 1))

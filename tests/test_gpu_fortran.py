@@ -43,7 +43,7 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
     L = np.eye(np_) - np.eye(np_, k=-1)
     hssr = 0.05 * (L.T @ L)
     lam, tol = 0.3, 1e-4
-    jac = rng.normal(size=(np_, 2 * nf))
+    jac = rng.normal(size=(np_, nf))
     cl, cr = lines.c_arrays(ls)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as fh:
@@ -51,7 +51,12 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
         fh.write(bytes(cl) + bytes(cr))
         fh.write(np.asarray(z, np.float64).tobytes())
         fh.write(cube4.astype(np.float32).tobytes() + inv4.astype(np.float32).tobytes())
-        fh.write(bytes(ctl))
+        # an unformatted READ of a derived type transfers its components one
+        # after another, without the C struct's alignment padding (4 bytes
+        # between PEN_TYP and PEN_ALP)
+        raw, pad = bytes(ctl), gpu.CInvertControl.pen_alp.offset
+        assert pad - (gpu.CInvertControl.pen_typ.offset + 32) == 4
+        fh.write(raw[:pad - 4] + raw[pad:])
         fh.write(np.asfortranarray(ijac).tobytes(order="F") + diff.tobytes() + pen.tobytes() +
                  np.asfortranarray(hssr).tobytes(order="F") + np.array([lam, tol]).tobytes() +
                  np.asfortranarray(jac).tobytes(order="F"))

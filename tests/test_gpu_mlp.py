@@ -69,7 +69,8 @@ def test_mlp_forward_vs_oracle(gpu_ctx):
     x = rng.normal(0.3, 0.15, (200, nin)).astype(np.float32)
     y = gpu_ctx.mlp_forward_torch(torch.from_numpy(x).cuda()).cpu().numpy()
     ref, _ = omlp.forward(p, x, nin, nh, nl, nout, emulate=True)
-    report("forward (200 rows) vs oracle, rel. to max|y|", np.max(np.abs(y - ref)) / np.abs(ref).max(), 2e-2)
+    # BF16 activations: a value at a rounding boundary may round the other way (1e-3 measured)
+    report("forward (200 rows) vs oracle, rel. to max|y|", np.max(np.abs(y - ref)) / np.abs(ref).max(), 5e-3)
     yd = gpu_ctx.mlp_forward_torch(torch.from_numpy(x.astype(np.float64)).cuda()).cpu().numpy()
     report("forward from float64 inputs == float32 inputs", np.max(np.abs(yd - y)), 0.0)
 
@@ -87,10 +88,10 @@ def test_mlp_gradient_vs_oracle(gpu_ctx):
                                          torch.from_numpy(idx).cuda())
     g = g.cpu().numpy()
     rl, rg = omlp.loss_and_grad(p, x[idx], t[idx], nin, nh, nl, nout, emulate=True)
-    report("loss vs oracle (relative)", abs(float(loss.item()) - rl) / rl, 1e-3)
+    report("loss vs oracle (relative)", abs(float(loss.item()) - rl) / rl, 2e-6)
     errs = per_layer(g, rg, nin, nh, nl, nout)
     for i, e in enumerate(errs):
-        report("gradient layer %d %s vs oracle (rel. to max)" % (i // 2, "Wb"[i % 2]), e, 3e-2)
+        report("gradient layer %d %s vs oracle (rel. to max)" % (i // 2, "Wb"[i % 2]), e, 1e-4)
 
 
 def test_mlp_targets_and_decode_vs_oracle(gpu_ctx, cpu):
@@ -113,9 +114,9 @@ def test_mlp_targets_and_decode_vs_oracle(gpu_ctx, cpu):
     dec = omlp.decode(y, tm, ts, coefs)
     for row, name in ((0, "T"), (7, "Bz"), (10, "vz")):
         sc = np.abs(dec[name]).max() + 1e-30
-        report("decoded %s vs oracle (rel. to max)" % name, np.max(np.abs(a[row].T - dec[name])) / sc, 3e-2)
+        report("decoded %s vs oracle (rel. to max)" % name, np.max(np.abs(a[row].T - dec[name])) / sc, 3e-3)
     bp = np.hypot(a[5], a[6]).T
-    report("decoded B_perp vs oracle", np.max(np.abs(bp - np.hypot(dec["Bx"], dec["By"]))) / (bp.max() + 1e-30), 3e-2)
+    report("decoded B_perp vs oracle", np.max(np.abs(bp - np.hypot(dec["Bx"], dec["By"]))) / (bp.max() + 1e-30), 3e-3)
     # rows the network does not propose are left as given
     assert np.array_equal(a[1:5], atm[1:5]) and np.array_equal(a[8:10], atm[8:10]) and np.array_equal(a[11], atm[11])
 

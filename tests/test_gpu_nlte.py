@@ -106,11 +106,16 @@ def test_nlte_seeded_cube_vs_oracle(gpu_ctx, cpu):
 
 
 def test_unit_departure_is_lte(gpu_ctx):
+    import torch
     ls = lines.fe6301_6302(nwave=56, start=-600.0, step=40.0)
     atm, z = atmos.seeded(2, 2, 32, seed=9)
     gpu_ctx.configure(ls, z)
     gpu_ctx.gain_hsra()
-    st0, j0, _ = _run(gpu_ctx, atm, gpu.RF_SPHERICAL)
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert int(gpu_ctx.hydrostatic_torch(t).max()) == 0
+    atm = t.cpu().numpy()
+    st0, j0, s0 = _run(gpu_ctx, atm, gpu.RF_SPHERICAL)
+    assert np.all(s0 == 0) and np.all(np.isfinite(st0))
     gpu_ctx.set_departure(np.ones((1, 32)), np.ones((1, 32)))
     st1, j1, _ = _run(gpu_ctx, atm, gpu.RF_SPHERICAL)
     gpu_ctx.set_departure(None)

@@ -218,3 +218,37 @@ def test_linear_forward_converges_to_least_squares(cpu):
     assert all(b <= a for a, b in zip(chis, chis[1:]))   # the best chi2 never increases
     assert chis[-1] < 1e-6 * chis[0]
     assert np.allclose(params(best), params(true), rtol=1e-4)
+
+
+@pytest.mark.parametrize("expansion", [1, 2])
+def test_zero_step_reprojection(cpu, expansion):
+    """CONTRACT_VECTOR (get_dmodel.f90:861-880) is a basis-weighted mean,
+    not the pseudo-inverse of EXPAND_VECTOR (:839-857), so with several
+    nodes a step of zero still moves the model (SORT_DELTA_TO_ATMOSPHERE,
+    :1715-1836): NODE EXPANSION 2 replaces a parameter by E(C x); NODE
+    EXPANSION 1 adds E(C(E(C x) - x)).  Checked with the node basis built as
+    numpy.interp hat functions, on v_z (no tail rule, no bounds in reach)."""
+    nz, numw, nj = 32, 20, 4
+    nodes = (0, 0, 0, 0, 0, 0, nj, 0)
+    cfg = _cfg(nz, numw, nodes=nodes, node_expansion=expansion)
+    pos = np.linspace(1.0, nz, nj)
+    E = np.stack([np.interp(np.arange(1, nz + 1), pos, np.eye(nj)[j]) for j in range(nj)], axis=1)
+    C = (E / E.sum(axis=0)).T
+    coefs = cpu.coefs_block(cfg, {6: cpu.int_coefs(nz, nj)})
+    nf = 4 * numw
+    rng = np.random.default_rng(9)
+    jac = rng.normal(size=(nf, nj)).astype(np.float32)
+    m = _toy(nz, numw)
+    m[10] = (1e5 + 4e4 * np.sin(np.linspace(0.0, 5.0, nz))).astype(np.float32)
+    syn = rng.normal(size=nf)
+    sla = np.array([0, 1e29, 1e29, 0, 0, 0, 0, 0, 10.0])
+    mc, mb, sl, bs, jb, delta = cpu.lm_step(cfg, coefs, syn.copy(), syn, syn.copy(), sla, jac, jac.copy(),
+                                            np.ones(nf), m, m)
+    x = m[10].astype(np.float64)
+    assert np.max(np.abs(C @ E - np.eye(nj))) > 0.1          # not a projection
+    if expansion == 2:
+        want = E @ (C @ x)
+    else:
+        want = x + E @ (C @ (E @ (C @ x) - x))
+    assert np.allclose(mc[10], want, rtol=2e-6, atol=0.05), np.max(np.abs(mc[10] - want))
+    assert np.max(np.abs(mc[10] - x)) > 100.0                # the model moved
