@@ -26,6 +26,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1280,6 +1281,9 @@ struct DevBuf {
   }
 };
 
+// per-device address of the fixed scratch array (fz_get_perturbation_dev_noalloc)
+static int scratch_setup(int device);
+
 extern "C" {
 
 int fz_create(fz_ctx** out, int device) {
@@ -1293,7 +1297,7 @@ int fz_create(fz_ctx** out, int device) {
     return FZ_EHIP;
   }
   hipLaunchKernelGGL(fz_init_constants_kernel, dim3(1), dim3(1), 0, 0);
-  if (hipDeviceSynchronize() != hipSuccess) {
+  if (hipDeviceSynchronize() != hipSuccess || scratch_setup(device) != FZ_OK) {
     delete c;
     return FZ_EHIP;
   }
@@ -2471,7 +2475,50 @@ static int run_pert(fz_ctx* c, int np, int nf, const double* d_ijac, const doubl
   return FZ_OK;
 }
 
+// fz_get_perturbation_dev_noalloc: the normal matrix and the solver's work
+// arrays in a fixed device array instead of memory the context allocates.
+// Every call on a GPU shares that GPU's copy (calls that overlap on one GPU
+// race); each call writes every entry it reads, so nothing is cleared.
+constexpr int NOALLOC_MAXNP = 512;
+constexpr size_t NOALLOC_DOUBLES = (size_t)NOALLOC_MAXNP * (NOALLOC_MAXNP + 1) +
+                                   2 * (size_t)NOALLOC_MAXNP * (NOALLOC_MAXNP + 1) + 8 * (size_t)NOALLOC_MAXNP +
+                                   LM_BIG_THREADS + 32;   // 6.3 MB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+// the address of each GPU's copy of scratch, from hipGetSymbolAddress with
+// that GPU current (the array's name is a host-side stand-in)
+static double* g_scratch_dev[64];
+
+static int scratch_setup(int device) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (device < 0 || device >= 64) return FZ_EARG;
+  if (g_scratch_dev[device]) return FZ_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FZ_EHIP;
+  g_scratch_dev[device] = (double*)ptr;
+  return FZ_OK;
+}
+
 extern "C" {
+
+int fz_get_perturbation_dev_noalloc(fz_ctx* c, int np, int nf, const double* d_ijacobian, const double* d_diff,
+                                    double lambdap, double* d_delta, const double* d_reg_pen,
+                                    const double* d_reg_hss, int dtype, double svdtol, void* stream) {
+  if (!c || !d_ijacobian || !d_diff || !d_delta) return FZ_EARG;
+  int r = pert_check(c, np, nf, dtype);
+  if (r) return r;
+  if (np > NOALLOC_MAXNP)
+    return set_err(c, FZ_EARG, "fz_get_perturbation_dev_noalloc: at most 512 parameters (fixed scratch)");
+  static_assert(NOALLOC_DOUBLES >= (size_t)NOALLOC_MAXNP * (NOALLOC_MAXNP + 1) + 2 * (size_t)NOALLOC_MAXNP *
+                                        (NOALLOC_MAXNP + 1) + 8 * (size_t)NOALLOC_MAXNP + LM_BIG_THREADS,
+                "scratch holds pert_scratch_doubles(NOALLOC_MAXNP)");
+  if (pert_scratch_doubles(np) > NOALLOC_DOUBLES) return set_err(c, FZ_EARG, "scratch too small");
+  double* ptr = g_scratch_dev[c->device];
+  if (!ptr) return set_err(c, FZ_EHIP, "scratch address of this device not set");
+  HIPCHK(c, hipSetDevice(c->device));
+  return run_pert(c, np, nf, d_ijacobian, d_diff, lambdap, d_delta, d_reg_pen, d_reg_hss, dtype, svdtol, ptr,
+                  (hipStream_t)stream);
+}
 
 int fz_get_perturbation_dev(fz_ctx* c, int np, int nf, const double* d_ijacobian, const double* d_diff,
                             double lambdap, double* d_delta, const double* d_reg_pen, const double* d_reg_hss,

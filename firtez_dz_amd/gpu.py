@@ -127,6 +127,8 @@ def library():
     L.fz_hessian.argtypes = [vp, ip, ip, dp, dp]
     L.fz_get_dmodel3ds.argtypes = [vp, ip, ip, dp, dp, dp, dp, ip, fp, fp, dp, ctypes.POINTER(fp),
                                    ctypes.POINTER(fp), ctypes.POINTER(ctypes.c_int32)]
+    L.fz_get_perturbation_dev_noalloc.argtypes = [vp, ip, ip, vp, vp, ctypes.c_double, vp, vp, vp, ip,
+                                                  ctypes.c_double, vp]
     _lib = L
     return L
 
@@ -141,7 +143,8 @@ def exported_symbols():
             "fz_invert3d",
             "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
             "fz_mlp_targets_dev", "fz_mlp_forward_dev", "fz_mlp_gradient_dev", "fz_mlp_train_step_dev",
-            "fz_learned_init_dev", "fz_get_perturbation", "fz_get_perturbation_dev", "fz_hessian", "fz_get_dmodel3ds"]
+            "fz_learned_init_dev", "fz_get_perturbation", "fz_get_perturbation_dev", "fz_hessian", "fz_get_dmodel3ds",
+            "fz_get_perturbation_dev_noalloc"]
 
 
 def invert_control(max_nodes: dict, invstk=(1, 1, 1, 1), wstk=(1.0, 1.0, 1.0, 1.0), noise=4e-3, cycpow=2,
@@ -531,6 +534,17 @@ class Context:
         self._check(self.lib.fz_get_perturbation_dev(self.h, np_, nf, jt.data_ptr(), diff.data_ptr(), float(lam),
                                                      delta.data_ptr(), ptr(reg_pen), ptr(reg_hss), int(dtype),
                                                      float(svdtol), self._stream(jt)), "fz_get_perturbation_dev")
+
+    def get_perturbation_noalloc_torch(self, jt, diff, lam, delta, reg_pen=None, reg_hss=None, dtype=1,
+                                       svdtol=1e-4):
+        """fz_get_perturbation_dev_noalloc (the library's fixed scratch, no
+        device allocation); arguments as get_perturbation_torch."""
+        np_, nf = jt.shape
+        ptr = lambda t: None if t is None else t.data_ptr()
+        self._check(self.lib.fz_get_perturbation_dev_noalloc(self.h, np_, nf, jt.data_ptr(), diff.data_ptr(),
+                                                             float(lam), delta.data_ptr(), ptr(reg_pen),
+                                                             ptr(reg_hss), int(dtype), float(svdtol),
+                                                             self._stream(jt)), "fz_get_perturbation_dev_noalloc")
 
     def get_dmodel3ds(self, obs3d, syn3d, best_syn, inv_mas, dsyn, best_dsyn, isigmap3d, model: dict,
                       best_model: dict, am_i_done):

@@ -285,6 +285,19 @@ int fz_get_perturbation_dev(fz_ctx* ctx, int np, int nf, const double* d_ijacobi
                             double lambdap, double* d_delta, const double* d_reg_pen, const double* d_reg_hss,
                             int dtype, double svdtol, void* stream);
 
+/* fz_get_perturbation_dev without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, hosts that forbid
+ * allocation).  The normal matrix and the solver's work arrays live in a
+ * fixed 6.3 MB device array of the library (one per GPU, held until the
+ * process ends, its address taken in fz_create); the caller passes inputs
+ * and outputs as for fz_get_perturbation_dev.  At most 512 parameters
+ * (FZ_EARG beyond).  Calls that overlap on one GPU share that array and
+ * race; each call writes every entry it reads, so no state carries over. */
+int fz_get_perturbation_dev_noalloc(fz_ctx* ctx, int np, int nf, const double* d_ijacobian, const double* d_diff,
+                                    double lambdap, double* d_delta, const double* d_reg_pen,
+                                    const double* d_reg_hss, int dtype, double svdtol, void* stream);
+
 /* HESSIAN(NP,NF,JACOBIAN,HSS) (lm.f90:30-39): HSS = JACOBIAN JACOBIAN^T on
  * host arrays, jacobian JACOBIAN(NP,NF) column-major (jacobian[f*np + p]),
  * hss HSS(NP,NP) out (symmetric).  FP64 matrix cores.  Synchronous. */
