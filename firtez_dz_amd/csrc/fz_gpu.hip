@@ -47,10 +47,16 @@ constexpr int NATM = 12;
 enum { A_T = 0, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VX, A_VY, A_VZ, A_TAU };
 enum { F_FLAG = 0, F_KC5 = 1, F_REG = 2 };
 
+// per-height record fields after the opacities: the field geometry (Geo,
+// seven floats) and 1/sqrt(T)
+constexpr int NGEO = 8;
+
 struct Cfg {
   int nlines, nreg, nslot, numw, nz, nfield;
+  int fgeo0;            // first geometry field of the per-height record
   LineConst line[MAXL];
   float dopfac[MAXL];   // Doppler width (mA) = dopfac * sqrt(T)
+  float idopfac[MAXL];  // 1 / dopfac
   float vfac[MAXL];     // Doppler shift (mA) = vfac * v_z   (absorption_matrix.f90:1896)
   int reg_nblend[MAXR], reg_slot0[MAXR], reg_pix0[MAXR], reg_nwave[MAXR];
   double reg_wstart[MAXR], reg_wstep[MAXR], reg_l0main[MAXR];
@@ -79,6 +85,7 @@ __constant__ Cfg c_cfg;
 
 __device__ __forceinline__ int fslot(int s, int q) { return F_REG + 4 * c_cfg.nreg + 8 * s + q; }
 __device__ __forceinline__ int freg(int r, int q) { return F_REG + 4 * r + q; }
+__device__ __forceinline__ int fgeo(int q) { return c_cfg.fgeo0 + q; }
 
 // ---------------------------------------------------------------- microphysics
 // FORWARD1D's per-height physics (forward.f90:911-1006) over a tile of
@@ -305,6 +312,19 @@ __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
   if (a.status[col] != 0) return;
   const long cs = a.ncol, ps = (long)nz * a.ncol;
   auto M = [&](int f) -> float& { return a.micro[f * ps + k * cs + col]; };
+  {
+    // FP32 field geometry and 1/sqrt(T) for the radiative transfer
+    const float* at = a.atm + k * cs + col;
+    const Geo g = geometry(at[A_BX * ps], at[A_BY * ps], at[A_BZ * ps]);
+    M(fgeo(0)) = g.B;
+    M(fgeo(1)) = g.cg;
+    M(fgeo(2)) = g.sg;
+    M(fgeo(3)) = g.sgf;
+    M(fgeo(4)) = g.cgf;
+    M(fgeo(5)) = g.c2p;
+    M(fgeo(6)) = g.s2p;
+    M(fgeo(7)) = rsqrtf(at[A_T * ps]);
+  }
   const double T = (double)a.atm[A_T * ps + k * cs + col], P = (double)a.atm[A_PG * ps + k * cs + col];
   const float rhof = a.atm[A_RHO * ps + k * cs + col];
   const double ne = a.eos[(long)k * a.ntile + ct], nh = a.eos[((long)nz + k) * a.ntile + ct];
@@ -510,32 +530,6 @@ struct RtArgs {
 
 enum { D_T = 0, D_P, D_R, D_V, D_B, D_G, D_F, NDIR };
 
-struct Geo {
-  float B, cg, sg, sgf, cgf, c2p, s2p;
-};
-
-__device__ __forceinline__ Geo geometry(float bx, float by, float bz) {
-  // absorption_matrix.f90:1800-1871
-  Geo g;
-  const float bt2 = bx * bx + by * by, b2 = bt2 + bz * bz;
-  g.B = sqrtf(b2);
-  if (g.B > 0.f) {
-    g.cg = bz / g.B;
-    g.sg = sqrtf(bt2) / g.B;
-    g.cgf = 1.f + bz * bz / b2;
-    g.sgf = 1.f - bz * bz / b2;
-  } else {
-    g.cg = 0.f; g.sg = 0.f; g.cgf = 1.f; g.sgf = 1.f;
-  }
-  if (bt2 > 0.f) {
-    g.c2p = (bx * bx - by * by) / bt2;
-    g.s2p = 2.f * bx * by / bt2;
-  } else {
-    g.c2p = 0.f; g.s2p = 0.f;
-  }
-  return g;
-}
-
 // profile combination of the three component types (sigma_r, pi, sigma_b)
 // into (I,Q,U,V) (absorption_matrix.f90:1946-2230)
 struct Lin {
@@ -555,8 +549,10 @@ __device__ __forceinline__ Lin lin_dphi(const Geo& g, float xr, float xp, float 
   return {0.f, -2.f * qb * g.s2p, 2.f * qb * g.c2p, 0.f};
 }
 
+// T, 1/T and 1/sqrt(T) (the inverse Doppler width of line li is
+// idopfac[li] / sqrt(T)), v_z and the field geometry of one height
 struct Layer {
-  float T, vz;
+  float T, iT, rsT, vz;
   Geo g;
 };
 
@@ -567,11 +563,10 @@ __device__ __forceinline__ void absorb(const RtArgs& a, int col, int k, int r, f
   const float* mic = a.micro + k * cs + col;
   x7[0] = mic[freg(r, 0) * ps];
   for (int c = 1; c < 7; ++c) x7[c] = 0.f;
-  const float sqT = sqrtf(ly.T);
   for (int b = 0; b < c_cfg.reg_nblend[r]; ++b) {
     const int sl = c_cfg.reg_slot0[r] + b, li = c_cfg.slot_line[sl];
     const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
-    const float idld = 1.0f / (c_cfg.dopfac[li] * sqT);
+    const float idld = c_cfg.idopfac[li] * ly.rsT;
     const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
     const float bsc = ly.g.B * idld;
     float hs[3], fs[3];
@@ -612,14 +607,13 @@ __device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int 
   dx[D_T][0] = mic[freg(r, 1) * ps];
   dx[D_P][0] = mic[freg(r, 2) * ps];
   dx[D_R][0] = mic[freg(r, 3) * ps];
-  const float sqT = sqrtf(ly.T);
   const float isp = 0.564189583547756f;   // 1/sqrt(pi)
   for (int b = 0; b < c_cfg.reg_nblend[r]; ++b) {
     const int sl = c_cfg.reg_slot0[r] + b, li = c_cfg.slot_line[sl];
     const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
     const float dkl[3] = {mic[fslot(sl, 2) * ps], mic[fslot(sl, 3) * ps], mic[fslot(sl, 4) * ps]};
     const float dda[3] = {mic[fslot(sl, 5) * ps], mic[fslot(sl, 6) * ps], mic[fslot(sl, 7) * ps]};
-    const float idld = 1.0f / (c_cfg.dopfac[li] * sqT);
+    const float idld = c_cfg.idopfac[li] * ly.rsT;
     const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
     const float bsc = ly.g.B * idld;
     // per component type: H, F, sum dH/du, dH/da, u*dH/du, u*dH/da, shift*dH/du, shift*dH/da
@@ -668,7 +662,7 @@ __device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int 
     const float b7[7] = {pHb.I, pHb.Q, pHb.U, pHb.V, -pHab.Q, -pHab.U, -pHab.V};
     const float g7[7] = {gH.I, gH.Q, gH.U, gH.V, gF.Q, gF.U, gF.V};
     const float f7[7] = {fH.I, fH.Q, fH.U, fH.V, fF.Q, fF.U, fF.V};
-    const float dudT = -0.5f / ly.T;                     // du/dT = -u/(2T)
+    const float dudT = -0.5f * ly.iT;                    // du/dT = -u/(2T)
     const float dudv = c_cfg.vfac[li] * idld;
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
@@ -687,10 +681,19 @@ __device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int 
 __device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
   const long ps = (long)c_cfg.nz * a.ncol, cs = a.ncol;
   const float* at = a.atm + k * cs + col;
+  const float* mg = a.micro + k * cs + col;
   Layer ly;
   ly.T = at[A_T * ps];
+  ly.iT = 1.0f / ly.T;
   ly.vz = at[A_VZ * ps];
-  ly.g = geometry(at[A_BX * ps], at[A_BY * ps], at[A_BZ * ps]);
+  ly.g.B = mg[fgeo(0) * ps];
+  ly.g.cg = mg[fgeo(1) * ps];
+  ly.g.sg = mg[fgeo(2) * ps];
+  ly.g.sgf = mg[fgeo(3) * ps];
+  ly.g.cgf = mg[fgeo(4) * ps];
+  ly.g.c2p = mg[fgeo(5) * ps];
+  ly.g.s2p = mg[fgeo(6) * ps];
+  ly.rsT = mg[fgeo(7) * ps];
   return ly;
 }
 
@@ -807,7 +810,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   };
   for (int k = 0; k < nz; ++k) {
     const Layer ly = load_layer(a, col, k);
-    const float S = c1 / (depart(k) * expf(c2 / ly.T) - 1.0f);
+    const float S = c1 * (1.0f / (depart(k) * expf(c2 * ly.iT) - 1.0f));
     if (RF) {
 #pragma unroll
       for (int s = 0; s < 4; ++s) scr[(long)(k * 4 + s) * sstr] = I[s];
@@ -822,7 +825,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
       float dx[NDIR][7];
       absorb_der(a, col, k, r, wave, ly, x7, dx);
       const float dz = c_cfg.dz_cm[k];
-      evol_build(x7, dz, E);
+      evol_build<true>(x7, dz, E);
       float v[4] = {I[0] - S, I[1], I[2], I[3]}, m4v[4], d2[4];
       m4mul(E.e, E.r, E.s0, v, m4v);
       evol_dir(E, dx[D_P], dz, v, m4v, d2);
@@ -839,7 +842,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
       }
     } else {
       absorb(a, col, k, r, wave, ly, x7);
-      evol_build(x7, c_cfg.dz_cm[k], E);
+      evol_build<false>(x7, c_cfg.dz_cm[k], E);
     }
     float In[4];
 #pragma unroll
@@ -882,14 +885,17 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 #pragma unroll
           for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
       } else {
-        const float ex = depart(k) * expf(c2 / ly.T);
-        const float S = c1 / (ex - 1.0f);
-        const float dS = c1 * c2 * ex / (ly.T * ly.T * (ex - 1.f) * (ex - 1.f));
+        // S = c1 / (ex - 1), dS/dT = S (ex / (ex - 1)) (c2 / T) / T
+        const float x = c2 * ly.iT;
+        const float ex = depart(k) * expf(x);
+        const float iem1 = 1.0f / (ex - 1.0f);
+        const float S = c1 * iem1;
+        const float dS = S * (ex * iem1) * x * ly.iT;
         float x7[7], dx[NDIR][7];
         absorb_der(a, col, k, r, wave, ly, x7, dx);
         Evol E;
         const float dz = c_cfg.dz_cm[k];
-        evol_build(x7, dz, E);
+        evol_build<true>(x7, dz, E);
         float v[4], m4v[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) v[s] = scr[(long)(k * 4 + s) * sstr] - (s == 0 ? S : 0.f);
@@ -914,7 +920,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
         for (int i = 0; i < 4; ++i) loc[D_T][i] += ((i == 0 ? 1.f : 0.f) - E.O[i][0]) * dS;
         if constexpr (HEX) {
           const float mu = a.atm[A_MU * ps + k * cs + col];
-          const float gk = -mu * c_cfg.he_cte[k] / ly.T / ly.T;
+          const float gk = -mu * c_cfg.he_cte[k] * ly.iT * ly.iT;
 #pragma unroll
           for (int i = 0; i < 4; ++i) loc[D_T][i] += gk * scr[(long)((nz + k) * 4 + i) * sstr];
         }
@@ -1413,6 +1419,7 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
     if (L.zn < 1 || L.zn > FZ_NELEM || L.ion < 1 || L.ion > 3) return set_err(c, FZ_EION, "bad element/ion");
     g.line[i] = {L.zn - 1, L.ion, L.lambda0, L.loggf, L.eplow, L.sigma, L.alpha};
     g.dopfac[i] = (float)(1.0e3 * L.lambda0 / LIGHT * sqrt(2.0 * KBOL / (matom[L.zn - 1] * MAMU)));
+    g.idopfac[i] = (float)(1.0 / (1.0e3 * L.lambda0 / LIGHT * sqrt(2.0 * KBOL / (matom[L.zn - 1] * MAMU))));
     g.vfac[i] = (float)(-1.0e3 * L.lambda0 / LIGHT);
     g.comp0[i] = (int)shift.size();
     if (zeeman_host(L, shift, str, g.ntype[i]) != 0) return set_err(c, FZ_ECONFIG, "|Delta J| > 1");
@@ -1452,7 +1459,8 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
       g.blk_n[g.nbpc] = std::min(RT_WAVE, g.reg_pix0[r] + g.reg_nwave[r] - l0);
       ++g.nbpc;
     }
-  g.nfield = F_REG + 4 * nreg + 8 * ns;
+  g.fgeo0 = F_REG + 4 * nreg + 8 * ns;
+  g.nfield = g.fgeo0 + NGEO;
   // layer thickness uses the spacing to the next height, the top one the
   // spacing below (rtesolver.f90:439-443); z rounded to REAL(4) like MODEL(:,13)
   for (int k = 0; k < nz; ++k) {

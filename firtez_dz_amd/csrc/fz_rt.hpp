@@ -61,6 +61,37 @@ __device__ __forceinline__ void voigt(float a, float u, float& H, float& F) {
   F = (u < 0.f) ? -im : im;
 }
 
+// Field geometry of one height (absorption_matrix.f90:1800-1871): |B|,
+// cos and sin of the inclination, 1 +- cos^2, cos 2phi, sin 2phi.  Formed
+// once per (height, column) with the opacities and stored in the per-height
+// record, so the radiative transfer reads it instead of forming it in
+// every lane of the column.
+struct Geo {
+  float B, cg, sg, sgf, cgf, c2p, s2p;
+};
+
+__device__ __forceinline__ Geo geometry(float bx, float by, float bz) {
+  // absorption_matrix.f90:1800-1871
+  Geo g;
+  const float bt2 = bx * bx + by * by, b2 = bt2 + bz * bz;
+  g.B = sqrtf(b2);
+  if (g.B > 0.f) {
+    g.cg = bz / g.B;
+    g.sg = sqrtf(bt2) / g.B;
+    g.cgf = 1.f + bz * bz / b2;
+    g.sgf = 1.f - bz * bz / b2;
+  } else {
+    g.cg = 0.f; g.sg = 0.f; g.cgf = 1.f; g.sgf = 1.f;
+  }
+  if (bt2 > 0.f) {
+    g.c2p = (bx * bx - by * by) / bt2;
+    g.s2p = 2.f * bx * by / bt2;
+  } else {
+    g.c2p = 0.f; g.s2p = 0.f;
+  }
+  return g;
+}
+
 // 4-vector helpers
 struct V4 {
   float v[4];
@@ -81,15 +112,113 @@ __device__ __forceinline__ void pmul(v3 a, v3 w, const float* v, float* o) {
 // formed without cancellation (lambda1*lambda2 = |eta.rho|), and the
 // exp(-eta_I dz) factor is folded into cosh/sinh so nothing overflows.
 //   O = A*1 + P(alpha, omega) + D4 * M4raw
+//
+// Near-degenerate polarisation eigenvalues.  The closed form divides by
+// th = sqrt(d^2 + 4 (e.r)^2) and by l1, l2; when th is small the terms of
+// the directional derivatives grow like 1/th^1.5 and cancel, and in FP32 the
+// error grows like eps (s0/th)^2 (s0 = (e.e + r.r)/2).  The four coefficient
+// functions the operator needs are entire functions of d and q = (e.r)^2:
+// with x, y the roots of z^2 - d z - q (x = l1^2, y = -l2^2),
+//   A  e^ei = (cosh l1 + cos l2)/2            = 1/2 sum_n P_n / (2n)!
+//   D4 e^ei = (cosh l1 - cos l2)/th           = sum_{n>=1} H_{n-1} / (2n)!
+//   ga e^ei = (l1 sinh l1 + l2 sin l2)/th     = sum_n H_n / (2n+1)!
+//   gb e^ei = e.r (sinh l1/l1 - sin l2/l2)/th = e.r sum_{n>=1} H_{n-1} / (2n+1)!
+// where P_n = x^n + y^n and H_n = sum_{i<=n} x^i y^(n-i) both follow
+// F_n = d F_{n-1} + q F_{n-2} (P_0 = 2, H_0 = 1, F_1 = d), so the series and
+// their d- and q-derivatives need no square root and no division.  For
+// th < TH_SER the operator and its derivatives use these series (NSER terms,
+// truncation below 1e-11 at th = 2); the closed form is kept above, where
+// th is not small against s0 unless the layer is optically thick and
+// e^-ei removes it.
+constexpr float TH_SER = 2.0f;
+constexpr int NSER = 9;
+
+// Fields of one branch of the operator in a shared array (a lane takes one
+// branch; sharing keeps the register footprint of the struct that of one):
+// closed form: 1/l1 or 1/l2, 1/th, l1, l2, sign(e.r), C2, B3, and the
+// e^-ei cosh/sinh/cos/sin products; series: the d- and q-derivatives of the
+// A, D4, ga and gb series, the gb series and e^-ei.
+enum { W_IL = 0, W_ITH, W_L1, W_L2, W_SIG, W_C2, W_B3, W_CHE, W_SHE, W_CE, W_SE, NW };
+enum { W_AD = 0, W_AQ, W_DD, W_DQ, W_GD, W_GQ, W_BD, W_BQ, W_SB, W_EX };
+
 struct Evol {
   float O[4][4];
   v3 e, r;
-  float ei, th, d, er, l1, l2, sig;
-  float A, C2, B3, D4, che, she, ce, se, s0;
+  float ei, th, d, er;
+  float A, D4, s0;
   // al = -(ga e + gb r) and om = -(gb e - ga r)
   float ga, gb;
+  bool ser;
+  float w[NW];
 };
 
+// the four coefficient series of the comment above (times e^ei) and, with
+// DER, their partial derivatives in d and q
+template <bool DER>
+__device__ __forceinline__ void evol_series(float d, float q, float& A0, float& D0, float& G0, float& B0, Evol& E) {
+  // 1/k! for k = 0 .. 2 NSER
+  constexpr float IF[2 * NSER + 1] = {1.f, 1.f, 1.f / 2.f, 1.f / 6.f, 1.f / 24.f, 1.f / 120.f, 1.f / 720.f,
+                                      1.f / 5040.f, 1.f / 40320.f, 1.f / 362880.f, 1.f / 3628800.f,
+                                      1.f / 39916800.f, 1.f / 479001600.f, 1.f / 6227020800.f,
+                                      1.f / 87178291200.f, 1.f / 1307674368000.f, 1.f / 20922789888000.f,
+                                      1.f / 355687428096000.f, 1.f / 6402373705728000.f};
+  // F_n and its d-, q-derivatives for H (h*) and P (p*), n = 0 .. NSER - 1
+  float h0 = 1.f, h1 = d, hd0 = 0.f, hd1 = 1.f, hq0 = 0.f, hq1 = 0.f;
+  float p0 = 2.f, p1 = d, pd0 = 0.f, pd1 = 1.f, pq0 = 0.f, pq1 = 0.f;
+  A0 = 0.5f * p0;
+  D0 = 0.f;
+  G0 = h0 * IF[1];
+  B0 = 0.f;
+  float Ad = 0.f, Aq = 0.f, Dd = 0.f, Dq = 0.f, Gd = 0.f, Gq = 0.f, Bd = 0.f, Bq = 0.f;
+#pragma unroll
+  for (int n = 1; n < NSER; ++n) {
+    // terms with H_{n-1} (held in h0 ...) and P_n, H_n (held in p1, h1 ...)
+    A0 += 0.5f * p1 * IF[2 * n];
+    D0 += h0 * IF[2 * n];
+    G0 += h1 * IF[2 * n + 1];
+    B0 += h0 * IF[2 * n + 1];
+    if (DER) {
+      Ad += 0.5f * pd1 * IF[2 * n];
+      Aq += 0.5f * pq1 * IF[2 * n];
+      Dd += hd0 * IF[2 * n];
+      Dq += hq0 * IF[2 * n];
+      Gd += hd1 * IF[2 * n + 1];
+      Gq += hq1 * IF[2 * n + 1];
+      Bd += hd0 * IF[2 * n + 1];
+      Bq += hq0 * IF[2 * n + 1];
+    }
+    // advance to n + 1
+    const float h2 = d * h1 + q * h0, p2 = d * p1 + q * p0;
+    if (DER) {
+      const float hd2 = h1 + d * hd1 + q * hd0, hq2 = d * hq1 + h0 + q * hq0;
+      const float pd2 = p1 + d * pd1 + q * pd0, pq2 = d * pq1 + p0 + q * pq0;
+      hd0 = hd1;
+      hd1 = hd2;
+      hq0 = hq1;
+      hq1 = hq2;
+      pd0 = pd1;
+      pd1 = pd2;
+      pq0 = pq1;
+      pq1 = pq2;
+    }
+    h0 = h1;
+    h1 = h2;
+    p0 = p1;
+    p1 = p2;
+  }
+  if (DER) {
+    E.w[W_AD] = Ad;
+    E.w[W_AQ] = Aq;
+    E.w[W_DD] = Dd;
+    E.w[W_DQ] = Dq;
+    E.w[W_GD] = Gd;
+    E.w[W_GQ] = Gq;
+    E.w[W_BD] = Bd;
+    E.w[W_BQ] = Bq;
+  }
+}
+
+template <bool DER>
 __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E) {
   E.ei = x7[0] * dz;
   E.e = mk(x7[1] * dz, x7[2] * dz, x7[3] * dz);
@@ -100,48 +229,63 @@ __device__ __forceinline__ void evol_build(const float x7[7], float dz, Evol& E)
   // hypot without its scaling unless the squares could overflow
   E.th = (fabsf(E.d) < 1.0e18f && fabsf(E.er) < 1.0e18f) ? sqrtf(E.d * E.d + 4.0f * E.er * E.er)
                                                           : hypotf(E.d, 2.0f * E.er);
-  float l1, l2;
-  if (E.d >= 0.f) {
-    l1 = sqrtf(0.5f * (E.th + E.d));
-    l2 = l1 > 0.f ? fabsf(E.er) / l1 : 0.f;
+  E.ser = E.th < TH_SER;
+  if (E.ser) {
+    float A0, D0, G0, B0;
+    evol_series<DER>(E.d, E.er * E.er, A0, D0, G0, B0, E);
+    E.w[W_EX] = expf(-E.ei);
+    E.w[W_SB] = B0;
+    E.A = E.w[W_EX] * A0;
+    E.D4 = E.w[W_EX] * D0;
+    E.ga = E.w[W_EX] * G0;
+    E.gb = E.w[W_EX] * (E.er * B0);
   } else {
-    l2 = sqrtf(0.5f * (E.th - E.d));
-    l1 = fabsf(E.er) / l2;
-  }
-  E.l1 = l1;
-  E.l2 = l2;
-  E.sig = E.er > 0.f ? 1.f : (E.er < 0.f ? -1.f : 0.f);
-  if (l1 < 40.f) {
-    // cosh and sinh from one exponential; sinh by its Taylor series below
-    // 0.5 (truncation 1e-11 relative), where E - 1/E would cancel
+    float l1, l2;
+    if (E.d >= 0.f) {
+      l1 = sqrtf(0.5f * (E.th + E.d));
+      E.w[W_IL] = l1 > 0.f ? 1.0f / l1 : 0.f;
+      l2 = fabsf(E.er) * E.w[W_IL];
+    } else {
+      l2 = sqrtf(0.5f * (E.th - E.d));
+      E.w[W_IL] = 1.0f / l2;
+      l1 = fabsf(E.er) * E.w[W_IL];
+    }
+    E.w[W_L1] = l1;
+    E.w[W_L2] = l2;
+    E.w[W_SIG] = E.er > 0.f ? 1.f : (E.er < 0.f ? -1.f : 0.f);
+    if (l1 < 40.f) {
+      // cosh and sinh from one exponential; sinh by its Taylor series below
+      // 0.5 (truncation 1e-11 relative), where E - 1/E would cancel
+      const float ex = expf(-E.ei);
+      const float El = expf(l1), iEl = 1.0f / El;
+      const float x2 = l1 * l1;
+      const float shs = l1 * (1.f + x2 * (1.f / 6.f) * (1.f + x2 * (1.f / 20.f) * (1.f + x2 * (1.f / 42.f) * (1.f + x2 * (1.f / 72.f)))));
+      E.w[W_CHE] = ex * (0.5f * (El + iEl));
+      E.w[W_SHE] = ex * (l1 < 0.5f ? shs : 0.5f * (El - iEl));
+    } else {
+      const float p = expf(l1 - E.ei), m = expf(-l1 - E.ei);
+      E.w[W_CHE] = 0.5f * (p + m);
+      E.w[W_SHE] = 0.5f * (p - m);
+    }
+    float sn, cs;
+    sincosf(l2, &sn, &cs);
     const float ex = expf(-E.ei);
-    const float El = expf(l1), iEl = 1.0f / El;
-    const float x2 = l1 * l1;
-    const float shs = l1 * (1.f + x2 * (1.f / 6.f) * (1.f + x2 * (1.f / 20.f) * (1.f + x2 * (1.f / 42.f) * (1.f + x2 * (1.f / 72.f)))));
-    E.che = ex * (0.5f * (El + iEl));
-    E.she = ex * (l1 < 0.5f ? shs : 0.5f * (El - iEl));
-  } else {
-    const float p = expf(l1 - E.ei), m = expf(-l1 - E.ei);
-    E.che = 0.5f * (p + m);
-    E.she = 0.5f * (p - m);
+    E.w[W_CE] = ex * cs;
+    E.w[W_SE] = ex * sn;
+    E.A = 0.5f * (E.w[W_CHE] + E.w[W_CE]);
+    const float ith = E.th > 0.f ? 1.0f / E.th : 0.f;
+    E.w[W_ITH] = ith;
+    E.w[W_C2] = E.w[W_SE] * ith;
+    E.w[W_B3] = E.w[W_SHE] * ith;
+    E.D4 = (E.w[W_CHE] - E.w[W_CE]) * ith;
+    // With a2 = l2 e - s l1 r, a3 = l1 e + s l2 r, w2 = -(s l1 e + l2 r) and
+    // w3 = s l2 e - l1 r, the vectors al = -(C2 a2 + B3 a3) and
+    // om = -(C2 w2 + B3 w3) take two scalars: ga = C2 l2 + B3 l1 and
+    // gb = s (B3 l2 - C2 l1).
+    const float s = E.w[W_SIG];
+    E.ga = E.w[W_C2] * l2 + E.w[W_B3] * l1;
+    E.gb = s * (E.w[W_B3] * l2 - E.w[W_C2] * l1);
   }
-  float sn, cs;
-  sincosf(l2, &sn, &cs);
-  const float ex = expf(-E.ei);
-  E.ce = ex * cs;
-  E.se = ex * sn;
-  E.A = 0.5f * (E.che + E.ce);
-  const float ith = E.th > 0.f ? 1.0f / E.th : 0.f;
-  E.C2 = E.se * ith;
-  E.B3 = E.she * ith;
-  E.D4 = (E.che - E.ce) * ith;
-  // With a2 = l2 e - s l1 r, a3 = l1 e + s l2 r, w2 = -(s l1 e + l2 r) and
-  // w3 = s l2 e - l1 r, the vectors al = -(C2 a2 + B3 a3) and
-  // om = -(C2 w2 + B3 w3) take two scalars: ga = C2 l2 + B3 l1 and
-  // gb = s (B3 l2 - C2 l1).
-  const float s = E.sig;
-  E.ga = E.C2 * l2 + E.B3 * l1;
-  E.gb = s * (E.B3 * l2 - E.C2 * l1);
   const v3 al = -(E.ga * E.e + E.gb * E.r);
   const v3 om = -(E.gb * E.e - E.ga * E.r);
   E.s0 = 0.5f * (e2 + r2);
@@ -186,25 +330,34 @@ __device__ __forceinline__ void evol_dir(const Evol& E, const float dx7[7], floa
   const v3 dr = mk(dx7[4] * dz, dx7[5] * dz, dx7[6] * dz);
   const float dd = 2.f * (dot(E.e, de) - dot(E.r, dr));
   const float der = dot(de, E.r) + dot(E.e, dr);
-  float dth = 0.f, dl1 = 0.f, dl2 = 0.f, ith = 0.f;
-  if (E.th > 0.f) {
-    ith = 1.0f / E.th;
-    dth = (E.d * dd + 4.f * E.er * der) * ith;
-    if (E.d >= 0.f) {
-      dl1 = (dth + dd) / (4.f * E.l1);
-      dl2 = (E.sig * der - E.l2 * dl1) / E.l1;
-    } else {
-      dl2 = (dth - dd) / (4.f * E.l2);
-      dl1 = (E.sig * der - E.l1 * dl2) / E.l2;
+  float dA, dD4, dga, dgb;
+  if (E.ser) {
+    const float dq = 2.f * E.er * der;
+    dA = -dei * E.A + E.w[W_EX] * (E.w[W_AD] * dd + E.w[W_AQ] * dq);
+    dD4 = -dei * E.D4 + E.w[W_EX] * (E.w[W_DD] * dd + E.w[W_DQ] * dq);
+    dga = -dei * E.ga + E.w[W_EX] * (E.w[W_GD] * dd + E.w[W_GQ] * dq);
+    dgb = -dei * E.gb + E.w[W_EX] * (der * E.w[W_SB] + E.er * (E.w[W_BD] * dd + E.w[W_BQ] * dq));
+  } else {
+    float dth = 0.f, dl1 = 0.f, dl2 = 0.f, ith = 0.f;
+    if (E.th > 0.f) {
+      ith = E.w[W_ITH];
+      dth = (E.d * dd + 4.f * E.er * der) * ith;
+      if (E.d >= 0.f) {
+        dl1 = (dth + dd) * (0.25f * E.w[W_IL]);
+        dl2 = (E.w[W_SIG] * der - E.w[W_L2] * dl1) * E.w[W_IL];
+      } else {
+        dl2 = (dth - dd) * (0.25f * E.w[W_IL]);
+        dl1 = (E.w[W_SIG] * der - E.w[W_L1] * dl2) * E.w[W_IL];
+      }
     }
+    dA = -dei * E.A + 0.5f * (E.w[W_SHE] * dl1 - E.w[W_SE] * dl2);
+    const float dC2 = -dei * E.w[W_C2] + (E.w[W_CE] * dl2 - E.w[W_C2] * dth) * ith;
+    const float dB3 = -dei * E.w[W_B3] + (E.w[W_CHE] * dl1 - E.w[W_B3] * dth) * ith;
+    dD4 = -dei * E.D4 + (E.w[W_SHE] * dl1 + E.w[W_SE] * dl2 - E.D4 * dth) * ith;
+    dga = dC2 * E.w[W_L2] + E.w[W_C2] * dl2 + dB3 * E.w[W_L1] + E.w[W_B3] * dl1;
+    dgb = E.w[W_SIG] * (dB3 * E.w[W_L2] + E.w[W_B3] * dl2 - dC2 * E.w[W_L1] - E.w[W_C2] * dl1);
   }
-  const float dA = -dei * E.A + 0.5f * (E.she * dl1 - E.se * dl2);
-  const float dC2 = -dei * E.C2 + (E.ce * dl2 - E.C2 * dth) * ith;
-  const float dB3 = -dei * E.B3 + (E.che * dl1 - E.B3 * dth) * ith;
-  const float dD4 = -dei * E.D4 + (E.she * dl1 + E.se * dl2 - E.D4 * dth) * ith;
   // d(al) = -(dga e + ga de + dgb r + gb dr), d(om) = -(dgb e + gb de - dga r - ga dr)
-  const float dga = dC2 * E.l2 + E.C2 * dl2 + dB3 * E.l1 + E.B3 * dl1;
-  const float dgb = E.sig * (dB3 * E.l2 + E.B3 * dl2 - dC2 * E.l1 - E.C2 * dl1);
   const v3 dal = -(dga * E.e + E.ga * de + dgb * E.r + E.gb * dr);
   const v3 dom = -(dgb * E.e + E.gb * de - dga * E.r - E.ga * dr);
   float p[4];

@@ -19,16 +19,16 @@ VOIGT = 66           # Hui 7/7 rational: r, s (4); 5 + 6 two-FMA rounds (44); re
 COMP_FWD = 2 + VOIGT + 4              # u, profile, 2 accumulations (absorb)
 COMP_DER = 2 + VOIGT + 4 + 5 + 20     # u, profile, dH/du, dH/da, 8 accumulations (absorb_der)
 # per line
-LINE_SETUP = 7                        # Doppler width, shift and field scale
+LINE_SETUP = 5                        # inverse Doppler width (from the stored 1/sqrt(T)), shift, field scale
 LINE_FWD = LINE_SETUP + 2 * 14 + 14   # 2 lin() combinations, 7 accumulations
 LIN_ALL = 8 * 14 + 2 * 15 + 2 * 8     # 8 lin(), 2 lin_dgamma(), 2 lin_dphi() combinations
 # accumulation into the 7 propagation-matrix elements per direction formed
 ACC = {"value": 2, "T": 7, "Pg": 5, "rho": 5, "v": 2, "B": 2, "gamma": 2, "phi": 2}
 DIRS = {5: ("T", "v", "B", "gamma", "phi"), 7: ("T", "Pg", "rho", "v", "B", "gamma", "phi")}
 # per layer
-GEOMETRY = 22
-SOURCE = 4
-SOURCE_DER = 11
+GEOMETRY = 0          # |B|, inclination and azimuth terms: formed by opac_kernel, read from the record
+SOURCE = 6            # 1/T, c2/T, exp, -1, reciprocal, x c1
+SOURCE_DER = 11       # the same and dS/dT = S (ex/(ex-1)) (c2/T) / T
 EVOL_BUILD = 155                      # evol_build: scaling, eigen-parameters, exponentials, sin/cos, O
 FORMAL = 40                           # I_k = (1-O) S + O I_{k-1}
 M4V = 46                              # v = I_in - S, M4raw v
@@ -69,8 +69,9 @@ def bytes_per_column_lambda(nz: int, numw: int, nparam: int, nfield: int) -> flo
 
 # HBM traffic of rt_kernel<1> per column of the forward + Jacobian workload
 # (nz = 96, 112 wavelengths, spherical Jacobian), from the FETCH_SIZE and
-# WRITE_SIZE counters of one launch (each scaled by its reading for a device
-# copy of known size: FETCH_SIZE x2, WRITE_SIZE x1)
-RT_HBM_BYTES_PER_COLUMN = 1.4178e6
-RT_HBM_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of rt_kernel<1>, 9216-column launch of this "
+# WRITE_SIZE counters of one 32768-column launch (each scaled by its reading
+# for a device copy of known size: FETCH_SIZE x2, WRITE_SIZE x1):
+# (4.102e6 KiB x 2 + 3.309e7 KiB) x 1024 / 32768 columns
+RT_HBM_BYTES_PER_COLUMN = 1.2905e6
+RT_HBM_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of rt_kernel<1>, a 32768-column launch of this "
                  "workload, scaled per column (profiles/rt_kernel_fwd_jacobian_288x288x96.md)")
