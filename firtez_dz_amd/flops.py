@@ -71,7 +71,7 @@ def bytes_per_column_lambda(nz: int, numw: int, nparam: int, nfield: int) -> flo
 # (nz = 96, 112 wavelengths, spherical Jacobian), from the FETCH_SIZE and
 # WRITE_SIZE counters of one 32768-column launch (each scaled by its reading
 # for a device copy of known size: FETCH_SIZE x2, WRITE_SIZE x1):
-# (4.102e6 KiB x 2 + 3.309e7 KiB) x 1024 / 32768 columns
-RT_HBM_BYTES_PER_COLUMN = 1.2905e6
+# (4.736e6 KiB x 2 + 3.309e7 KiB) x 1024 / 32768 columns
+RT_HBM_BYTES_PER_COLUMN = 1.3300e6
 RT_HBM_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of rt_kernel<1>, a 32768-column launch of this "
                  "workload, scaled per column (profiles/rt_kernel_fwd_jacobian_288x288x96.md)")
