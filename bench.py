@@ -60,7 +60,7 @@ def cpu_baseline_reference(nz, seconds=10.0):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def oracle_error(ls, z, gain, atm_h, st_h, jac_h, cols, ids=None):
+def oracle_error(ls, z, gain, atm_h, st_h, jac_h, cols, ids=None, xi=0.0):
     """Error of the timed cube against the FP64 oracle on sampled columns:
     Stokes |dS| in units of I_c; spherical Jacobians (the oracle's Cartesian
     response functions through the closed-form chain rule to |B|,
@@ -76,6 +76,7 @@ def oracle_error(ls, z, gain, atm_h, st_h, jac_h, cols, ids=None):
     from firtez_dz_amd import atmos
     cpu = oracle.Cpu()
     cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 0))
+    cpu.set_microturbulence(xi)
     cpu.set_gain(gain)
     models = np.stack([atmos.column_model(atm_h, z, int(c)) for c in cols])
     _, syn, dsyn = cpu.forward_batch(models)
@@ -110,7 +111,7 @@ def oracle_error(ls, z, gain, atm_h, st_h, jac_h, cols, ids=None):
             "oracle": "FP64 C restatement of the reference column path"}
 
 
-def cpu_baseline(ls, nz, gain, seconds=12.0, threads=0):
+def cpu_baseline(ls, nz, gain, seconds=12.0, threads=0, xi=0.0):
     """The oracle (FP64 C restatement, OpenMP over columns) on a bounded sample
     of the same workload: columns of the seeded cube, full Jacobians."""
     import oracle
@@ -120,6 +121,7 @@ def cpu_baseline(ls, nz, gain, seconds=12.0, threads=0):
     cpu = oracle.Cpu()
     numw = cpu.setup(ls, resp=True, atmpar=(1, 1, 1, 1, 1, 1, 1, 0))
     cpu.set_gain(gain)
+    cpu.set_microturbulence(xi)
     atm, z = atmos.seeded(16, 16, nz, seed=2024)
     # hydrostatic Pg for the sample columns (not timed)
     models = []
@@ -270,12 +272,17 @@ def main():
     ap.add_argument("--max-passes", type=int, default=3)
     ap.add_argument("--uneven", action="store_true",
                     help="with --lm: per-pixel line sets and windows (8-line list, 1-4 lines, 60-300 wavelengths)")
+    ap.add_argument("--micro", type=float, default=0.5, help="microturbulence (km/s) of the forward workload")
+    ap.add_argument("--coupled", action="store_true",
+                    help="PSF-coupled inversion of a 256x256x128 cube through a 31x31 Airy + wing PSF")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (columns are independent; "
               "see the sharding design).", file=sys.stderr)
         sys.exit(2)
 
+    if args.coupled:
+        return bench_coupled(args)
     if args.lm and args.uneven:
         return bench_uneven(args)
     if args.lm:
@@ -291,6 +298,8 @@ def main():
     ctx = gpu.Context(0)
     numw = ctx.configure(ls, z)
     gain = ctx.gain_hsra()
+    xi = args.micro * 1.0e5
+    ctx.set_microturbulence(xi)
     ctx.set_tile(args.tile, 256)
     d_atm = torch.from_numpy(atm).cuda()
     st = ctx.hydrostatic_torch(d_atm)
@@ -347,7 +356,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32 (f64 EOS/opacities)", "data": "synthetic (seeded atmosphere)",
         "config": {"workload": "forward+Jacobian %dx%d columns x nz=%d, Fe I 6301.5+6302.5, %d wavelengths, "
-                               "d/d(T,|B|,incl,azim,v_los) per height" % (args.nx, args.ny, args.nz, numw),
+                               "d/d(T,|B|,incl,azim,v_los) per height, microturbulence %.2g km/s"
+                               % (args.nx, args.ny, args.nz, numw, args.micro),
                    "columns": ncol, "nz": args.nz, "wavelengths": numw, "parameters": npar,
                    "kernel_ms": {"micro": float(np.mean(micro_ms)), "rt": rt_avg_ms},
                    "tiles": {"rt_columns": args.tile, "micro_columns": tiles[1] if tiles else None,
@@ -366,7 +376,7 @@ def main():
         cols = np.sort(np.random.default_rng(64).choice(ncol, 64, replace=False))
         icols = torch.from_numpy(cols).cuda()
         err = oracle_error(ls, z, gain, d_atm[:, :, icols].cpu().numpy(), stokes[icols].cpu().numpy(),
-                           jac[icols].cpu().numpy(), np.arange(64), ids=cols)
+                           jac[icols].cpu().numpy(), np.arange(64), ids=cols, xi=xi)
         print("[bench] error vs oracle (64 columns): Stokes max %.2e Ic, Jacobian max %.2e"
               % (err["stokes_max_abs_Ic"], err["jacobian_max_rel"]), file=sys.stderr, flush=True)
         line["error"] = err
@@ -376,7 +386,7 @@ def main():
             base = cpu_baseline_reference(args.nz, seconds=args.cpu_seconds)
         if base is None:
             print("[bench] CPU baseline (oracle) ...", file=sys.stderr, flush=True)
-            base = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds)
+            base = cpu_baseline(ls, args.nz, gain, seconds=args.cpu_seconds, xi=xi)
         line["cpu_baseline"] = base
     if not args.no_lm:
         # the second half of the metric: LM-converged pixels/s, learned vs plain
@@ -616,6 +626,92 @@ def bench_uneven(args):
                             "union_wavelengths": numw, "lines_per_pixel_mean": float(nlin.mean()),
                             "lm_iterations": its, "seconds": dt,
                             "column_wavelengths_per_s_of_iterations": tot * its / dt}, **stats)}
+    print(json.dumps(line), flush=True)
+
+
+def bench_coupled(args):
+    """INVERT3D through a spatial PSF (fz_coupled_invert3d_dev): a seeded
+    256x256x128 truth, observed as PSF * forward(truth) + 1e-3 noise, with a
+    31x31 PSF (Airy core of first-zero radius 2.44 pixels, i.e. 1.22 lambda/D
+    with lambda/D = 0.16" on 0.08" pixels, plus a Gaussian wing of 4 pixels
+    carrying 20 % of the light).  Reported: the coupled inversion's
+    pixels/s and accuracy, the uncoupled INVERT3D of the same observations
+    beside it, and the PSF convolution kernel against its roofline."""
+    import torch
+    from firtez_dz_amd import atmos, gpu, lines
+
+    nx = ny = 256 if args.nx == 288 else args.nx
+    nz = 128 if args.nz == 96 else args.nz
+    ncol = nx * ny
+    ls = lines.fe6301_6302(nwave=112, start=-624.0, step=20.0)
+    print("[bench-coupled] seeding %dx%dx%d truth" % (nx, ny, nz), file=sys.stderr, flush=True)
+    atm, z = atmos.seeded(nx, ny, nz, dz_km=15.0, seed=7)
+    ctx = gpu.Context(0)
+    numw = ctx.configure(ls, z)
+    ctx.gain_hsra()
+    psf = atmos.airy_gauss_psf(31, 2.44, 4.0, 0.2)
+    ctx.set_psf(psf)
+    truth = torch.from_numpy(atm).cuda()
+    assert int(ctx.hydrostatic_torch(truth).max().item()) == 0
+    st, _, status = ctx.forward_torch(truth, gpu.RF_NONE)
+    torch.cuda.synchronize()
+    assert int(status.max().item()) == 0
+    nmap = 4 * numw
+    maps = st.reshape(ncol, nmap).t().contiguous()
+    del st
+    conv = ctx.psf_convolve_torch(maps, nx, ny)
+    # the convolution kernel alone, timed on the stream it runs on
+    reps = 10
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        ctx.psf_convolve_torch(maps, nx, ny, out=conv)
+    ev1.record()
+    torch.cuda.synchronize()
+    conv_ms = ev0.elapsed_time(ev1) / reps
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(11)
+    obs = (conv.t().double() +
+           1e-3 * torch.randn((ncol, nmap), generator=gen, device="cuda", dtype=torch.float64)).contiguous()
+    del maps, conv
+    init = plain_init(ctx, truth)
+    ctl = gpu.invert_control({"T": 4, "Bx": 4, "By": 4, "Bz": 4, "vz": 4}, noise=1e-3, smoothing=False)
+    print("[bench-coupled] PSF convolution of %d maps: %.3f ms; coupled inversion of %d pixels"
+          % (nmap, conv_ms, ncol), file=sys.stderr, flush=True)
+    model = init.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, chi_c, its_c = ctx.coupled_invert3d_torch(model, obs, ctl, nx, ny)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    coupled = dict(lm_stats(model, truth, chi_c.cpu().numpy()), lm_iterations=its_c, seconds=dt)
+    print("[bench-coupled] coupled: %.1f s, %d iterations, median chi2 %.3g, mean |T - truth| %.1f K"
+          % (dt, its_c, coupled["median_chi2"], coupled["mean_abs_T_error_K"]), file=sys.stderr, flush=True)
+    del model
+    model = init.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, chi_u, its_u = ctx.invert3d_torch(model, obs, ctl, nx, ny)
+    torch.cuda.synchronize()
+    du = time.perf_counter() - t0
+    uncoupled = dict(lm_stats(model, truth, chi_u.cpu().numpy()), lm_iterations=its_u, seconds=du)
+    print("[bench-coupled] uncoupled: %.1f s, %d iterations, median chi2 %.3g, mean |T - truth| %.1f K"
+          % (du, its_u, uncoupled["median_chi2"], uncoupled["mean_abs_T_error_K"]), file=sys.stderr, flush=True)
+    taps = 31 * 31
+    conv_flop = 2.0 * taps * nmap * ncol
+    conv_bytes = 2.0 * 4 * nmap * ncol
+    workload = ("coupled INVERT3D %dx%d pixels, nz=%d, Fe I 6301.5+6302.5, %d wavelengths, 31x31 Airy+wing PSF, "
+                "T/Bx/By/Bz/vz up to 4 nodes (2 cycles), plain initial model" % (nx, ny, nz, numw))
+    line = {"metric": "LM-converged pixels/s (PSF-coupled)", "value": ncol / dt, "unit": "pixel/s", "n_gpus": 1,
+            "steps": 1, "warmup": 0, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer and PSF convolution",
+            "data": "synthetic (seeded truth, PSF-degraded, + 1e-3 noise)",
+            "config": {"workload": workload, "coupled": coupled, "uncoupled_same_observations": uncoupled,
+                       "ms_per_lm_iteration": dt * 1e3 / max(its_c, 1)},
+            "roofline": {"kernel": "psf_conv_kernel (%d maps of %dx%d, %d taps)" % (nmap, nx, ny, taps),
+                         "bound": "fp32 valu", "achieved": conv_flop / (conv_ms * 1e-3) / 1e12, "peak": 157.3,
+                         "unit": "TFLOP/s", "frac": conv_flop / (conv_ms * 1e-3) / 1e12 / 157.3,
+                         "ms": conv_ms, "hbm_GB_per_s": conv_bytes / (conv_ms * 1e-3) / 1e9}}
     print(json.dumps(line), flush=True)
 
 

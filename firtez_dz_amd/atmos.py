@@ -69,3 +69,22 @@ def column_model(atm, z_km, col):
     m[:12] = atm[:, :, col]
     m[12] = np.asarray(z_km, np.float32)
     return m
+
+
+def airy_gauss_psf(n: int, core: float, wing: float, wing_frac: float = 0.1) -> np.ndarray:
+    """A synthetic n x n spatial PSF, PSF(NPY, NPX) with the centre at
+    (n//2, n//2): an Airy pattern whose first zero lies `core` pixels from
+    the centre (1.22 lambda/D over the pixel size) plus a Gaussian wing of
+    standard deviation `wing` pixels carrying `wing_frac` of the light; unit
+    sum."""
+    from scipy.special import j1
+    r0 = n // 2
+    y, x = np.mgrid[-r0:n - r0, -r0:n - r0].astype(np.float64)
+    r = np.hypot(x, y)
+    v = 3.8317 * r / core             # first zero of J1 at v = 3.8317
+    with np.errstate(invalid="ignore", divide="ignore"):
+        airy = np.where(v > 0, (2.0 * j1(v) / v) ** 2, 1.0)
+    airy /= airy.sum()
+    g = np.exp(-0.5 * (r / wing) ** 2)
+    g /= g.sum()
+    return (1.0 - wing_frac) * airy + wing_frac * g

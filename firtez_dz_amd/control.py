@@ -22,9 +22,10 @@ RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
-CONTINUUM NORMALIZATION, COUPLED INVERSION, LINE OPTICAL DEPTH, AUTO
-Stokes weights, FULL_STOKES) raise NotImplementedError instead of being
-ignored.  MISC SETUP TEMP / PGAS / BVEC / VLOS switch on SET_TAU's assist
+CONTINUUM NORMALIZATION, LINE OPTICAL DEPTH, AUTO Stokes weights,
+FULL_STOKES) raise NotImplementedError instead of being ignored.  COUPLED
+INVERSION (yes and FILENAME, the PSF file in READ_PSF_2D's format under
+DATAPATH) runs MODE inversion through that spatial PSF.  MISC SETUP TEMP / PGAS / BVEC / VLOS switch on SET_TAU's assist
 initialisers.  LINE SPREAD FUNCTION
 (Gaussian rows or LSF files) sets the library's LSF; NLTE DEPARTURE
 COEFFICIENTS (per-line files of BETAL, BETAU per height) sets POPL/POPU.
@@ -45,7 +46,7 @@ from . import modelio
 PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
-_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "COUPLED INVERSION:", "LINE OPTICAL DEPTH:")
+_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "LINE OPTICAL DEPTH:")
 
 
 @dataclass
@@ -80,6 +81,9 @@ class Control:
     nlte: List[Tuple[int, str]] = field(default_factory=list)
     # MISC SETUP TEMP / PGAS / BVEC / VLOS: ASSIST_* bits
     assist: int = 0
+    # COUPLED INVERSION: yes/no and the PSF file (DATAPATH-relative)
+    coupled: bool = False
+    psf_file: Optional[str] = None
 
 
 def _number(tok: str) -> float:
@@ -231,6 +235,25 @@ def parse(text: str) -> Control:
                     c.lsf.append(None if cols[0].upper() == "EMPTY" else cols[0])
                 else:
                     raise ValueError('wrong input format in "LINE SPREAD FUNCTION": %r' % row)
+        elif key == "COUPLED INVERSION:":
+            # NSET_COUPLED_INVERSION (input_file.f90:1524-1607): a one-column
+            # row y/n, "FILENAME <psf file>"; BLCKSZ / NMTHRD / PSFRAD set up
+            # the reference's unwired block solver and are read and not used
+            body, i = section(i)
+            for row in body:
+                cols = row.split()
+                if len(cols) == 1:
+                    if cols[0][:1] in ("y", "Y"):
+                        c.coupled = True
+                elif len(cols) == 2:
+                    if cols[0] == "FILENAME":
+                        c.psf_file = cols[1]
+                    elif cols[0] in ("BLCKSZ", "NMTHRD", "PSFRAD"):
+                        _number(cols[1])
+                else:
+                    raise ValueError('"COUPLED INVERSION:" rows are yes/no and FILENAME <psf file>: %r' % row)
+            if c.coupled and c.psf_file is None:
+                raise ValueError('"COUPLED INVERSION:" with yes needs a FILENAME row')
         elif key == "MISC SETUP:":
             body, i = section(i)
             for row in body:
@@ -426,7 +449,18 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
                 raise ValueError("observed profiles (%s) do not match the lines and the model" % (prof.shape,))
             rows = [s for s in range(4) if c.invstk[s]]
             obs3d = np.asfortranarray(np.concatenate([prof[s] for s in rows], axis=0).astype(np.float64))
-            syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz))
+            if c.coupled:
+                # READ_PSF (pre_post_duties.f90:488-555) reads DATAPATH//FILENAME
+                ppath = path(c.datapath, c.psf_file)
+                if not os.path.exists(ppath):
+                    raise FileNotFoundError("cannot find PSF file %s in directory %s" % (c.psf_file, c.datapath))
+                ctx.set_psf(modelio.read_psf(ppath))
+                try:
+                    syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz), coupled=True)
+                finally:
+                    ctx.set_psf(None)         # END_COUPLED_INVERSION_*
+            else:
+                syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz))
             st = _stokes4(syn3d, numw, ny, nx)
             for s in range(4):
                 if not c.invstk[s]:

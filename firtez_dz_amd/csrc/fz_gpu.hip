@@ -80,6 +80,11 @@ struct Cfg {
   // spectral LSF (MLSF): on/off and LSF_VALID per region
   int lsf_on;
   int reg_lsf[MAXR];
+  // microturbulence xi: the Doppler width of line li is that of the
+  // temperature T + xi_t[li], xi_t = m xi^2 / (2 k) (xi_on 0: thermal only)
+  int xi_on;
+  double xi_t[MAXL];
+  float xi_tf[MAXL];
 };
 __constant__ Cfg c_cfg;
 
@@ -361,8 +366,9 @@ __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
         // the region's departure coefficients for every line of the region (forward.f90:949)
         const double bl = a.pop ? a.pop[((long)0 * nz + k) * c_cfg.nreg + r] : 1.0;
         const double bu = a.pop ? a.pop[((long)nz + k) * c_cfg.nreg + r] : 1.0;
-        err = opac_line(L, T, ne, nh, s, d, a.der != 0, kl, bl, bu);
-        damping(L, T, nh, d, a.der != 0, dm);
+        const double Td = T + c_cfg.xi_t[c_cfg.slot_line[sl]];
+        err = opac_line(L, T, ne, nh, s, d, a.der != 0, kl, bl, bu, Td);
+        damping(L, T, nh, d, a.der != 0, dm, Td);
         M(fslot(sl, 0)) = (float)kl[0];
         M(fslot(sl, 1)) = (float)dm[0];
         M(fslot(sl, 2)) = (float)kl[1];
@@ -566,7 +572,8 @@ __device__ __forceinline__ void absorb(const RtArgs& a, int col, int k, int r, f
   for (int b = 0; b < c_cfg.reg_nblend[r]; ++b) {
     const int sl = c_cfg.reg_slot0[r] + b, li = c_cfg.slot_line[sl];
     const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
-    const float idld = c_cfg.idopfac[li] * ly.rsT;
+    const float rsd = c_cfg.xi_on ? rsqrtf(ly.T + c_cfg.xi_tf[li]) : ly.rsT;   // 1/sqrt(Doppler temperature)
+    const float idld = c_cfg.idopfac[li] * rsd;
     const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
     const float bsc = ly.g.B * idld;
     float hs[3], fs[3];
@@ -613,7 +620,8 @@ __device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int 
     const float kl = mic[fslot(sl, 0) * ps], dmp = mic[fslot(sl, 1) * ps];
     const float dkl[3] = {mic[fslot(sl, 2) * ps], mic[fslot(sl, 3) * ps], mic[fslot(sl, 4) * ps]};
     const float dda[3] = {mic[fslot(sl, 5) * ps], mic[fslot(sl, 6) * ps], mic[fslot(sl, 7) * ps]};
-    const float idld = c_cfg.idopfac[li] * ly.rsT;
+    const float rsd = c_cfg.xi_on ? rsqrtf(ly.T + c_cfg.xi_tf[li]) : ly.rsT;   // 1/sqrt(Doppler temperature)
+    const float idld = c_cfg.idopfac[li] * rsd;
     const float base = (wave - c_cfg.slot_bldiff[sl] + c_cfg.vfac[li] * ly.vz) * idld;
     const float bsc = ly.g.B * idld;
     // per component type: H, F, sum dH/du, dH/da, u*dH/du, u*dH/da, shift*dH/du, shift*dH/da
@@ -662,7 +670,7 @@ __device__ __forceinline__ void absorb_der(const RtArgs& a, int col, int k, int 
     const float b7[7] = {pHb.I, pHb.Q, pHb.U, pHb.V, -pHab.Q, -pHab.U, -pHab.V};
     const float g7[7] = {gH.I, gH.Q, gH.U, gH.V, gF.Q, gF.U, gF.V};
     const float f7[7] = {fH.I, fH.Q, fH.U, fH.V, fF.Q, fF.U, fF.V};
-    const float dudT = -0.5f * ly.iT;                    // du/dT = -u/(2T)
+    const float dudT = c_cfg.xi_on ? -0.5f * rsd * rsd : -0.5f * ly.iT;   // du/dT = -u/(2 T_Doppler)
     const float dudv = c_cfg.vfac[li] * idld;
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
@@ -1081,6 +1089,7 @@ __global__ void soa_to_cube(const float* __restrict__ src, float* __restrict__ d
 #include "fz_lm.hpp"
 #include "fz_pert.hpp"
 #include "fz_invert.hpp"
+#include "fz_psf.hpp"
 #include "fz_mlp.hpp"
 
 namespace fz {
@@ -1221,6 +1230,7 @@ struct fz_ctx {
   float* d_derb = nullptr;    // EOS derivatives [8][nz][mtile] (double)
   size_t derb_cap = 0;
   bool hydro = false;         // HYDROSTATIC: yes
+  double xi = 0.0;            // microturbulence (cm/s); survives fz_configure
   size_t lm_lds = 0;
   bool lm_p0 = false;         // P0 among the inverted parameters
   bool lm_pr = false;         // Pg or rho among the inverted parameters
@@ -1242,6 +1252,13 @@ struct fz_ctx {
   float* d_pstage = nullptr;
   size_t pstage_cap = 0;
   MlpState mlp;
+  // spatial PSF of the coupled inversion: PSF(NPY, NPX) as given, and the
+  // taps of the last map size it was cut for
+  std::vector<double> h_psf;
+  int psf_npx = 0, psf_npy = 0;
+  float* d_taps = nullptr;
+  int taps_nx = 0, taps_ny = 0, taps_rx = 0, taps_ry = 0, taps_ntx = 0;
+  double taps_sq = 0.0;   // sum of the squared taps
 };
 
 
@@ -1270,6 +1287,16 @@ static int run_lsf(fz_ctx* c, float* d_rows, long nrows, hipStream_t st) {
                      nrows, (const float*)c->d_lsf);
   HIPCHK(c, hipGetLastError());
   return FZ_OK;
+}
+
+// microturbulence xi (cm/s) into the configuration: Doppler temperature
+// offsets m xi^2 / (2 k) per line
+static void set_xi(Cfg& g, const double* matom, double xi) {
+  g.xi_on = xi > 0.0 ? 1 : 0;
+  for (int i = 0; i < g.nlines; ++i) {
+    g.xi_t[i] = xi * xi * matom[g.line[i].zn] * MAMU / (2.0 * KBOL);
+    g.xi_tf[i] = (float)g.xi_t[i];
+  }
 }
 
 static int upload_cfg(fz_ctx* c) {
@@ -1337,6 +1364,7 @@ void fz_destroy(fz_ctx* c) {
   if (c->d_pop) (void)hipFree(c->d_pop);
   if (c->d_pert) (void)hipFree(c->d_pert);
   if (c->d_pstage) (void)hipFree(c->d_pstage);
+  if (c->d_taps) (void)hipFree(c->d_taps);
   c->mlp.release();
   for (int i = 0; i < 3; ++i)
     if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
@@ -1461,6 +1489,7 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
     }
   g.fgeo0 = F_REG + 4 * nreg + 8 * ns;
   g.nfield = g.fgeo0 + NGEO;
+  set_xi(g, matom, c->xi);
   // layer thickness uses the spacing to the next height, the top one the
   // spacing below (rtesolver.f90:439-443); z rounded to REAL(4) like MODEL(:,13)
   for (int k = 0; k < nz; ++k) {
@@ -1491,6 +1520,7 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
   for (int l = 0; l < numw; ++l) ones[4 * l + 3] = -1.0f;
   HIPCHK(c, hipMemcpy(c->d_gain, ones.data(), sizeof(float) * 4 * numw, hipMemcpyHostToDevice));
   clear_windows(c);
+  c->pop_on = false;   // LTE until fz_set_departure (input_file.f90:504-508)
   c->configured = true;
   c->have_gain = false;
   return FZ_OK;
@@ -1531,11 +1561,106 @@ int fz_set_departure(fz_ctx* c, const double* popl, const double* popu) {
   return FZ_OK;
 }
 
+int fz_set_psf(fz_ctx* c, const double* psf, int npy, int npx) {
+  if (!c) return FZ_EARG;
+  if (!psf) {
+    c->h_psf.clear();
+    c->psf_npx = c->psf_npy = 0;
+    c->taps_nx = c->taps_ny = 0;
+    return FZ_OK;
+  }
+  if (npx < 1 || npy < 1 || npx / 2 > PSF_RMAX || npy / 2 > PSF_RMAX)
+    return set_err(c, FZ_EARG, "PSF size outside 1 .. 63 pixels per axis");
+  for (long i = 0; i < (long)npx * npy; ++i)
+    if (!std::isfinite(psf[i])) return set_err(c, FZ_EARG, "PSF values must be finite");
+  c->h_psf.assign(psf, psf + (size_t)npx * npy);
+  c->psf_npx = npx;
+  c->psf_npy = npy;
+  c->taps_nx = c->taps_ny = 0;
+  return FZ_OK;
+}
+
+}  // extern "C"
+
+// the taps of the PSF for an nx x ny map (READ_PSF's central part of
+// half-widths NPX/2, NPY/2 around (NPY/2 + 1, NPX/2 + 1)); the periodic sum
+// needs 2 r + 1 <= n along each axis
+static int psf_taps(fz_ctx* c, int nx, int ny) {
+  if (c->h_psf.empty()) return set_err(c, FZ_ECONFIG, "no PSF set (fz_set_psf)");
+  if (c->taps_nx == nx && c->taps_ny == ny && c->d_taps) return FZ_OK;
+  const int rx = c->psf_npx / 2, ry = c->psf_npy / 2;
+  if (2 * rx + 1 > nx || 2 * ry + 1 > ny) return set_err(c, FZ_ECONFIG, "PSF wider than the map");
+  const int x0 = c->psf_npx / 2, y0 = c->psf_npy / 2, nu = 2 * ry + 1;
+  const int ntx = (2 * rx + 1 + 7) / 8 * 8;
+  // two tap tables [nu][ntx] in the kernel's window order: the convolution's
+  // (tap (t, u) = PSF(Y0 + ry - u, X0 + rx - t)) and the adjoint's
+  // (PSF(Y0 + u - ry, X0 + t - rx)); zero beyond t = 2 rx
+  std::vector<float> t((size_t)2 * nu * ntx, 0.f);
+  double sq = 0.0;
+  for (int dx = -rx; dx <= rx; ++dx)
+    for (int dy = -ry; dy <= ry; ++dy) {
+      const float v = (float)c->h_psf[(size_t)(y0 + dy) + (size_t)c->psf_npy * (x0 + dx)];
+      t[(size_t)(ry - dy) * ntx + (rx - dx)] = v;
+      t[(size_t)nu * ntx + (size_t)(ry + dy) * ntx + (rx + dx)] = v;
+      sq += (double)v * (double)v;
+    }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->d_taps) (void)hipFree(c->d_taps);
+  c->d_taps = nullptr;
+  HIPCHK(c, hipMalloc(&c->d_taps, sizeof(float) * t.size()));
+  HIPCHK(c, hipMemcpy(c->d_taps, t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+  c->taps_nx = nx;
+  c->taps_ny = ny;
+  c->taps_rx = rx;
+  c->taps_ry = ry;
+  c->taps_ntx = ntx;
+  c->taps_sq = sq;
+  return FZ_OK;
+}
+
+static void launch_psf(fz_ctx* c, int nx, int ny, int nmaps, const float* in, float* out, int adjoint, hipStream_t st) {
+  PsfArgs a;
+  a.nx = nx;
+  a.ny = ny;
+  a.nmaps = nmaps;
+  a.rx = c->taps_rx;
+  a.ry = c->taps_ry;
+  a.npix = (long)nx * ny;
+  const int ntx = c->taps_ntx;
+  a.taps = c->d_taps + (adjoint ? (size_t)(2 * a.ry + 1) * ntx : 0);
+  a.in = in;
+  a.out = out;
+  const size_t lds = sizeof(float) * (PSF_TX + ntx - 1) * (PSF_TY + 2 * a.ry);
+  const dim3 grid((unsigned)((ny + PSF_TY - 1) / PSF_TY), (unsigned)((nx + PSF_TX - 1) / PSF_TX), (unsigned)nmaps);
+  switch (ntx) {
+    case 8: hipLaunchKernelGGL(psf_conv_kernel<8>, grid, dim3(256), lds, st, a); break;
+    case 16: hipLaunchKernelGGL(psf_conv_kernel<16>, grid, dim3(256), lds, st, a); break;
+    case 24: hipLaunchKernelGGL(psf_conv_kernel<24>, grid, dim3(256), lds, st, a); break;
+    case 32: hipLaunchKernelGGL(psf_conv_kernel<32>, grid, dim3(256), lds, st, a); break;
+    case 40: hipLaunchKernelGGL(psf_conv_kernel<40>, grid, dim3(256), lds, st, a); break;
+    case 48: hipLaunchKernelGGL(psf_conv_kernel<48>, grid, dim3(256), lds, st, a); break;
+    case 56: hipLaunchKernelGGL(psf_conv_kernel<56>, grid, dim3(256), lds, st, a); break;
+    default: hipLaunchKernelGGL(psf_conv_kernel<64>, grid, dim3(256), lds, st, a); break;
+  }
+}
+
+extern "C" {
+
+int fz_psf_convolve_dev(fz_ctx* c, int nx, int ny, int nmaps, const float* d_in, float* d_out, int adjoint,
+                        void* stream) {
+  if (!c || nx < 1 || ny < 1 || nmaps < 1 || nmaps > 65535 || !d_in || !d_out || d_in == d_out) return FZ_EARG;
+  int r = psf_taps(c, nx, ny);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  launch_psf(c, nx, ny, nmaps, d_in, d_out, adjoint != 0, (hipStream_t)stream);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
 int fz_set_lsf(fz_ctx* c, const double* lsf_kernel, const int32_t* lsf_valid) {
   if (!c || !c->configured) return FZ_EARG;
   Cfg& g = c->cfg;
   HIPCHK(c, hipSetDevice(c->device));
-  c->pop_on = false;   // LTE until fz_set_departure (input_file.f90:504-508)
   g.lsf_on = 0;
   for (int r = 0; r < g.nreg; ++r) g.reg_lsf[r] = 0;
   if (lsf_kernel) {
@@ -1784,6 +1909,19 @@ static int run_micro(fz_ctx* c, int ncol, float* d_atm, int32_t** status, bool d
 int fz_set_hydrostatic(fz_ctx* c, int on) {
   if (!c) return FZ_EARG;
   c->hydro = on != 0;
+  return FZ_OK;
+}
+
+int fz_set_microturbulence(fz_ctx* c, double xi_cm_s) {
+  if (!c || !std::isfinite(xi_cm_s) || xi_cm_s < 0.0 || xi_cm_s > 1.0e7) return FZ_EARG;
+  c->xi = xi_cm_s;
+  if (!c->configured) return FZ_OK;
+  double matom[FZ_NELEM];
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyFromSymbol(matom, HIP_SYMBOL(fz_matom), sizeof(matom)));
+  set_xi(c->cfg, matom, xi_cm_s);
+  HIPCHK(c, hipMemcpyToSymbol(HIP_SYMBOL(c_cfg), &c->cfg, sizeof(Cfg)));
+  if (c->device < 64) g_owner[c->device] = c;
   return FZ_OK;
 }
 
@@ -2983,10 +3121,271 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   return FZ_OK;
 }
 
-int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
-                float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d, float* vz3d,
-                float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
-                int* iterations) {
+// INVERT3D with the spatial PSF coupling the pixels (the coupled inversion
+// whose buffers coupled_inversion.f90:10-207 allocates; the reference does
+// not wire its mathematics, get_dmodel.f90:1707-1711).  The model of every
+// pixel is fitted to the observations through the PSF: chi^2 of the map =
+// sum over pixels of the weighted squared residual obs - PSF * syn.  A
+// Levenberg-Marquardt iteration over the whole map:
+//   * forward with node-contracted response functions of the current model,
+//     PSF convolution of the synthesis, residual and chi^2 of the map;
+//   * acceptance by the map's chi^2 (pixels are coupled, so one pixel's
+//     chi^2 does not decide its own step), lambda divided by sqrt(10) on an
+//     accepted step and multiplied by 10 on a rejected one (the rules of
+//     UPDATE_LAMBDA, get_dmodel.f90:917-956, applied to the map), the best
+//     model, response functions and residual kept;
+//   * the exact gradient of the map's chi^2 for each pixel, J_q^T W (PSF^T r)_q
+//     with the adjoint convolution of the best residual, and the pixel's own
+//     block of the coupled Gauss-Newton matrix, (sum PSF^2) J_q^T W J_q; the
+//     damped solve and the node-basis update of lm_kernel.
+// The cycle schedule, SET_TAU, the 1-2-1 smoothing and the assists are those
+// of fz_invert3d_dev.  The cycle ends after MAXSTEPS rejected steps in a row
+// with no net gain (the map's step count as INV_SLA(1) counts a pixel's) or
+// the iteration cap.
+static int coupled_forward(fz_ctx* c, int nx, int ny, float* d_atm, float* d_stokes, float* d_eqv, float* d_t1,
+                           float* d_t2, hipStream_t st, bool nodes) {
+  const int numw = c->cfg.numw, ncol = nx * ny;
+  int r = nodes ? fz_forward_nodes_dev(c, ncol, d_atm, d_stokes, d_eqv, nullptr, st)
+                : fz_forward_dev(c, ncol, d_atm, FZ_RF_NONE, d_stokes, nullptr, nullptr, st);
+  if (r) return r;
+  const long nch = 4L * numw;
+  // [ncol][4 numw] -> [4 numw][ncol], convolve
+  hipLaunchKernelGGL((transpose_kernel<float, float>), dim3((unsigned)((nch + 31) / 32), (unsigned)((ncol + 31) / 32)),
+                     dim3(256), 0, st, (const float*)d_stokes, d_t1, (long)ncol, nch);
+  launch_psf(c, nx, ny, (int)nch, d_t1, d_t2, 0, st);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
+extern "C" {
+
+int fz_coupled_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs,
+                            const fz_invert_control* ctl, float* d_stokes, double* d_chi2, int* iterations,
+                            void* stream) {
+  if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
+  if (c->win_on) return set_err(c, FZ_ECONFIG, "the coupled inversion takes the dense spectral grid");
+  const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = psf_taps(c, nx, ny);
+  if (r) return r;
+  int nfreev = 0, nfreep = 0, infreepmax = 0;
+  for (int p = 0; p < 8; ++p) {
+    if (!ctl->atmpar[p]) continue;
+    const int vmax = ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+    ++nfreev;
+    nfreep += vmax;
+    infreepmax = std::max(infreepmax, vmax);
+  }
+  if (nfreev == 0) return set_err(c, FZ_ECONFIG, "no parameter inverted");
+  const int cycpow = ctl->cycpow > 0 ? ctl->cycpow : 2;
+  const int maxiter = (nfreep == nfreev) ? 1 : (int)std::floor(std::log((double)infreepmax) / std::log((double)cycpow));
+  if (maxiter < 1) return set_err(c, FZ_ECONFIG, "no inversion cycle");
+  const int maxsteps = ctl->maxsteps > 0 ? ctl->maxsteps : 5;
+  const int itcap = ctl->max_iterations > 0 ? ctl->max_iterations : 1000;
+  int nstk = 0;
+  float wsum = 0.f, wst[4];
+  for (int s = 0; s < 4; ++s) {
+    wst[s] = ctl->invstk[s] ? ctl->wstk[s] : 0.f;
+    if (ctl->invstk[s]) ++nstk;
+    wsum += wst[s] * wst[s];
+  }
+  if (nstk == 0) return set_err(c, FZ_ECONFIG, "no Stokes parameter inverted");
+  const float minw = sqrtf(wsum) / sqrtf((float)nstk);
+  const double isigma = (double)(float)(ctl->noise > 0 ? ctl->noise : 4.0e-3);
+  const int nfreq = nstk * numw;
+  std::vector<double> isig(nfreq);
+  {
+    int b = 0;
+    for (int s = 0; s < 4; ++s) {
+      if (!ctl->invstk[s]) continue;
+      const float w = wst[s] / minw;
+      for (int l = 0; l < numw; ++l) isig[b * numw + l] = (double)(w * w) / (isigma * isigma);
+      ++b;
+    }
+  }
+  int npmax = 0;
+  for (int p = 0; p < 8; ++p)
+    if (ctl->atmpar[p]) npmax += ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+  if (npmax > LM_MAXNP_BIG) return set_err(c, FZ_ECONFIG, "more than 2048 free parameters in the last cycle");
+  const size_t natm = (size_t)12 * nz * ncol, nrow = (size_t)ncol * nfreq, nmap = (size_t)4 * numw * ncol;
+  DevBuf b_isig, b_best, b_eqv, b_eqvb, b_syn, b_res, b_resb, b_gres, b_chi, b_sum, b_t1, b_t2, b_s0, b_s1;
+  HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));
+  HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * natm));
+  HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * nrow * npmax));
+  HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * nrow * npmax));
+  HIPCHK(c, hipMalloc(&b_syn.p, sizeof(double) * nrow));
+  HIPCHK(c, hipMalloc(&b_res.p, sizeof(double) * nrow));
+  HIPCHK(c, hipMalloc(&b_resb.p, sizeof(double) * nrow));
+  HIPCHK(c, hipMalloc(&b_gres.p, sizeof(double) * nrow));
+  HIPCHK(c, hipMalloc(&b_chi.p, sizeof(double) * ncol));
+  HIPCHK(c, hipMalloc(&b_sum.p, sizeof(double)));
+  HIPCHK(c, hipMalloc(&b_t1.p, sizeof(float) * std::max(nmap, nrow)));
+  HIPCHK(c, hipMalloc(&b_t2.p, sizeof(float) * std::max(nmap, nrow)));
+  HIPCHK(c, hipMalloc(&b_s0.p, sizeof(double) * (size_t)nz * ncol));
+  HIPCHK(c, hipMalloc(&b_s1.p, sizeof(double) * (size_t)nz * ncol));
+  HIPCHK(c, hipMemcpyAsync(b_isig.p, isig.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, st));
+  float* best = (float*)b_best.p;
+  float *eqv = (float*)b_eqv.p, *eqvb = (float*)b_eqvb.p;
+  float *t1 = (float*)b_t1.p, *t2 = (float*)b_t2.p;
+  double *res = (double*)b_res.p, *resb = (double*)b_resb.p, *gres = (double*)b_gres.p;
+  const unsigned gatm = (unsigned)((natm + 255) / 256), grow = (unsigned)(((size_t)nz * ncol + 255) / 256);
+  static const int rowof[8] = {A_T, A_PG, A_RHO, A_BX, A_BY, A_BZ, A_VZ, A_PG};
+  CoupledArgs ca;
+  ca.npix = ncol;
+  ca.nfreq = nfreq;
+  ca.numw = numw;
+  ca.nstk = nstk;
+  ca.conv = t2;
+  ca.obs = d_obs;
+  ca.isig = (const double*)b_isig.p;
+  ca.syn_c = (double*)b_syn.p;
+  ca.chi = (double*)b_chi.p;
+  // the map's chi^2 of the current model (forward, convolution, residual)
+  auto map_chi2 = [&](bool nodes, double* rdst, double& chi) -> int {
+    int rr = coupled_forward(c, nx, ny, d_atm, d_stokes, eqv, t1, t2, st, nodes);
+    if (rr) return rr;
+    ca.res = rdst;
+    for (int b = 0; b < 4; ++b) ca.rowstk[b] = c->lm.rowstk[b];
+    ca.inu = (double)nfreq;
+    hipLaunchKernelGGL(coupled_resid_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, ca);
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(256), 0, st, (const double*)ca.chi, ncol, (double*)b_sum.p);
+    HIPCHK(c, hipMemcpyAsync(&chi, b_sum.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    chi /= (double)ncol;
+    return FZ_OK;
+  };
+  int total_it = 0;
+  const int cyc0 = std::min(std::max(1, ctl->first_cycle), maxiter);
+  for (int cyc = cyc0; cyc <= maxiter; ++cyc) {
+    fz_inversion inv;
+    std::memset(&inv, 0, sizeof(inv));
+    for (int p = 0; p < 8; ++p) {
+      if (!ctl->atmpar[p]) continue;
+      const int vmax = ctl->nslb_max[p] > 0 ? std::min(ctl->nslb_max[p], nz) : nz;
+      inv.atmpar[p] = 1;
+      inv.nodes[p] = std::min(njevals(p + 1, cyc, maxiter, cycpow, nz), vmax);
+      inv.pen_typ[p] = ctl->pen_typ[p];
+      inv.pen_alp[p] = ctl->pen_alp[p];
+    }
+    for (int s = 0; s < 4; ++s) inv.invstk[s] = ctl->invstk[s];
+    inv.regul = ctl->regul;
+    inv.solver = ctl->solver > 0 ? ctl->solver : 1;
+    inv.node_expansion = ctl->node_expansion > 0 ? ctl->node_expansion : 1;
+    inv.maxsteps = maxsteps;
+    inv.svdtol = ctl->svdtol;
+    r = fz_set_inversion(c, &inv, nullptr, nullptr);
+    if (r) return r;
+    // SET_TAU with its assists, then the smoothing of the start model
+    if (!ctl->atmpar[1] && !ctl->atmpar[7]) {
+      if (cyc == cyc0 && ctl->assist) {
+        r = run_assist(c, ncol, d_obs, ctl->assist & 15, d_atm, st);
+        if (r) return r;
+      }
+      r = fz_forward_dev(c, ncol, d_atm, FZ_RF_NONE, d_stokes, nullptr, nullptr, stream);
+      if (r) return r;
+    }
+    if (ctl->smoothing) {
+      for (int p = 0; p < 8; ++p) {
+        if (!ctl->atmpar[p]) continue;
+        float* rc = d_atm + (size_t)rowof[p] * nz * ncol;
+        double *s0 = (double*)b_s0.p, *s1 = (double*)b_s1.p;
+        hipLaunchKernelGGL(to_double_kernel, dim3(grow), dim3(256), 0, st, rc, s0, (long)nz * ncol);
+        const int ext[3] = {nx, ny, nz};
+        for (int ax = 0; ax < 3; ++ax) {
+          if (ext[ax] <= 4) continue;
+          hipLaunchKernelGGL(smooth_axis_kernel, dim3(grow), dim3(256), 0, st, s0, s1, nz, ny, nx, ax);
+          std::swap(s0, s1);
+        }
+        hipLaunchKernelGGL(from_double_kernel, dim3(grow), dim3(256), 0, st, s0, rc, rc, (long)nz * ncol);
+      }
+    }
+    HIPCHK(c, hipGetLastError());
+    const size_t neqv = nrow * (size_t)c->lm.np;
+    double chibest = 1.0e29, lam = 10.0, steps = 0.0, precyc = 0.0;
+    for (int it = 0; it < itcap; ++it) {
+      double chi = 0.0;
+      r = map_chi2(true, res, chi);
+      if (r) return r;
+      ++total_it;
+      // UPDATE_LAMBDA on the map's chi^2
+      const bool accept = chi < chibest;
+      if (accept) {
+        if ((chibest - chi) > 0.01 * chibest) steps = std::max(0.0, steps - 1.0);
+        else steps = steps + 1.0;
+        chibest = chi;
+        lam = lam / std::sqrt(10.0);
+        hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, (const float*)d_atm, best, (long)natm);
+        HIPCHK(c, hipMemcpyAsync(eqvb, eqv, sizeof(float) * neqv, hipMemcpyDeviceToDevice, st));
+        HIPCHK(c, hipMemcpyAsync(resb, res, sizeof(double) * nrow, hipMemcpyDeviceToDevice, st));
+      } else {
+        if (std::fabs(precyc - 1.0) < 1.0e-2) {
+          precyc = 0.0;
+          lam = lam * 3.0;
+        } else {
+          lam = lam * 10.0;
+        }
+        steps = steps + 1.0;
+        hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, (const float*)best, d_atm, (long)natm);
+      }
+      lam = std::max(1.0e-2, lam);
+      // MAKE_DECISION (invert3d.f90:820-859) on the map's step count
+      if (steps >= (double)std::max(1, maxsteps - (maxiter - cyc))) break;
+      // gradient residual: the adjoint PSF convolution of the best residual
+      hipLaunchKernelGGL((transpose_kernel<double, float>), dim3((unsigned)((nfreq + 31) / 32), (unsigned)((ncol + 31) / 32)),
+                         dim3(256), 0, st, (const double*)resb, t1, (long)ncol, (long)nfreq);
+      launch_psf(c, nx, ny, nfreq, t1, t2, 1, st);
+      hipLaunchKernelGGL((transpose_kernel<float, double>), dim3((unsigned)((ncol + 31) / 32), (unsigned)((nfreq + 31) / 32)),
+                         dim3(256), 0, st, (const float*)t2, gres, (long)nfreq, (long)ncol);
+      HIPCHK(c, hipGetLastError());
+      LmArgs a = c->lm;
+      a.ncol = ncol;
+      a.obs = d_obs;
+      a.stokes = d_stokes;
+      a.syn_d = (const double*)b_syn.p;
+      a.best_syn = nullptr;
+      a.sla = nullptr;
+      a.eqv = eqvb;
+      a.eqv_best = nullptr;
+      a.isigmap = (const double*)b_isig.p;
+      a.isig_stride = 0;
+      a.atm = d_atm;
+      a.atm_best = nullptr;
+      a.done = nullptr;
+      a.delta = nullptr;
+      a.pix_of = nullptr;
+      a.poff = nullptr;
+      a.gres = gres;
+      a.hscale = c->taps_sq;
+      a.ext_lambda = lam;
+      r = launch_lm(c, a, ncol, st);
+      if (r) return r;
+    }
+    // END_CYCLE: current := best
+    hipLaunchKernelGGL(copy_f32_kernel, dim3(gatm), dim3(256), 0, st, (const float*)best, d_atm, (long)natm);
+    HIPCHK(c, hipGetLastError());
+  }
+  // the convolved synthesis of the final model and each pixel's chi^2
+  double chi = 0.0;
+  r = map_chi2(false, res, chi);
+  if (r) return r;
+  hipLaunchKernelGGL((transpose_kernel<float, float>), dim3((unsigned)((ncol + 31) / 32), (unsigned)((4 * numw + 31) / 32)),
+                     dim3(256), 0, st, (const float*)t2, d_stokes, (long)(4 * numw), (long)ncol);
+  if (d_chi2) HIPCHK(c, hipMemcpyAsync(d_chi2, b_chi.p, sizeof(double) * ncol, hipMemcpyDeviceToDevice, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  if (iterations) *iterations = total_it;
+  return FZ_OK;
+}
+
+}  // extern "C"
+
+}  // extern "C"
+
+// INVERT3D on the host cubes: upload, fz_invert3d_dev or (coupled) fz_coupled_invert3d_dev, download
+static int invert3d_host(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d,
+                         float* mw3d, float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d,
+                         float* vz3d, float* tau3d5, const double* obs3d, const fz_invert_control* ctl,
+                         double* syn3d, double* chi2, int* iterations, bool coupled) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !obs3d || !ctl || !syn3d) return FZ_EARG;
   if (c->win_on) return set_err(c, FZ_ECONFIG, "per-pixel windows: the cubes of fz_invert3d are dense");
   const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
@@ -3013,8 +3412,10 @@ int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho
   }
   // OBS3D(NFREQ,NY,NX) is already [column][NFREQ]
   HIPCHK(c, hipMemcpy(b_obs.p, obs3d, sizeof(double) * nfreq * ncol, hipMemcpyHostToDevice));
-  int rc = fz_invert3d_dev(c, nx, ny, d_atm, (const double*)b_obs.p, ctl, (float*)b_st.p, (double*)b_chi.p,
-                           iterations, nullptr);
+  const int rc = coupled ? fz_coupled_invert3d_dev(c, nx, ny, d_atm, (const double*)b_obs.p, ctl, (float*)b_st.p,
+                                                    (double*)b_chi.p, iterations, nullptr)
+                         : fz_invert3d_dev(c, nx, ny, d_atm, (const double*)b_obs.p, ctl, (float*)b_st.p,
+                                           (double*)b_chi.p, iterations, nullptr);
   if (rc) return rc;
   // the model (every parameter the inversion or the EOS changed) back to the cubes
   const int outp[] = {A_T, A_PG, A_RHO, A_PEL, A_MU, A_BX, A_BY, A_BZ, A_VZ, A_TAU};
@@ -3028,6 +3429,24 @@ int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho
   for (size_t i = 0; i < (size_t)4 * numw * ncol; ++i) syn3d[i] = (double)st[i];
   if (chi2) HIPCHK(c, hipMemcpy(chi2, b_chi.p, sizeof(double) * ncol, hipMemcpyDeviceToHost));
   return FZ_OK;
+}
+
+extern "C" {
+
+int fz_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d, float* mw3d,
+                float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d, float* vz3d,
+                float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
+                int* iterations) {
+  return invert3d_host(c, nx, ny, tem3d, pg3d, rho3d, pel3d, mw3d, bx3d, by3d, bz3d, vx3d, vy3d, vz3d, tau3d5, obs3d,
+                       ctl, syn3d, chi2, iterations, false);
+}
+
+int fz_coupled_invert3d(fz_ctx* c, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d,
+                        float* mw3d, float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d,
+                        float* vz3d, float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d,
+                        double* chi2, int* iterations) {
+  return invert3d_host(c, nx, ny, tem3d, pg3d, rho3d, pel3d, mw3d, bx3d, by3d, bz3d, vx3d, vy3d, vz3d, tau3d5, obs3d,
+                       ctl, syn3d, chi2, iterations, true);
 }
 
 }  // extern "C"

@@ -77,6 +77,16 @@ struct LmArgs {
   double* work;
   long work_stride;
   int row0;
+  // externally controlled step (the PSF-coupled inversion, whose acceptance
+  // and damping are decided over the whole map): gres non-null gives the
+  // residual of the gradient J^T W gres per pixel ([ncol][nfreq], the
+  // adjoint-convolved residual), hscale multiplies J^T W J (the sum of the
+  // squared PSF taps: the pixel's own block of the coupled normal matrix) and
+  // ext_lambda is the damping; chi^2, acceptance, INV_SLA, the best model
+  // and AM_I_DONE are left to the caller, eqv is the Jacobian of the model
+  // in atm
+  const double* gres = nullptr;
+  double hscale = 1.0, ext_lambda = 0.0;
 };
 
 // ---------------------------------------------------------------- node contraction
@@ -494,7 +504,7 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
   const long ncol = a.ncol;
   // ---- load the current model and the pixel's scalars
   for (int i = t; i < 12 * nz; i += NT) mod[i] = a.atm[(long)i * ncol + pix];
-  if (t < 9) s_sla[t] = a.sla[(long)pix * 9 + t];
+  if (t < 9) s_sla[t] = a.sla ? a.sla[(long)pix * 9 + t] : 0.0;
   // columns whose absolute sum overflows REAL(4) are zeroed (forward.f90:672-678)
   const float* ecur = a.eqv + o_ecur;
   for (int j = t; j < np; j += NT) {
@@ -504,6 +514,7 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
   }
   __syncthreads();
 
+  const bool ext = a.gres != nullptr;
   // ---- NEW_CALCULATE_CHI2 (get_dmodel.f90:960-1032)
   const double* obs = a.obs + o_obs;
   const float* stk = a.stokes + o_stk;
@@ -527,12 +538,14 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     for (int i = 0; i < np; ++i) s += pen_res[i] * 1.0 * 1.0 * pen_res[i];
     chi = chi + s;
   }
-  if (t == 0) s_accept = chi < s_sla[2] ? 1 : 0;
+  if (t == 0) s_accept = (ext || chi < s_sla[2]) ? 1 : 0;
   __syncthreads();
   const bool accept = s_accept != 0;
-  double* bsyn = a.best_syn + o_obs;
-  float* ebest = a.eqv_best + o_ebest;
-  if (accept) {
+  double* bsyn = ext ? nullptr : a.best_syn + o_obs;
+  float* ebest = ext ? nullptr : a.eqv_best + o_ebest;
+  if (ext) {
+    // the caller keeps the best model; the step starts from atm
+  } else if (accept) {
     // best := current (model, equivalent response functions, synthesis)
     for (int i = t; i < 12 * nz; i += NT) a.atm_best[(long)i * ncol + pix] = mod[i];
     for (long i = t; i < (long)np * nf; i += NT) ebest[i] = izero[i / nf] ? 0.f : ecur[i];
@@ -545,7 +558,10 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
   if (!accept && a.regul) {
     if (t == 0) lm_penalties(a, mod, pen_res, pen_hss, nod, nod + np);
   }
-  if (t == 0) {
+  if (t == 0 && ext) {
+    s_sla[1] = chi;
+    s_sla[8] = a.ext_lambda;
+  } else if (t == 0) {
     // UPDATE_LAMBDA (get_dmodel.f90:917-956) with INV_SLA(2,3,9,1,4), factor 0.01
     double chicur = chi, chipre = s_sla[2], lam = s_sla[8], steps = s_sla[0], precyc = s_sla[3];
     if ((chipre - chicur) > 0) {
@@ -609,7 +625,7 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     }
     for (int c = t; c < nc; c += NT) {
       const int f = f0 + c;
-      rch[c] = (obs[f] - bsyn[f]) * sqrt(isig(f)) / snu;
+      rch[c] = (ext ? a.gres[o_obs + f] : obs[f] - bsyn[f]) * sqrt(isig(f)) / snu;
     }
     __syncthreads();
     for (int q = t; q < nq; q += NT) {
@@ -627,6 +643,13 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
     __syncthreads();
   }
   const double lam = s_sla[8];
+  if (ext && a.hscale != 1.0) {
+    for (int e = t; e < np * np; e += NT) {
+      const int i = e / np, j = e - i * np;
+      if (j >= i) A[i * ld + j] *= a.hscale;
+    }
+    __syncthreads();
+  }
   // A = H + REG_HSS, diag += diag * lambda; g = J^T r - REG_PEN
   for (int e = t; e < np * np; e += NT) {
     const int i = e / np, j = e - i * np;
@@ -745,10 +768,10 @@ __global__ void __launch_bounds__(BIG ? LM_BIG_THREADS : LM_THREADS) lm_kernel(L
 
   // ---- write back
   for (int i = t; i < 12 * nz; i += NT) a.atm[(long)i * ncol + pix] = mod[i];
-  if (t < 9) a.sla[(long)pix * 9 + t] = s_sla[t];
+  if (t < 9 && a.sla) a.sla[(long)pix * 9 + t] = s_sla[t];
   if (a.delta)
     for (int i = t; i < np; i += NT) a.delta[(long)pix * np + i] = del[i];
-  if (t == 0 && a.done && s_sla[0] > a.maxsteps) a.done[pix] = 1;
+  if (t == 0 && !ext && a.done && s_sla[0] > a.maxsteps) a.done[pix] = 1;
 }
 
 // bytes of lm_kernel's work arrays (LDS, or the device-memory slice of one

@@ -92,9 +92,13 @@ def library():
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
+    L.fz_set_microturbulence.argtypes = [vp, ctypes.c_double]
     L.fz_set_lsf.argtypes = [vp, vp, vp]
     L.fz_set_departure.argtypes = [vp, vp, vp]
     L.fz_assist_dev.argtypes = [vp, ip, vp, ip, vp, vp]
+    L.fz_set_psf.argtypes = [vp, vp, ip, ip]
+    L.fz_coupled_invert3d_dev.argtypes = [vp, ip, ip, vp, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int), vp]
+    L.fz_psf_convolve_dev.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp]
     L.fz_set_windows.argtypes = [vp, ip, vp]
     L.fz_window_offsets.argtypes = [vp, vp]
     L.fz_synchronize.argtypes = [vp]
@@ -110,6 +114,7 @@ def library():
                                   ctypes.POINTER(ctypes.c_int), vp]
     L.fz_invert3d.argtypes = [vp, ip, ip] + [fp] * 12 + [dp, ctypes.POINTER(CInvertControl), dp, dp,
                                                        ctypes.POINTER(ctypes.c_int)]
+    L.fz_coupled_invert3d.argtypes = L.fz_invert3d.argtypes
     L.fz_autotune.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.fz_mlp_create.argtypes = [vp, ip, ip, ip, ip, ctypes.c_uint64]
     L.fz_mlp_size.argtypes = [vp]
@@ -138,9 +143,9 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
-            "fz_set_hydrostatic", "fz_set_lsf", "fz_set_departure", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
+            "fz_set_hydrostatic", "fz_set_microturbulence", "fz_set_lsf", "fz_set_departure", "fz_set_psf", "fz_psf_convolve_dev", "fz_coupled_invert3d_dev", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
-            "fz_invert3d",
+            "fz_invert3d", "fz_coupled_invert3d",
             "fz_autotune", "fz_mlp_create", "fz_mlp_size", "fz_mlp_get_params", "fz_mlp_set_params", "fz_mlp_fit_norms_dev",
             "fz_mlp_targets_dev", "fz_mlp_forward_dev", "fz_mlp_gradient_dev", "fz_mlp_train_step_dev",
             "fz_learned_init_dev", "fz_get_perturbation", "fz_get_perturbation_dev", "fz_hessian", "fz_get_dmodel3ds",
@@ -254,11 +259,39 @@ class Context:
         g = np.ascontiguousarray(np.asarray(gain, np.float32).T)
         self._check(self.lib.fz_set_gain(self.h, _f(g)), "fz_set_gain")
 
+    def set_microturbulence(self, xi_cm_s: float = 0.0):
+        """Microturbulence xi (cm/s) in every line's Doppler width
+        sqrt(2kT/m + xi^2); 0 is the reference's thermal width."""
+        self._check(self.lib.fz_set_microturbulence(self.h, float(xi_cm_s)), "fz_set_microturbulence")
+
     def set_hydrostatic(self, on: bool = True):
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside every
         forward, and the hydrostatic term in dI/dT."""
         self._check(self.lib.fz_set_hydrostatic(self.h, int(on)), "fz_set_hydrostatic")
         self.hydrostatic = bool(on)
+
+    def set_psf(self, psf=None):
+        """Spatial PSF of the coupled inversion, PSF(NPY, NPX) as READ_PSF reads
+        it (a numpy array shaped (npy, npx)); None clears it."""
+        if psf is None:
+            self._check(self.lib.fz_set_psf(self.h, None, 0, 0), "fz_set_psf")
+            return
+        p = np.asfortranarray(psf, dtype=np.float64)
+        self._psf = p
+        self._check(self.lib.fz_set_psf(self.h, p.ctypes.data, p.shape[0], p.shape[1]), "fz_set_psf")
+
+    def psf_convolve_torch(self, maps, nx, ny, adjoint=False, out=None):
+        """maps: float32 cuda tensor [nmaps][nx*ny] (col = y + ny*x); returns the
+        PSF's periodic convolution (or its adjoint)."""
+        import torch
+        assert maps.is_cuda and maps.dtype == torch.float32 and maps.is_contiguous()
+        nmaps = maps.shape[0]
+        if out is None:
+            out = torch.empty_like(maps)
+        self._check(self.lib.fz_psf_convolve_dev(self.h, nx, ny, nmaps, maps.data_ptr(), out.data_ptr(), int(adjoint),
+                                                 torch.cuda.current_stream(maps.device).cuda_stream),
+                    "fz_psf_convolve_dev")
+        return out
 
     def set_lsf(self, lsf_kernel=None, valid=None):
         """MLSF: LSF_KERNEL(NUMW) and LSF_VALID(NUML) as LSF_INIT leaves them
@@ -607,10 +640,31 @@ class Context:
                     "fz_invert3d_dev")
         return stokes, chi2, its.value
 
-    def invert3d(self, cube: dict, obs3d, control: CInvertControl):
+    def coupled_invert3d_torch(self, atm, obs, control: CInvertControl, nx: int, ny: int, stokes=None, chi2=None):
+        """INVERT3D through the spatial PSF of set_psf (fz_coupled_invert3d_dev):
+        as invert3d_torch, obs being the PSF-degraded observations; stokes
+        receives the PSF-convolved synthesis of the inverted model."""
+        import torch
+        ncol = nx * ny
+        assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous() and atm.shape[2] == ncol
+        assert obs.is_cuda and obs.dtype == torch.float64 and obs.is_contiguous() and obs.shape[0] == ncol
+        if stokes is None:
+            stokes = torch.empty(self._stokes_shape(ncol), dtype=torch.float32, device=atm.device)
+        if chi2 is None:
+            chi2 = torch.empty(ncol, dtype=torch.float64, device=atm.device)
+        its = ctypes.c_int()
+        stream = torch.cuda.current_stream(atm.device).cuda_stream
+        self._check(self.lib.fz_coupled_invert3d_dev(self.h, nx, ny, atm.data_ptr(), obs.data_ptr(),
+                                                     ctypes.byref(control), stokes.data_ptr(), chi2.data_ptr(),
+                                                     ctypes.byref(its), stream), "fz_coupled_invert3d_dev")
+        return stokes, chi2, its.value
+
+    def invert3d(self, cube: dict, obs3d, control: CInvertControl, coupled: bool = False):
         """INVERT3D on host cubes (invert3d.f90:57-178): cube as for forward3d
         (updated in place with the inverted model), obs3d (NFREQ, ny, nx) float64
-        Fortran order.  Returns (SYN3D (4*numw, ny, nx), chi2 (ny, nx), iterations)."""
+        Fortran order.  Returns (SYN3D (4*numw, ny, nx), chi2 (ny, nx), iterations).
+        coupled: the PSF-coupled inversion (fz_coupled_invert3d) with the PSF
+        of set_psf; obs3d then holds PSF-degraded profiles."""
         nz, ny, nx = cube["T"].shape
         arrs = []
         for name in ATM:
@@ -623,9 +677,9 @@ class Context:
         chi = np.zeros((ny, nx), np.float64, order="F")
         its = ctypes.c_int()
         dp = ctypes.POINTER(ctypes.c_double)
-        self._check(self.lib.fz_invert3d(self.h, nx, ny, *[_f(a) for a in arrs], obs.ctypes.data_as(dp),
-                                         ctypes.byref(control), syn.ctypes.data_as(dp), chi.ctypes.data_as(dp),
-                                         ctypes.byref(its)), "fz_invert3d")
+        fn = self.lib.fz_coupled_invert3d if coupled else self.lib.fz_invert3d
+        self._check(fn(self.h, nx, ny, *[_f(a) for a in arrs], obs.ctypes.data_as(dp), ctypes.byref(control),
+                       syn.ctypes.data_as(dp), chi.ctypes.data_as(dp), ctypes.byref(its)), fn.__name__)
         return syn, chi, its.value
 
     # ---------------------------------------------------------------- FORWARD3D mirror

@@ -23,6 +23,7 @@ MODEL_MAGIC = 130904
 PROFILE_MAGIC = 160904
 LSF_MAGIC = 3330904
 RF_MAGIC = 18060904
+PSF_MAGIC = 161906
 VERSION = (3000, 3)      # SS, VV (format 3000, version 3) written as SS+VV, SS-VV (misc.f90:944-953)
 NMAXREC = 1073741784 // 4
 ORDER = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "Pe", "mu", "tau5", "x", "y", "z")
@@ -230,3 +231,40 @@ def read_departure(path, nz):
         bu.append(float(v[1]))
         pos += 4 + n + 4
     return np.array(bl), np.array(bu)
+
+
+def write_psf(path, psf):
+    """Spatial PSF file, psf (npy, npx) like PSF(NPY, NPX): the header record
+    (VV, -VV, 161906, records, 2, NPX, NPY) with version VV = 2, then the
+    values row by row (x fastest), as READ_PSF_2D (misc.f90:2206-2348)
+    reads them and the reference's Python writer writes them."""
+    a = np.asarray(psf, np.float32)
+    if a.ndim != 2:
+        raise ValueError("a PSF is two-dimensional")
+    npy, npx = a.shape
+    header = np.array([2, -2, PSF_MAGIC, _nrecs(a.size) + 1, 2, npx, npy], np.float32)
+    _write_records(path, header, [a.reshape(-1)])
+
+
+def read_psf(path):
+    """PSF(NPY, NPX) (float64 of the file's float32 values) of a spatial PSF
+    file (READ_PSF_2D, misc.f90:2206-2348: version 2 only, two dimensions,
+    PSF(iy, ix) = value iy * NPX + ix of the data records)."""
+    recs = _records(path)
+    hdr = recs[0]
+    pos, neg = int(hdr[0]), int(hdr[1])
+    med = (pos + neg) // 2
+    if (pos - med) + (neg - med) > 0.01:
+        raise ValueError("%s: not a recognised format" % path)
+    if abs(int(round(float(hdr[2]))) - PSF_MAGIC) > 0:
+        raise ValueError("%s is not a spatial PSF file" % path)
+    if int(hdr[4]) != 2:
+        raise ValueError("%s: a spatial PSF file has two dimensions" % path)
+    vv = ((pos - med) - (neg - med)) // 2
+    if vv != 2:
+        raise ValueError("%s: PSF file version %d is not supported (the reference reads version 2)" % (path, vv))
+    npx, npy = int(hdr[5]), int(hdr[6])
+    data = np.concatenate(recs[1:]) if len(recs) > 1 else np.zeros(0, np.float32)
+    if data.size != npx * npy:
+        raise ValueError("%s: %d values, header says %d x %d" % (path, data.size, npx, npy))
+    return data.reshape(npy, npx).astype(np.float64)

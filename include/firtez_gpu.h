@@ -97,6 +97,26 @@ int fz_set_lsf(fz_ctx* ctx, const double* lsf_kernel, const int32_t* lsf_valid);
  * LTE.  Values must be finite and positive. */
 int fz_set_departure(fz_ctx* ctx, const double* popl, const double* popu);
 
+/* Spatial PSF of the coupled inversion, as READ_PSF reads it into COU_PSF
+ * (pre_post_duties.f90:488-555) for the coupled-inversion buffers of
+ * INIT_COUPLED_INVERSION_STATIC_VARS (coupled_inversion.f90:10-84):
+ * psf PSF(NPY,NPX) column-major (element (iy,ix) at iy + npy*ix), centre at
+ * (NPY/2+1, NPX/2+1), used as given (no renormalisation, like the
+ * reference's FFT kernel).  At most 63 pixels per axis.  NULL clears it
+ * (END_COUPLED_INVERSION_*, :136-207).  The PSF is a property of the
+ * context and survives fz_configure. */
+int fz_set_psf(fz_ctx* ctx, const double* psf, int npy, int npx);
+
+/* The PSF's periodic 2-D convolution of nmaps maps on the nx*ny grid
+ * (the FFT product of SMOOTH_MODEL, invert3d.f90:412-488, formed directly):
+ *   out(y,x) = sum PSF(Y0+dy, X0+dx) in(y-dy mod ny, x-dx mod nx),
+ * or with adjoint != 0 its transpose, the correlation with +dy, +dx.
+ * Maps are channel-major float, map m at [m*nx*ny + y + ny*x]; the PSF's
+ * half-widths NPX/2, NPY/2 must leave 2r+1 <= nx, ny.  d_in and d_out must
+ * differ.  Asynchronous on the stream. */
+int fz_psf_convolve_dev(fz_ctx* ctx, int nx, int ny, int nmaps, const float* d_in, float* d_out, int adjoint,
+                        void* stream);
+
 /* Per-pixel spectral windows: each of the ncol pixels observes its own
  * sub-range of every configured region.  The reference reads one set of
  * regions for the whole map (NREAD_LINES, input_file.f90:253-367: NUML
@@ -168,6 +188,14 @@ int fz_last_timing(fz_ctx* ctx, float* micro_ms, float* rt_ms, int* rt_launches)
  * hydrostatic term to dI/dT (rtesolver.f90:218-231, HYDROSTATICDER).  The P0
  * response (forward.f90:543-545) is not produced. */
 int fz_set_hydrostatic(fz_ctx* ctx, int on);
+
+/* Microturbulence xi (cm/s, 0 <= xi <= 1e7) in every line's Doppler width,
+ * sqrt(2kT/m + xi^2) in the line opacity, the damping parameter and the
+ * Voigt argument (and their T derivatives).  No counterpart in the
+ * reference, whose width is thermal only (absorption_matrix.f90:1794-1795,
+ * damping.f90:842-1012, line_opacity.f90:34-217); 0, the default, is that
+ * width exactly.  A property of the context: it survives fz_configure. */
+int fz_set_microturbulence(fz_ctx* ctx, double xi_cm_s);
 
 /* Columns processed per radiative-transfer launch (scratch is sized for this
  * tile); the autotuner sets it.  Default 8192. */
@@ -356,6 +384,27 @@ int fz_assist_dev(fz_ctx* ctx, int ncol, const double* d_obs, int flags, float* 
 int fz_invert3d_dev(fz_ctx* ctx, int nx, int ny, float* d_atm, const double* d_obs, const fz_invert_control* ctl,
                     float* d_stokes, double* d_chi2, int* iterations, void* stream);
 
+/* INVERT3D of a map observed through the spatial PSF of fz_set_psf: the
+ * coupled inversion whose state INIT_COUPLED_INVERSION_STATIC_VARS /
+ * _DYNAMIC_VARS allocate (coupled_inversion.f90:10-134) and whose
+ * mathematics the reference leaves unwired (GET_DMODEL3DC is a stub,
+ * get_dmodel.f90:1707-1711).  Arguments as fz_invert3d_dev; d_obs holds the
+ * observed (PSF-degraded) profiles.  Each LM iteration convolves the
+ * synthesis with the PSF, accepts or rejects on the chi^2 of the whole map
+ * (lambda / sqrt(10) or x 10 as UPDATE_LAMBDA, get_dmodel.f90:917-956),
+ * and steps every pixel with the exact gradient of the map's chi^2
+ * (J_q^T W (PSF^T r)_q, the adjoint convolution of the residual) and the
+ * pixel's own block of the coupled Gauss-Newton matrix ((sum PSF^2)
+ * J_q^T W J_q).  d_stokes receives the PSF-convolved synthesis of the
+ * inverted model, d_chi2 each pixel's chi^2 against d_obs.  A cycle ends
+ * when the map's step count (UPDATE_LAMBDA's STEPS, one for the map)
+ * reaches MAX(1, MAXSTEPS - (MAXITER - cycle)) (MAKE_DECISION,
+ * invert3d.f90:820-859) or at max_iterations.  Dense spectral grid only
+ * (no fz_set_windows). */
+int fz_coupled_invert3d_dev(fz_ctx* ctx, int nx, int ny, float* d_atm, const double* d_obs,
+                            const fz_invert_control* ctl, float* d_stokes, double* d_chi2, int* iterations,
+                            void* stream);
+
 /* INVERT3D on host arrays in the reference's layout (invert3d.f90:57-178
  * with its module globals): the twelve cubes (NZ,NY,NX) float are the
  * initial model on entry and the inverted model on return; OBS3D(NFREQ,NY,NX)
@@ -366,6 +415,15 @@ int fz_invert3d(fz_ctx* ctx, int nx, int ny, float* tem3d, float* pg3d, float* r
                 float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d, float* vz3d,
                 float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d, double* chi2,
                 int* iterations);
+
+/* The coupled inversion (fz_coupled_invert3d_dev) on host arrays, arguments
+ * as fz_invert3d: OBS3D holds the PSF-degraded observations, SYN3D receives
+ * the PSF-convolved synthesis of the inverted model.  The PSF is the one of
+ * fz_set_psf. */
+int fz_coupled_invert3d(fz_ctx* ctx, int nx, int ny, float* tem3d, float* pg3d, float* rho3d, float* pel3d,
+                        float* mw3d, float* bx3d, float* by3d, float* bz3d, const float* vx3d, const float* vy3d,
+                        float* vz3d, float* tau3d5, const double* obs3d, const fz_invert_control* ctl, double* syn3d,
+                        double* chi2, int* iterations);
 
 /* ---- Learned initial model.  No counterpart in the reference, which starts
  * every inversion from the input model file (invert3d.f90:57-178); these

@@ -98,6 +98,7 @@ class Cpu:
         L.fzc_forward_batch.argtypes = [ctypes.c_int, ctypes.c_int, _fp, _dp, _dp, ctypes.c_int]
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
         L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
+        L.fzc_set_microturbulence.argtypes = [ctypes.c_double]
         L.fzc_set_lsf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_set_departure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
@@ -201,6 +202,10 @@ class Cpu:
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside forward1d
         and the hydrostatic term in dI/dT."""
         self.lib.fzc_set_hydrostatic(int(on))
+
+    def set_microturbulence(self, xi_cm_s=0.0):
+        """Doppler width sqrt(2kT/m + xi^2) for every line (0: thermal only)."""
+        self.lib.fzc_set_microturbulence(float(xi_cm_s))
 
     def set_lsf(self, lsf_kernel=None, valid=None):
         """MLSF: LSF_KERNEL(NUMW) (region taps YK/SKI, tap 0 first) and

@@ -40,6 +40,11 @@ static int NLN;
 static fzc_region RG[FZC_MAXREG];
 static int NRG, NUMW, RESP;
 static int HYDRO;   /* HYDROSTATIC (and HYDROSTATICDER) */
+/* microturbulence xi^2 (cm^2 s^-2): the Doppler width sqrt(2 k T / m + xi^2)
+ * of every line, i.e. the thermal width at T + m xi^2 / (2 k) (0: the
+ * reference's thermal width, absorption_matrix.f90:1794-1795) */
+static double XI2 = 0.0;
+static double xi_t(int z) { return XI2 * fz_matom[z] * MAMU / (2.0 * KBOL); }
 static int ATMPAR[8];
 static int PIX0[FZC_MAXREG], PIX1[FZC_MAXREG];
 static double *WAVE;
@@ -546,7 +551,8 @@ static int opac_line(double BL, double BU, int li, float Tf, double NE, double N
       (nume / deno) * abf * dv->dnhyddtemp_rho + N * (dn_rho * deno - dd_rho * nume) / pow(deno, 2.0);
   double upi = s->u[z][ion - 1], dupi = s->du[z][ion - 1];
   double t1 = sqrt(DPI) * pow(QELE, 2.0) / (MELE * LIGHT) * pow(10.0, L->loggf) * L->lambda0 * 1.0e-8;
-  double t2 = sqrt(fz_matom[z] * MAMU / (2.0 * KBOL * T8));
+  const double Td = T8 + xi_t(z);   /* Doppler temperature */
+  double t2 = sqrt(fz_matom[z] * MAMU / (2.0 * KBOL * Td));
   double t3 = BL * npi / upi;
   double t4 = exp(-L->eplow * EVOLT / (KBOL * T8));
   double t5 = 1.0 - (BU / BL) * exp(-HPLA * LIGHT / (1.0e-8 * L->lambda0 * KBOL * T8));
@@ -571,7 +577,7 @@ static int opac_line(double BL, double BU, int li, float Tf, double NE, double N
     }
     dv->dnirdpg_temp = dnir[0];
     dv->dnirdrho_temp = dnir[1];
-    double d2 = -0.5 / (double)Tf * t2;
+    double d2 = -0.5 / Td * t2;
     double d3 = BL * (dnir_pg - npi * dupi) / upi;
     double d4 = L->eplow * EVOLT / (KBOL * (double)Tf * (double)Tf) * t4;
     double d5 = -HPLA * LIGHT / (1.0e-8 * L->lambda0 * KBOL * T8 * T8) * (1.0 - t5);
@@ -605,7 +611,8 @@ static double damping(int li, float Tf, double NH, derivvar_t* dv) {
   const fzc_line* L = &LN[li];
   int z = L->zn - 1;
   double T8 = (double)Tf, lam = L->lambda0 * 1e-8;
-  double vd = sqrt(2.0 * KBOL * T8 / (MAMU * fz_matom[z]));
+  const double Td = T8 + xi_t(z);   /* Doppler temperature */
+  double vd = sqrt(2.0 * KBOL * Td / (MAMU * fz_matom[z]));
   double grad = 0.2223 / pow(lam, 2.0);
   double muh = MAMU * (fz_matom[z] * fz_matom[0]) / (fz_matom[z] + fz_matom[0]);
   double muhe = MAMU * (fz_matom[z] * fz_matom[1]) / (fz_matom[z] + fz_matom[1]);
@@ -650,8 +657,8 @@ static double damping(int li, float Tf, double NH, derivvar_t* dv) {
   }
   double damp = (grad + gcol) * lam / (4.0 * DPI * vd);
   if (RESP) {
-    dv->ddamdtemp_pg = damp / (grad + gcol) * dgp - damp / vd * (vd / (2.0 * T8));
-    dv->ddamdtemp_rho = damp / (grad + gcol) * dgr - damp / vd * (vd / (2.0 * T8));
+    dv->ddamdtemp_pg = damp / (grad + gcol) * dgp - damp / vd * (vd / (2.0 * Td));
+    dv->ddamdtemp_rho = damp / (grad + gcol) * dgr - damp / vd * (vd / (2.0 * Td));
     dv->ddamdpg_temp = lam / (4.0 * DPI * vd) * gcol / NH * dv->dnhyddpg_temp;
     dv->ddamdrho_temp = lam / (4.0 * DPI * vd) * gcol / NH * dv->dnhyddrho_temp;
   }
@@ -820,6 +827,7 @@ int fzc_numw(void) { return NUMW; }
 void fzc_wave(double* w) { memcpy(w, WAVE, sizeof(double) * NUMW); }
 void fzc_set_gain(const float* g) { memcpy(GAIN, g, sizeof(float) * 4 * NUMW); }
 void fzc_set_hydrostatic(int on) { HYDRO = on ? 1 : 0; }
+void fzc_set_microturbulence(double xi) { XI2 = xi * xi; }
 
 /* lsf_kernel: NUMW doubles, region r's taps YK/SKI at PIX0[r]..PIX1[r]-1 with
  * tap 0 first (LSF_INIT builds them so: pre_post_duties.f90:855-876); valid:
@@ -946,7 +954,8 @@ static int abs_mat_block(int K, int L, int B, float Tf, const float* bx, const f
     for (int c = 0; c < 7; ++c)
       for (int l = p0; l < p1; ++l) DETA(de, p, c, l) = 0.0;
   double T8 = (double)Tf;
-  double vdop = sqrt(2.0 * KBOL * T8 / (fz_matom[LL->zn - 1] * MAMU));
+  const double Td = T8 + xi_t(LL->zn - 1);   /* Doppler temperature */
+  double vdop = sqrt(2.0 * KBOL * Td / (fz_matom[LL->zn - 1] * MAMU));
   double dld = 1.0e3 * (vdop * LL->lambda0 / LIGHT);
   double BX = bx[0], BY = by[0], BZ = bz[0];
   double bx2 = pow(BX, 2.0), by2 = pow(BY, 2.0), bz2 = pow(BZ, 2.0);
@@ -1027,7 +1036,7 @@ static int abs_mat_block(int K, int L, int B, float Tf, const float* bx, const f
       double dl_dd = dh_dv;
       double dh_dd = 2.0 * (-1.0 / sqrt(DPI) + damp * hv + u * fv) * str;
       double dl_dv = -dh_dd;
-      double duudt = -0.5 / T8 * u;
+      double duudt = -0.5 / Td * u;
       double shh = str * hv, sff = str * fv;
       for (int c = 0; c < 7; ++c) {
         int isf = c >= 4;

@@ -59,6 +59,8 @@ void fzc_set_gain(const float* gain);
 /* HYDROSTATIC=yes (and HYDROSTATICDER): the forward re-integrates Pg from
  * the top (RK4_INTEGRATION_LOG) and adds the hydrostatic term to dI/dT. */
 void fzc_set_hydrostatic(int on);
+/* microturbulence xi (cm/s) of every line's Doppler width; 0 (the default): thermal only */
+void fzc_set_microturbulence(double xi);
 int fzc_set_lsf(const double* lsf_kernel, const int32_t* valid);
 
 /* One column (forward.f90:857-1029, HYDROSTATIC=no):

@@ -192,3 +192,33 @@ def test_misc_setup_assists():
     assert control.parse(base + "MISC SETUP:\nPGAS\nEND:\n").assist == gpu.ASSIST_P
     with pytest.raises(NotImplementedError):
         control.parse(base + "MISC SETUP:\nFULL_STOKES\nEND:\n")
+
+
+def test_coupled_inversion_section(tmp_path):
+    """NSET_COUPLED_INVERSION (input_file.f90:1524-1607): a yes/no row and
+    FILENAME <psf file> (BLCKSZ / NMTHRD / PSFRAD accepted); yes without a
+    FILENAME and a row of three columns are errors.  READ_PSF_2D's file
+    (misc.f90:2206-2348): the header record (2, -2, 161906, records, 2, NPX,
+    NPY) and the values row by row, x fastest, written and read back."""
+    from firtez_dz_amd import modelio
+    base = ("LINES:\n4 20 -200.0 20.0\nFILEPROFILE:\np.bin\nFILEMODEL:\nm.bin\nMODE:\ninversion\n"
+            "STOKES SETUP:\nSTKI 1\nINVERSION SETUP:\nTEM 2\n")
+    c = control.parse(base + "COUPLED INVERSION:\nyes\nFILENAME psf.bin\nPSFRAD 5\nEND:\n")
+    assert c.coupled and c.psf_file == "psf.bin"
+    c = control.parse(base + "COUPLED INVERSION:\nno\nFILENAME psf.bin\n")
+    assert not c.coupled
+    with pytest.raises(ValueError):
+        control.parse(base + "COUPLED INVERSION:\nyes\n")
+    with pytest.raises(ValueError):
+        control.parse(base + "COUPLED INVERSION:\nyes\nFILENAME a b\n")
+    psf = np.arange(15, dtype=np.float64).reshape(3, 5) / 7.0        # PSF(NPY=3, NPX=5)
+    f = str(tmp_path / "psf.bin")
+    modelio.write_psf(f, psf)
+    raw = open(f, "rb").read()
+    hdr = np.frombuffer(raw, np.float32, 7, 4)
+    assert np.frombuffer(raw, np.int32, 1, 0)[0] == 28
+    assert list(hdr) == [2, -2, 161906, 2, 2, 5, 3]
+    data = np.frombuffer(raw, np.float32, 15, 4 + 28 + 4 + 4)
+    assert data[1] == np.float32(psf[0, 1]) and data[5] == np.float32(psf[1, 0])      # x fastest
+    back = modelio.read_psf(f)
+    assert back.shape == (3, 5) and np.array_equal(back, psf.astype(np.float32).astype(np.float64))

@@ -152,3 +152,82 @@ def test_syn1_with_line_spread_function(gpu_ctx, golden, tree):
     report("test_syn1 + LSF vs convolved profiles", float(np.max(np.abs(st1[:, :, 0, 0] - ref))), 1e-5)
     assert float(np.max(np.abs(st1 - st0))) > 1e-3          # the LSF changed the profiles
     gpu_ctx.set_lsf(None)
+
+
+def test_coupled_inversion_control_file(gpu_ctx, tmp_path):
+    """MODE inversion with "COUPLED INVERSION:" yes / FILENAME: the runner
+    reads the PSF file (READ_PSF_2D's format under DATAPATH), inverts the
+    PSF-degraded profiles through it and writes the reference's outputs; they
+    equal the library's coupled inversion called directly on the same files,
+    bit for bit, and the context is left without a PSF."""
+    import torch
+    from firtez_dz_amd import atmos, gpu
+    from oracle import psf as P
+    nx, ny, nz = 4, 3, 48
+    text = ("LINES:\n4,5 24 -240.0 20.0\nFILEPROFILE:\nobs.bin\nFILEMODEL:\nstart.bin\nMODE:\ninversion\n"
+            "STOKES SETUP:\nSTKI 1\nSTKQ 1\nSTKU 1\nSTKV 1\n"
+            "INVERSION SETUP:\nTEM 1\nBX 1\nBY 1\nBZ 1\nVLOS 1\n"
+            "MISC SETUP:\nDATAPATH ./\nMODPATH ./\nOUTPATH out/\nNOISE 1e-3\n"
+            "COUPLED INVERSION:\nyes\nFILENAME psf.bin\n")
+    (tmp_path / "control_file.dat").write_text(text)
+    (tmp_path / "out").mkdir()
+    c = control.parse(text)
+    ls = control.line_setup(c)
+    atm, z = atmos.seeded(nx, ny, nz, seed=41)
+    gpu_ctx.configure(ls, z)
+    gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert int(gpu_ctx.hydrostatic_torch(t).max()) == 0
+    a = t.cpu().numpy()
+    names = {"T": 0, "Pg": 1, "rho": 2, "Pe": 3, "mu": 4, "Bx": 5, "By": 6, "Bz": 7, "vx": 8, "vy": 9, "vz": 10,
+             "tau5": 11}
+    cube = {n: np.asfortranarray(a[i].reshape(nz, nx, ny).transpose(0, 2, 1)) for n, i in names.items()}
+    syn = gpu_ctx.forward3d(cube)
+    syn = syn[0] if isinstance(syn, tuple) else syn                  # SYN3D (4 numw, ny, nx)
+    numw = gpu_ctx.numw
+    nf = 4 * numw
+    psf = np.random.default_rng(2).random((3, 3)) + np.pad([[3.0]], 1)
+    psf /= psf.sum()
+    modelio.write_psf(str(tmp_path / "psf.bin"), psf)
+    psf = modelio.read_psf(str(tmp_path / "psf.bin"))
+    maps = np.asarray(syn).reshape(nf, ny * nx, order="F").reshape(nf, nx, ny)
+    obs = P.convolve(maps, psf).reshape(nf, nx, ny).transpose(0, 2, 1)             # (nf, ny, nx)
+    obs = obs + np.random.default_rng(3).normal(0, 1e-3, obs.shape)
+    st = obs.reshape(numw, 4, ny, nx, order="F").transpose(1, 0, 2, 3)
+    wave, index = control._wave_index(ls)
+    modelio.write_profiles(str(tmp_path / "obs.bin"), st, wave, index)
+    start = {n: np.asfortranarray(v.copy()) for n, v in cube.items()}
+    start["T"] += np.float32(120.0)
+    start["vz"] += np.float32(4e4)
+    start["Bx"] *= np.float32(0.8)
+    start["z_km"] = np.asarray(z, np.float64)
+    modelio.write_model(str(tmp_path / "start.bin"), start)
+    print("[control] coupled inversion control file: %dx%dx%d, PSF 3x3" % (nx, ny, nz), flush=True)
+    out = control.run(c, str(tmp_path), gpu_ctx)
+    with pytest.raises(gpu.FzError):
+        gpu_ctx.psf_convolve_torch(torch.zeros((1, nx * ny), device="cuda"), nx, ny)
+    # the same inversion called directly on the files
+    cube2 = modelio.read_model(str(tmp_path / "start.bin"))
+    for n in ("vx", "vy"):
+        cube2.setdefault(n, np.zeros_like(cube2["T"], order="F"))
+    prof, _, _ = modelio.read_profiles(str(tmp_path / "obs.bin"))
+    obs3d = np.asfortranarray(np.concatenate([prof[s] for s in range(4)], axis=0).astype(np.float64))
+    gpu_ctx.set_psf(modelio.read_psf(str(tmp_path / "psf.bin")))
+    try:
+        syn_d, chi_d, its_d = gpu_ctx.invert3d(cube2, obs3d, control.control_for_inversion(c, nz), coupled=True)
+    finally:
+        gpu_ctx.set_psf(None)
+    chi = np.asarray(out["chi2"])
+    print("[control] coupled: %d iterations, mean chi2 %.4g" % (out["iterations"], float(chi.mean())), flush=True)
+    assert out["iterations"] == its_d and np.array_equal(chi, chi_d)
+    m = modelio.read_model(str(tmp_path / "out" / "out_start.bin"))
+    for n in ("T", "Bx", "By", "Bz", "vz"):
+        assert np.array_equal(m[n], cube2[n]), n
+    w, _, _ = modelio.read_profiles(str(tmp_path / "out" / "out_obs.bin"))
+    assert np.array_equal(w, _st4(syn_d, numw).astype(np.float32))
+    assert float(chi.mean()) < 2.0
+
+
+def _st4(syn3d, numw):
+    ny, nx = syn3d.shape[1:]
+    return np.asarray(syn3d).reshape((numw, 4, ny, nx), order="F").transpose(1, 0, 2, 3)

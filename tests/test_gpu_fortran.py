@@ -1,9 +1,11 @@
 """The Fortran side of the boundary on the GPU: tests/fortran/fz_caller.f90,
 compiled against the FIRTEZ_GPU module and the library by
 firtez_dz_amd.build.build_fortran(), calls FZ_FORWARD3D (spherical response
-functions), FZ_INVERT3D, FZ_GET_PERTURBATION (including a rejected DTYPE)
-and FZ_HESSIAN on inputs written here; every output must equal the same
-entry point called through ctypes on the same inputs, bit for bit."""
+functions), FZ_INVERT3D, FZ_GET_PERTURBATION (including a rejected DTYPE),
+FZ_HESSIAN and, between FZ_INIT_COUPLED_INVERSION and FZ_END_COUPLED_INVERSION,
+FZ_COUPLED_INVERT3D (refused once the PSF is gone) on inputs written here;
+every output must equal the same entry point called through ctypes on the
+same inputs, bit for bit."""
 import os
 import subprocess
 
@@ -44,6 +46,8 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
     hssr = 0.05 * (L.T @ L)
     lam, tol = 0.3, 1e-4
     jac = rng.normal(size=(np_, nf))
+    # a 1 x 3 PSF (the 3 x 2 map allows half-widths 1 along x, 0 along y)
+    psf = np.array([[0.2, 0.7, 0.1]])
     cl, cr = lines.c_arrays(ls)
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as fh:
@@ -60,6 +64,7 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
         fh.write(np.asfortranarray(ijac).tobytes(order="F") + diff.tobytes() + pen.tobytes() +
                  np.asfortranarray(hssr).tobytes(order="F") + np.array([lam, tol]).tobytes() +
                  np.asfortranarray(jac).tobytes(order="F"))
+        fh.write(np.array(psf.shape, np.int32).tobytes() + np.asfortranarray(psf).tobytes(order="F"))
     print("[fortran] running fz_caller", flush=True)
     r = subprocess.run([CALLER, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
     print(r.stdout[-500:], r.stderr[-500:], flush=True)
@@ -67,10 +72,17 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
     # the same calls through ctypes
     cube = {n: np.asfortranarray(cube4[p].transpose(2, 1, 0)) for p, n in enumerate(gpu.ATM)}
     syn, dsyn = gpu_ctx.forward3d(cube, rf_mode=gpu.RF_SPHERICAL)
+    inv_start = inv4.copy()        # the views below let invert3d overwrite inv4
     inv = {n: np.asfortranarray(inv4[p].transpose(2, 1, 0)) for p, n in enumerate(gpu.ATM)}
     syni, chi, its = gpu_ctx.invert3d(inv, syn, ctl)
     delta = gpu_ctx.get_perturbation(ijac, diff, lam, pen, hssr, dtype=1, svdtol=tol)
     hss = gpu_ctx.hessian(jac)
+    invc = {n: np.asfortranarray(inv_start[p].transpose(2, 1, 0)) for p, n in enumerate(gpu.ATM)}
+    gpu_ctx.set_psf(psf)
+    try:
+        sync, chic, itsc = gpu_ctx.invert3d(invc, syn, ctl, coupled=True)
+    finally:
+        gpu_ctx.set_psf(None)
     numw = gpu_ctx.numw
     with open(fout, "rb") as fh:
         f_numw, f_its = np.frombuffer(fh.read(8), np.int32)
@@ -83,6 +95,10 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
         f_chi = rd(ny * nx, np.float64).reshape(nx, ny)
         f_delta = rd(np_, np.float64)
         f_hss = rd(np_ * np_, np.float64).reshape(np_, np_).T
+        f_itsc = int(rd(1, np.int32)[0])
+        f_invc = rd(12 * nx * ny * nz, np.float32).reshape(12, nx, ny, nz)
+        f_sync = rd(4 * numw * ny * nx, np.float64).reshape(nx, ny, 4 * numw)
+        f_chic = rd(ny * nx, np.float64).reshape(nx, ny)
     assert f_numw == numw and f_its == its
     F = lambda a: np.ascontiguousarray(np.asarray(a).transpose(tuple(range(a.ndim))[::-1]))
     assert np.array_equal(f_syn, F(syn)) and np.array_equal(f_dsyn, F(dsyn))
@@ -91,5 +107,8 @@ def test_fortran_caller_matches_python(gpu_ctx, tmp_path):
         assert np.array_equal(f_inv[p], F(inv[n])), n
     assert np.array_equal(f_syni, F(syni)) and np.array_equal(f_chi, F(chi))
     assert np.array_equal(f_delta, delta) and np.array_equal(f_hss, hss)
-    print("[fortran] FZ_FORWARD3D, FZ_INVERT3D (%d iterations), FZ_GET_PERTURBATION, FZ_HESSIAN: "
-          "identical to the ctypes calls" % its, flush=True)
+    assert f_itsc == itsc and np.array_equal(f_sync, F(sync)) and np.array_equal(f_chic, F(chic))
+    for p, n in enumerate(gpu.ATM):
+        assert np.array_equal(f_invc[p], F(invc[n])), n
+    print("[fortran] FZ_FORWARD3D, FZ_INVERT3D (%d iterations), FZ_GET_PERTURBATION, FZ_HESSIAN, "
+          "FZ_COUPLED_INVERT3D (%d iterations): identical to the ctypes calls" % (its, itsc), flush=True)

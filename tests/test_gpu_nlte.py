@@ -124,3 +124,41 @@ def test_unit_departure_is_lte(gpu_ctx):
     assert np.array_equal(st0, st2) and np.array_equal(j0, j2)
     with pytest.raises(gpu.FzError):
         gpu_ctx.set_departure(np.zeros((1, 32)), np.ones((1, 32)))
+
+
+def test_departure_survives_lsf_and_resets_with_configure(gpu_ctx):
+    """fz_set_lsf keeps the departure coefficients (the order of the two
+    set-up calls does not matter); fz_configure resets them to LTE."""
+    import torch
+    ls = lines.fe6301_6302(nwave=56, start=-600.0, step=40.0)
+    atm, z = atmos.seeded(2, 2, 32, seed=9)
+    gpu_ctx.configure(ls, z)
+    gpu_ctx.gain_hsra()
+    t = torch.from_numpy(atm.copy()).cuda()
+    assert int(gpu_ctx.hydrostatic_torch(t).max()) == 0
+    atm = t.cpu().numpy()
+    lte, _, _ = _run(gpu_ctx, atm, gpu.RF_NONE)
+    x = np.linspace(0.0, 1.0, 32)
+    popl, popu = (1.0 + 0.3 * x)[None, :], (1.0 - 0.3 * x)[None, :]
+    kern, valid = lines.lsf_kernel(ls, [40.0], [0.0])
+    try:
+        gpu_ctx.set_departure(popl, popu)
+        gpu_ctx.set_lsf(kern, valid)
+        a, _, _ = _run(gpu_ctx, atm, gpu.RF_NONE)
+        gpu_ctx.set_lsf(None)
+        gpu_ctx.set_departure(None)
+        gpu_ctx.set_lsf(kern, valid)
+        gpu_ctx.set_departure(popl, popu)
+        b, _, _ = _run(gpu_ctx, atm, gpu.RF_NONE)
+        gpu_ctx.set_departure(None)
+        c, _, _ = _run(gpu_ctx, atm, gpu.RF_NONE)
+    finally:
+        gpu_ctx.set_lsf(None)
+        gpu_ctx.set_departure(None)
+    assert np.array_equal(a, b)
+    assert np.max(np.abs(a - c)) > 1e-3
+    gpu_ctx.set_departure(popl, popu)
+    gpu_ctx.configure(ls, z)
+    gpu_ctx.gain_hsra()
+    d, _, _ = _run(gpu_ctx, atm, gpu.RF_NONE)
+    assert np.array_equal(d, lte)
