@@ -16,7 +16,7 @@
 //                  propagation matrix with derivatives, the local response
 //                  dO(I_in - S) + (1-O) dS, and propagates it to the top with
 //                  the running product O_nz...O_{k+1} (forward.f90:511-548).
-//   hsra_kernel / gain_kernel : continuum normalisation (normalization.f90).
+//   hsra_saha/rho/opac_kernel, gain_kernel : continuum normalisation (normalization.f90).
 //   saha_mid/he_rk4 kernels : hydrostatic RK4 from the top
 //                  (hydrostatic_tools.f90:150-396), standalone or inside the
 //                  forward with HYDROSTATIC: yes.
@@ -1013,24 +1013,64 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
 // ---------------------------------------------------------------- HSRA normalisation
 // Equation of state and continuum opacity along the embedded HSRA model
 // (normalization.f90:696-754); one thread, heights in order (warm start).
-__global__ void hsra_kernel(double* __restrict__ kc_out /*[387][nreg]*/, int32_t* __restrict__ err_out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  Saha s;
-  Deriv d = {};
+// HSRA's continuum opacities at the regions' central wavelengths
+// (normalization.f90:696-754): the EOS of every HSRA height warm-started
+// from the height below, as GET_RHO runs it there.  The Saha factors of all
+// heights and elements are independent (one thread each), the fixed point is
+// sequential in height (one group of RHO_G lanes, the fold in the
+// reference's order, so the iteration path is get_rho's bit for bit), and the
+// opacities are independent again (one thread per height and region).
+__global__ void __launch_bounds__(256) hsra_saha_kernel(double* __restrict__ sb /*[387][92][4]*/) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= FZ_HSRA_LEN * FZ_NELEM) return;
+  const int k = t / FZ_NELEM, i = t - k * FZ_NELEM;
+  const double T = (double)fz_hsra_t[k];
+  const double lam = g_c.csaha1 * pow(T, -1.5);
+  double a1, a2, da1, da2;
+  saha_element(i, T, lam, -3.0 / (2.0 * T), false, a1, a2, da1, da2);
+  double* o = sb + ((long)k * FZ_NELEM + i) * 4;
+  o[0] = a1;
+  o[1] = a2;
+}
+
+__global__ void __launch_bounds__(64) hsra_rho_kernel(const double* __restrict__ sb, double* __restrict__ eos /*[2][387]*/,
+                                                      int32_t* __restrict__ err_out /*[2]: code, first bad height*/) {
+  __shared__ double sh[RHO_ROW];
+  const int q = threadIdx.x;
+  if (q >= RHO_G) return;
   double ne = 0.0;
-  int err = 0;
-  for (int k = 0; k < FZ_HSRA_LEN && !err; ++k) {
-    const double T = (double)fz_hsra_t[k], P = (double)fz_hsra_pg[k];
-    saha_init(s, T, false);
-    double nh, pel, dens, mu;
-    err = get_rho(s, T, P, ne, nh, pel, dens, mu);
-    for (int r = 0; r < c_cfg.nreg && !err; ++r) {
-      double kc[4];
-      err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, false, kc);
-      kc_out[k * c_cfg.nreg + r] = kc[0];
+  int err = ERR_OK, kbad = FZ_HSRA_LEN;
+  for (int k = 0; k < FZ_HSRA_LEN; ++k) {
+    const RhoOut o = rho_group_at(sb + (long)k * FZ_NELEM * 4, 1, q, sh, (double)fz_hsra_t[k],
+                                  (double)fz_hsra_pg[k], ne);
+    if (o.err) {
+      err = o.err;
+      kbad = k;
+      break;
+    }
+    ne = o.ne;
+    if (q == 0) {
+      eos[k] = o.ne;
+      eos[FZ_HSRA_LEN + k] = o.nh;
     }
   }
-  *err_out = err;
+  if (q == 0) {
+    err_out[0] = err;
+    err_out[1] = kbad;
+  }
+}
+
+__global__ void __launch_bounds__(256) hsra_opac_kernel(const double* __restrict__ eos, int32_t* __restrict__ err,
+                                                        double* __restrict__ kc_out /*[387][nreg]*/) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= FZ_HSRA_LEN * c_cfg.nreg) return;
+  const int k = t / c_cfg.nreg, r = t - k * c_cfg.nreg;
+  if (k >= err[1]) return;
+  Deriv d = {};
+  double kc[4];
+  const int e = opac_cont((double)fz_hsra_t[k], c_cfg.reg_l0main[r], eos[k], eos[FZ_HSRA_LEN + k], d, false, kc);
+  kc_out[k * c_cfg.nreg + r] = kc[0];
+  if (e) atomicMax(&err[2], e);
 }
 
 // Continuum formal solution through HSRA per wavelength; every step is
@@ -1195,7 +1235,7 @@ struct fz_ctx {
   size_t micro_cap = 0;
   float* d_scratch = nullptr;
   size_t scratch_cap = 0;
-  int tile = 8192;
+  int tile = 32768;
   int block = 256;
   bool prof = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -1743,14 +1783,25 @@ int fz_gain_hsra(fz_ctx* c, float* gain) {
   HIPCHK(c, hipSetDevice(c->device));
   int ru = upload_cfg(c);
   if (ru) return ru;
-  DevBuf b_kc, b_err;
+  DevBuf b_kc, b_err, b_sb, b_eos;
   HIPCHK(c, hipMalloc(&b_kc.p, sizeof(double) * FZ_HSRA_LEN * c->cfg.nreg));
-  HIPCHK(c, hipMalloc(&b_err.p, sizeof(int32_t)));
+  HIPCHK(c, hipMalloc(&b_err.p, sizeof(int32_t) * 3));
+  HIPCHK(c, hipMalloc(&b_sb.p, sizeof(double) * FZ_HSRA_LEN * FZ_NELEM * 4));
+  HIPCHK(c, hipMalloc(&b_eos.p, sizeof(double) * 2 * FZ_HSRA_LEN));
   double* kc = (double*)b_kc.p;
   int32_t* derr = (int32_t*)b_err.p;
-  hipLaunchKernelGGL(hsra_kernel, dim3(1), dim3(64), 0, 0, kc, derr);
-  int32_t e = 0;
-  HIPCHK(c, hipMemcpy(&e, derr, sizeof(int32_t), hipMemcpyDeviceToHost));
+  const int32_t zero3[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpy(derr, zero3, sizeof(zero3), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(hsra_saha_kernel, dim3((FZ_HSRA_LEN * FZ_NELEM + 255) / 256), dim3(256), 0, 0, (double*)b_sb.p);
+  hipLaunchKernelGGL(hsra_rho_kernel, dim3(1), dim3(64), 0, 0, (const double*)b_sb.p, (double*)b_eos.p, derr);
+  hipLaunchKernelGGL(hsra_opac_kernel, dim3((FZ_HSRA_LEN * c->cfg.nreg + 255) / 256), dim3(256), 0, 0,
+                     (const double*)b_eos.p, derr, kc);
+  HIPCHK(c, hipGetLastError());
+  int32_t eh[3] = {0, 0, 0};
+  HIPCHK(c, hipMemcpy(eh, derr, sizeof(eh), hipMemcpyDeviceToHost));
+  // the first failure in height order, as the sequential loop meets it: the
+  // EOS of height kbad, or an opacity of a height below it
+  const int32_t e = eh[2] ? eh[2] : eh[0];
   if (e == ERR_LAMBDA) return set_err(c, e, "HSRA normalisation: wavelength below 1800 A");
   if (e) return set_err(c, e, "HSRA normalisation: equation of state or opacity failed");
   hipLaunchKernelGGL(gain_kernel, dim3((c->cfg.numw + 63) / 64), dim3(64), 0, 0, kc, c->d_gain);

@@ -198,7 +198,7 @@ int fz_set_hydrostatic(fz_ctx* ctx, int on);
 int fz_set_microturbulence(fz_ctx* ctx, double xi_cm_s);
 
 /* Columns processed per radiative-transfer launch (scratch is sized for this
- * tile); the autotuner sets it.  Default 8192. */
+ * tile); the autotuner sets it.  Default 32768. */
 int fz_set_tile(fz_ctx* ctx, int columns_per_launch, int block_threads);
 
 /* Autotuner (no counterpart in the reference): times fz_forward_dev on the
