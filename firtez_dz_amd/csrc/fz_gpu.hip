@@ -1611,6 +1611,9 @@ int fz_set_psf(fz_ctx* c, const double* psf, int npy, int npx) {
   }
   if (npx < 1 || npy < 1 || npx / 2 > PSF_RMAX || npy / 2 > PSF_RMAX)
     return set_err(c, FZ_EARG, "PSF size outside 1 .. 63 pixels per axis");
+  // READ_PSF takes NPX/2 pixels either side of (NPY/2 + 1, NPX/2 + 1): an even
+  // side would reach one pixel past the array
+  if (npx % 2 == 0 || npy % 2 == 0) return set_err(c, FZ_EARG, "PSF sides must be odd");
   for (long i = 0; i < (long)npx * npy; ++i)
     if (!std::isfinite(psf[i])) return set_err(c, FZ_EARG, "PSF values must be finite");
   c->h_psf.assign(psf, psf + (size_t)npx * npy);

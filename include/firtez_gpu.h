@@ -102,7 +102,9 @@ int fz_set_departure(fz_ctx* ctx, const double* popl, const double* popu);
  * INIT_COUPLED_INVERSION_STATIC_VARS (coupled_inversion.f90:10-84):
  * psf PSF(NPY,NPX) column-major (element (iy,ix) at iy + npy*ix), centre at
  * (NPY/2+1, NPX/2+1), used as given (no renormalisation, like the
- * reference's FFT kernel).  At most 63 pixels per axis.  NULL clears it
+ * reference's FFT kernel).  Odd sides of at most 63 pixels (the reference
+ * takes NPX/2 pixels either side of the centre, which an even side does not
+ * hold).  NULL clears it
  * (END_COUPLED_INVERSION_*, :136-207).  The PSF is a property of the
  * context and survives fz_configure. */
 int fz_set_psf(fz_ctx* ctx, const double* psf, int npy, int npx);

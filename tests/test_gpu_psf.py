@@ -48,9 +48,34 @@ def test_convolution_vs_oracle(gpu_ctx, nx, ny, psf_kind):
         gpu_ctx.set_psf(None)
 
 
+def test_bench_shape_and_widest_psf(gpu_ctx):
+    """The coupled benchmark's shape (448 maps of 256 x 256, the 31 x 31
+    Airy + wing PSF) and the widest PSF the kernel takes (63 x 63, the
+    64-tap instantiation) on a non-square map, against the oracle."""
+    rng = np.random.default_rng(4)
+    for nx, ny, nm, psf in ((256, 256, 448, atmos.airy_gauss_psf(31, 2.44, 4.0, 0.2)),
+                            (70, 128, 3, rng.random((63, 63)))):
+        gpu_ctx.set_psf(psf)
+        try:
+            maps = rng.normal(size=(nm, nx, ny)).astype(np.float32)
+            for adj in (False, True):
+                got = _dev(gpu_ctx, maps, nx, ny, adj)
+                ref = P.convolve(maps.astype(np.float64), psf.astype(np.float32).astype(np.float64), adjoint=adj)
+                err = np.abs(got - ref).max() / np.abs(ref).max()
+                # FP32 accumulation over the taps: the rounding grows like sqrt(taps)
+                tol = 1e-7 * np.sqrt(psf.size)
+                print("[psf] %d maps %dx%d, PSF %dx%d adjoint=%d: max err %.2e of max |out| (tol %.1e)"
+                      % (nm, nx, ny, psf.shape[0], psf.shape[1], adj, err, tol), flush=True)
+                assert err < tol
+        finally:
+            gpu_ctx.set_psf(None)
+
+
 def test_psf_errors(gpu_ctx):
     import torch
     t = torch.zeros((1, 20 * 20), dtype=torch.float32, device="cuda")
+    with pytest.raises(gpu.FzError, match="odd"):
+        gpu_ctx.set_psf(np.ones((4, 5)))
     with pytest.raises(gpu.FzError, match="no PSF"):
         gpu_ctx.psf_convolve_torch(t, 20, 20)
     gpu_ctx.set_psf(atmos.airy_gauss_psf(31, 2.5, 6.0))
