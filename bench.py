@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="with --lm: per-pixel line sets and windows (8-line list, 1-4 lines, 60-300 wavelengths)")
     ap.add_argument("--micro", type=float, default=0.5, help="microturbulence (km/s) of the forward workload")
+    ap.add_argument("--train-cubes-coupled", type=int, default=4,
+                    help="with --coupled: seeded 256x256 cubes of PSF-degraded training pairs")
     ap.add_argument("--coupled", action="store_true",
                     help="PSF-coupled inversion of a 256x256x128 cube through a 31x31 Airy + wing PSF")
     args = ap.parse_args()
@@ -697,6 +699,39 @@ def bench_coupled(args):
     uncoupled = dict(lm_stats(model, truth, chi_u.cpu().numpy()), lm_iterations=its_u, seconds=du)
     print("[bench-coupled] uncoupled: %.1f s, %d iterations, median chi2 %.3g, mean |T - truth| %.1f K"
           % (du, its_u, uncoupled["median_chi2"], uncoupled["mean_abs_T_error_K"]), file=sys.stderr, flush=True)
+    # the learned initial model (the coupled workload's definition): a network
+    # trained on PSF-degraded pairs of other seeded cubes proposes the model
+    # on the final node basis, which the coupled inversion then refines (its
+    # last cycle only, as for the uncoupled learned pipeline)
+    from firtez_dz_amd import learned
+
+    def blur(rows):
+        return ctx.psf_convolve_torch(rows.t().contiguous(), nx, ny).t()
+
+    ctx.mlp_create(nmap, 512, 3, 4, seed=1)
+    X, Tg = learned.training_set(ctx, args.train_cubes_coupled, nx, ny, nz, seed0=2000, noise=1e-3, transform=blur)
+    rep = learned.train(ctx, X, Tg, epochs=args.epochs, batch=1024, lr=1e-3)
+    del X, Tg
+    print("[bench-coupled] network trained on %d PSF-degraded pairs in %.1f s"
+          % (args.train_cubes_coupled * ncol, rep.seconds), file=sys.stderr, flush=True)
+    ctl_l = gpu.CInvertControl.from_buffer_copy(ctl)
+    ctl_l.first_cycle = 1 << 20
+    del model
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    model = init.clone()
+    ctx.learned_init_torch(obs, model)
+    bad = ctx.hydrostatic_torch(model) != 0
+    if bool(bad.any()):
+        model[:, :, bad] = init[:, :, bad]
+        ctx.hydrostatic_torch(model)
+    _, chi_l, its_l = ctx.coupled_invert3d_torch(model, obs, ctl_l, nx, ny)
+    torch.cuda.synchronize()
+    dl = time.perf_counter() - t0
+    lrn = dict(lm_stats(model, truth, chi_l.cpu().numpy()), lm_iterations=its_l, seconds=dl,
+               pixels_per_s=ncol / dl, training_seconds=rep.seconds)
+    print("[bench-coupled] learned + coupled: %.1f s, %d iterations, median chi2 %.3g, mean |T - truth| %.1f K"
+          % (dl, its_l, lrn["median_chi2"], lrn["mean_abs_T_error_K"]), file=sys.stderr, flush=True)
     taps = 31 * 31
     conv_flop = 2.0 * taps * nmap * ncol
     conv_bytes = 2.0 * 4 * nmap * ncol
@@ -707,7 +742,7 @@ def bench_coupled(args):
             "vs_baseline": None, "dtype": "f64 LM step, f32 radiative transfer and PSF convolution",
             "data": "synthetic (seeded truth, PSF-degraded, + 1e-3 noise)",
             "config": {"workload": workload, "coupled": coupled, "uncoupled_same_observations": uncoupled,
-                       "ms_per_lm_iteration": dt * 1e3 / max(its_c, 1)},
+                       "learned_then_coupled": lrn, "ms_per_lm_iteration": dt * 1e3 / max(its_c, 1)},
             "roofline": {"kernel": "psf_conv_kernel (%d maps of %dx%d, %d taps)" % (nmap, nx, ny, taps),
                          "bound": "fp32 valu", "achieved": conv_flop / (conv_ms * 1e-3) / 1e12, "peak": 157.3,
                          "unit": "TFLOP/s", "frac": conv_flop / (conv_ms * 1e-3) / 1e12 / 157.3,

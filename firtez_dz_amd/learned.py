@@ -29,10 +29,13 @@ from . import gpu as _gpu
 
 
 def training_set(ctx, ncubes: int, nx: int, ny: int, nz: int, dz_km: float = 15.0, seed0: int = 1000,
-                 noise: float = 1e-3, log=None):
+                 noise: float = 1e-3, log=None, transform=None):
     """Returns (X [n][4*numw] float32, T [n][5*nodes] float32) on the device,
     n = ncubes*nx*ny.  Each cube is atmos.seeded(seed0 + i) put in
-    hydrostatic equilibrium and synthesised by the device forward."""
+    hydrostatic equilibrium and synthesised by the device forward;
+    transform (optional) maps a cube's Stokes rows [ncol][4*numw] to the
+    observed ones before the noise (e.g. the spatial PSF of a coupled
+    inversion)."""
     import torch
     ncol = nx * ny
     nin = 4 * ctx.numw
@@ -47,7 +50,10 @@ def training_set(ctx, ncubes: int, nx: int, ny: int, nz: int, dz_km: float = 15.
         stokes, _, status = ctx.forward_torch(a, _gpu.RF_NONE)
         ok = (st == 0) & (status == 0)
         rows = slice(i * ncol, (i + 1) * ncol)
-        X[rows] = stokes.reshape(ncol, nin) + noise * torch.randn((ncol, nin), generator=gen, device="cuda")
+        obs = stokes.reshape(ncol, nin)
+        if transform is not None:
+            obs = transform(obs)
+        X[rows] = obs + noise * torch.randn((ncol, nin), generator=gen, device="cuda")
         T[rows] = ctx.mlp_targets_torch(a)
         if not bool(ok.all()):
             # a column the EOS rejected is replaced by the first good one
