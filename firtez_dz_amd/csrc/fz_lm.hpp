@@ -169,17 +169,29 @@ __global__ void __launch_bounds__(256) eqv_kernel(EqvArgs a) {
       const double* tc = a.tab_c + (long)ip * a.nz * 2;
       int jl = jb;                                   // lowest unfinished node
       double alo = 0.0, ahi = 0.0;                   // sums of nodes jl, jl+1
-      for (int k = a.kbeg[jb]; k <= a.kend[je]; ++k) {
-        const int w = tj[k];
-        while (jl < (w & 0xffff)) {
-          out[(long)jl * fstride] = (float)alo;
-          alo = ahi;
-          ahi = 0.0;
-          ++jl;
+      const int k0 = a.kbeg[jb], k1 = a.kend[je];
+      // the loads of eight heights are issued together (one strided read per
+      // height and lane: eight in flight instead of one), the sums then take
+      // them in height order as before
+      for (int kb = k0; kb <= k1; kb += 8) {
+        float vb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) vb[i] = (kb + i <= k1) ? jp[(long)(kb + i) * cs] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int k = kb + i;
+          if (k > k1) break;
+          const int w = tj[k];
+          while (jl < (w & 0xffff)) {
+            out[(long)jl * fstride] = (float)alo;
+            alo = ahi;
+            ahi = 0.0;
+            ++jl;
+          }
+          const double v = (double)vb[i];
+          if (w & (1 << 30)) alo += v * tc[2 * k];
+          if (w & (1 << 29)) ahi += v * tc[2 * k + 1];
         }
-        const double v = (double)jp[k * cs];
-        if (w & (1 << 30)) alo += v * tc[2 * k];
-        if (w & (1 << 29)) ahi += v * tc[2 * k + 1];
       }
       out[(long)jl * fstride] = (float)alo;
       if (jl < je) out[(long)(jl + 1) * fstride] = (float)ahi;
