@@ -50,7 +50,12 @@ struct GemmArgs {
 // Tile loader: rows r0..r0+63 of an operand with element (r, k) at
 // P[r * sr + k * sk] into LDS [64][MLP_LDK], zero outside rows < R, k < K.
 // The thread mapping follows the operand's contiguous direction so that the
-// global reads of a wave are consecutive.
+// global reads of a wave are consecutive; where eight consecutive elements
+// along that direction are in range and 16-byte aligned, a thread moves
+// them with one 16-byte load (and, along rows, scatters them down a column
+// of the tile).
+__device__ __forceinline__ bool mlp_al16(const bf16* p) { return ((uintptr_t)p & 15) == 0; }
+
 __device__ __forceinline__ void mlp_load_tile(const bf16* __restrict__ P, long sr, long sk, int R, int K, int r0, int k0,
                                               bf16* __restrict__ S) {
   const int t = threadIdx.x;
@@ -58,14 +63,35 @@ __device__ __forceinline__ void mlp_load_tile(const bf16* __restrict__ P, long s
   if (sk == 1) {
     // 4 threads per row, 8 consecutive k each
     const int r = t >> 2, kk = (t & 3) * 8;
-    const int gr = r0 + r;
+    const int gr = r0 + r, gk0 = k0 + kk;
+    const bf16* src = P + (long)gr * sr + gk0;
+    if (gr < R && gk0 + 8 <= K && mlp_al16(src)) {
+      *reinterpret_cast<uint4*>(&S[r * MLP_LDK + kk]) = *reinterpret_cast<const uint4*>(src);
+      return;
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int gk = k0 + kk + i;
+      const int gk = gk0 + i;
       S[r * MLP_LDK + kk + i] = (gr < R && gk < K) ? P[(long)gr * sr + gk] : z;
     }
+  } else if (sr == 1) {
+    // rows contiguous: 8 threads along the 64 rows (8 rows each), 32 k
+    const int rr = (t & 7) * 8, kk = t >> 3;
+    const int gr0 = r0 + rr, gk = k0 + kk;
+    const bf16* src = P + gr0 + (long)gk * sk;
+    if (gk < K && gr0 + 8 <= R && mlp_al16(src)) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(src);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) S[(rr + i) * MLP_LDK + kk] = v[i];
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int gr = gr0 + i;
+      S[(rr + i) * MLP_LDK + kk] = (gr < R && gk < K) ? P[gr + (long)gk * sk] : z;
+    }
   } else {
-    // rows contiguous (or neither): 64 threads along rows, 4 k-groups of 8
+    // neither direction contiguous: 64 threads along rows, 4 k-groups of 8
     const int r = t & 63, kk = (t >> 6) * 8;
     const int gr = r0 + r;
 #pragma unroll
