@@ -273,6 +273,8 @@ def main():
     ap.add_argument("--uneven", action="store_true",
                     help="with --lm: per-pixel line sets and windows (8-line list, 1-4 lines, 60-300 wavelengths)")
     ap.add_argument("--micro", type=float, default=0.5, help="microturbulence (km/s) of the forward workload")
+    ap.add_argument("--overlap", action="store_true",
+                    help="microphysics of the next tile on a second stream during the current tile's transfer")
     ap.add_argument("--train-cubes-coupled", type=int, default=4,
                     help="with --coupled: seeded 256x256 cubes of PSF-degraded training pairs")
     ap.add_argument("--coupled", action="store_true",
@@ -302,6 +304,7 @@ def main():
     gain = ctx.gain_hsra()
     xi = args.micro * 1.0e5
     ctx.set_microturbulence(xi)
+    ctx.set_overlap(args.overlap)
     ctx.set_tile(args.tile, 256)
     d_atm = torch.from_numpy(atm).cuda()
     st = ctx.hydrostatic_torch(d_atm)

@@ -91,6 +91,7 @@ def library():
     L.fz_hydrostatic_dev.argtypes = [vp, ip, vp, vp, vp]
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
+    L.fz_set_overlap.argtypes = [vp, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_microturbulence.argtypes = [vp, ctypes.c_double]
     L.fz_set_lsf.argtypes = [vp, vp, vp]
@@ -142,7 +143,7 @@ def exported_symbols():
     """Names declared in include/firtez_gpu.h."""
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
-            "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_last_timing",
+            "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_set_overlap", "fz_last_timing",
             "fz_set_hydrostatic", "fz_set_microturbulence", "fz_set_lsf", "fz_set_departure", "fz_set_psf", "fz_psf_convolve_dev", "fz_coupled_invert3d_dev", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d", "fz_coupled_invert3d",
@@ -263,6 +264,11 @@ class Context:
         """Microturbulence xi (cm/s) in every line's Doppler width
         sqrt(2kT/m + xi^2); 0 is the reference's thermal width."""
         self._check(self.lib.fz_set_microturbulence(self.h, float(xi_cm_s)), "fz_set_microturbulence")
+
+    def set_overlap(self, on: bool = True):
+        """Microphysics of the next radiative-transfer tile on a second stream
+        while the current tile's transfer runs (the library's default is off)."""
+        self._check(self.lib.fz_set_overlap(self.h, int(on)), "fz_set_overlap")
 
     def set_hydrostatic(self, on: bool = True):
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside every

@@ -182,6 +182,17 @@ int fz_synchronize(fz_ctx* ctx);
  * launches of each fz_forward_dev call; fz_last_timing waits for them and
  * returns the elapsed milliseconds of the last call. */
 int fz_set_profiling(fz_ctx* ctx, int on);
+
+/* Overlap (default off): with more than one radiative-transfer tile,
+ * fz_forward_dev runs the microphysics of the next tile on a second stream
+ * of the library while the current tile's radiative transfer runs on the
+ * caller's stream; the call stays asynchronous and ordered on the caller's
+ * stream.  With profiling on, the microphysics time is then the first tile's
+ * only.  On the 288x288x96 forward with response functions the concurrent
+ * kernels slow the radiative transfer more than they hide (112.5 ms per
+ * step against 108).  No counterpart in the reference (its columns are
+ * independent MPI work items). */
+int fz_set_overlap(fz_ctx* ctx, int on);
 int fz_last_timing(fz_ctx* ctx, float* micro_ms, float* rt_ms, int* rt_launches);
 
 /* HYDROSTATIC: yes / no (input_file.f90:516-520).  With it every forward
