@@ -3272,6 +3272,8 @@ int fz_coupled_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const doubl
                             void* stream) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
   if (c->win_on) return set_err(c, FZ_ECONFIG, "the coupled inversion takes the dense spectral grid");
+  // the map transposes put 32 pixels per workgroup row on the grid's y axis
+  if ((long)nx * ny > 32L * 65535) return set_err(c, FZ_EARG, "more than 2097120 pixels in one coupled map");
   const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(c, hipSetDevice(c->device));
