@@ -22,8 +22,10 @@ RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
-CONTINUUM NORMALIZATION, LINE OPTICAL DEPTH, AUTO Stokes weights,
-FULL_STOKES) raise NotImplementedError instead of being ignored.  COUPLED
+LINE OPTICAL DEPTH, AUTO Stokes weights, FULL_STOKES) raise
+NotImplementedError instead of being ignored.  CONTINUUM NORMALIZATION
+(T: the HSRA continuum, the default; F: no normalisation, GAIN = 1 with the
+V sign of GET_NORM) chooses GAIN1D.  COUPLED
 INVERSION (yes and FILENAME, the PSF file in READ_PSF_2D's format under
 DATAPATH) runs MODE inversion through that spatial PSF.  MISC SETUP TEMP / PGAS / BVEC / VLOS switch on SET_TAU's assist
 initialisers.  LINE SPREAD FUNCTION
@@ -46,7 +48,7 @@ from . import modelio
 PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
-_UNSUPPORTED = ("BOX:", "CONTINUUM NORMALIZATION:", "LINE OPTICAL DEPTH:")
+_UNSUPPORTED = ("BOX:", "LINE OPTICAL DEPTH:")
 
 
 @dataclass
@@ -84,6 +86,8 @@ class Control:
     # COUPLED INVERSION: yes/no and the PSF file (DATAPATH-relative)
     coupled: bool = False
     psf_file: Optional[str] = None
+    # CONTINUUM NORMALIZATION: T (HSRA, the default) or F (GAIN = 1)
+    hsra_norm: bool = True
 
 
 def _number(tok: str) -> float:
@@ -235,6 +239,14 @@ def parse(text: str) -> Control:
                     c.lsf.append(None if cols[0].upper() == "EMPTY" else cols[0])
                 else:
                     raise ValueError('wrong input format in "LINE SPREAD FUNCTION": %r' % row)
+        elif key == "CONTINUUM NORMALIZATION:":
+            # NSET_CONTINUUM_NORMALIZATION (input_file.f90:1480-1520): the
+            # first character of the section's last row, T or F
+            body, i = section(i)
+            first = body[-1].strip()[:1] if body else ""
+            if first not in ("T", "F"):
+                raise ValueError('"CONTINUUM NORMALIZATION:" must be either TRUE or FALSE, not %r' % first)
+            c.hsra_norm = first == "T"
         elif key == "COUPLED INVERSION:":
             # NSET_COUPLED_INVERSION (input_file.f90:1524-1607): a one-column
             # row y/n, "FILENAME <psf file>"; BLCKSZ / NMTHRD / PSFRAD set up
@@ -398,7 +410,14 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
             cube.setdefault(name, np.zeros_like(cube["T"], order="F"))
         ls = line_setup(c)
         ctx.configure(ls, z)
-        ctx.gain_hsra()
+        if c.hsra_norm:
+            ctx.gain_hsra()
+        else:
+            # GET_NORM without HSRA (normalization.f90:746-751): GAIN = 1,
+            # V with the opposite sign
+            g = np.ones((4, ctx.numw), np.float32)
+            g[3] = -1.0
+            ctx.set_gain(g)
         if c.lsf:
             ctx.set_lsf(*lsf_arrays(c, ls, workdir))
         if any(f.upper() != "LTE" for _, f in c.nlte):

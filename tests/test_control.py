@@ -125,6 +125,13 @@ def test_reader_errors():
         control.parse("FILEMODEL:\nm.bin\nMODE:\nsynthesis\nFILEPROFILE:\np\n")   # no lines
     with pytest.raises(NotImplementedError):
         control.parse("BOX:\n1 1\n1 1\n1 1\nFILEMODEL:\nm\nMODE:\ntau\n")
+    # CONTINUUM NORMALIZATION (input_file.f90:1480-1520): T or F on the
+    # section's last row, anything else is an error
+    assert control.parse("FILEMODEL:\nm\nMODE:\ntau\n").hsra_norm
+    assert not control.parse("CONTINUUM NORMALIZATION:\nF\nFILEMODEL:\nm\nMODE:\ntau\n").hsra_norm
+    assert control.parse("CONTINUUM NORMALIZATION:\nF\nTRUE\nFILEMODEL:\nm\nMODE:\ntau\n").hsra_norm
+    with pytest.raises(ValueError):
+        control.parse("CONTINUUM NORMALIZATION:\nyes\nFILEMODEL:\nm\nMODE:\ntau\n")
     with pytest.raises(NotImplementedError):
         control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nFULL_STOKES\n")
 

@@ -130,6 +130,26 @@ def test_inversion_after_synthesis(gpu_ctx, tree):
     assert np.all(np.isfinite(m["T"])) and np.all(m["T"] >= 2500.0)
 
 
+def test_syn1_without_continuum_normalisation(gpu_ctx, tree):
+    """test_syn1 with CONTINUUM NORMALIZATION: F (GET_NORM without HSRA,
+    normalization.f90:746-751: GAIN = 1, V with the opposite sign): the
+    written profiles are the normalised ones divided by the HSRA gain."""
+    _, _, wd = run(tree, "test_syn1", gpu_ctx)
+    st_t, _, _ = modelio.read_profiles(os.path.join(wd, "syn_profile.bin"))
+    gain = gpu_ctx.gain_hsra()                                  # (4, numw) of the same lines
+
+    def no_norm(c):
+        c.hsra_norm = False
+        c.fileprofile = "raw_profile.bin"
+
+    run(tree, "test_syn1", gpu_ctx, no_norm)
+    st_f, _, _ = modelio.read_profiles(os.path.join(wd, "raw_profile.bin"))
+    sign = np.array([1.0, 1.0, 1.0, -1.0])[:, None, None, None]
+    ref = st_t / (gain[:, :, None, None] * sign)
+    err = float(np.max(np.abs(st_f - ref)) / np.max(np.abs(ref)))
+    report("test_syn1, CONTINUUM NORMALIZATION: F", err, 1e-6)
+
+
 def test_syn1_with_line_spread_function(gpu_ctx, golden, tree):
     """test_syn1 with a LINE SPREAD FUNCTION: section (a Gaussian of sigma
     30 mA centred on 0, NSET_LSF / LSF_INIT): the written profiles equal the
