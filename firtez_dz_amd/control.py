@@ -23,7 +23,10 @@ SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
 LINE OPTICAL DEPTH, AUTO Stokes weights, FULL_STOKES) raise
-NotImplementedError instead of being ignored.  CONTINUUM NORMALIZATION
+NotImplementedError instead of being ignored.  BOX (three rows "N D" for x,
+y, z) declares the grid instead of the model header (BOX_AVAILABLE,
+input_file.f90:711-731); the runner requires the model file to hold that
+grid.  CONTINUUM NORMALIZATION
 (T: the HSRA continuum, the default; F: no normalisation, GAIN = 1 with the
 V sign of GET_NORM) chooses GAIN1D.  COUPLED
 INVERSION (yes and FILENAME, the PSF file in READ_PSF_2D's format under
@@ -48,7 +51,7 @@ from . import modelio
 PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
-_UNSUPPORTED = ("BOX:", "LINE OPTICAL DEPTH:")
+_UNSUPPORTED = ("LINE OPTICAL DEPTH:",)
 
 
 @dataclass
@@ -88,6 +91,8 @@ class Control:
     psf_file: Optional[str] = None
     # CONTINUUM NORMALIZATION: T (HSRA, the default) or F (GAIN = 1)
     hsra_norm: bool = True
+    # BOX: ((NX, DX), (NY, DY), (NZ, DZ)) or None (grid from the model header)
+    box: Optional[Tuple[Tuple[int, float], ...]] = None
 
 
 def _number(tok: str) -> float:
@@ -239,6 +244,17 @@ def parse(text: str) -> Control:
                     c.lsf.append(None if cols[0].upper() == "EMPTY" else cols[0])
                 else:
                     raise ValueError('wrong input format in "LINE SPREAD FUNCTION": %r' % row)
+        elif key == "BOX:":
+            # NREAD_BOX (input_file.f90:219-250): the next three rows, "N D"
+            # for x, y and z (SPLIT_READ_BOX_LINE, :771-829)
+            rows = []
+            for _ in range(3):
+                i += 1
+                cols = ls[i].split() if i < len(ls) else []
+                if len(cols) < 2:
+                    raise ValueError('"BOX:" needs three rows of "N D"')
+                rows.append((int(_number(cols[0])), float(_number(cols[1]))))
+            c.box = tuple(rows)
         elif key == "CONTINUUM NORMALIZATION:":
             # NSET_CONTINUUM_NORMALIZATION (input_file.f90:1480-1520): the
             # first character of the section's last row, T or F
@@ -406,6 +422,9 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
         cube = modelio.read_model(path(c.modpath, c.filemodel))
         z = cube["z_km"]
         nz, ny, nx = cube["T"].shape
+        if c.box is not None and (c.box[0][0], c.box[1][0], c.box[2][0]) != (nx, ny, nz):
+            raise ValueError("BOX declares %dx%dx%d, the model file holds %dx%dx%d"
+                             % (c.box[0][0], c.box[1][0], c.box[2][0], nx, ny, nz))
         for name in ("vx", "vy"):
             cube.setdefault(name, np.zeros_like(cube["T"], order="F"))
         ls = line_setup(c)

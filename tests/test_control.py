@@ -123,8 +123,13 @@ def test_reader_errors():
         control.parse("FILEMODEL:\nm.bin\nMODE:\ninversion\nLINES:\n4 10 0 1\nFILEPROFILE:\np\n")  # no Stokes
     with pytest.raises(ValueError):
         control.parse("FILEMODEL:\nm.bin\nMODE:\nsynthesis\nFILEPROFILE:\np\n")   # no lines
+    # BOX (NREAD_BOX, input_file.f90:219-250): three rows "N D"
+    c = control.parse("BOX:\n4 16.0\n3 16.0\n64 15.0\nFILEMODEL:\nm\nMODE:\ntau\n")
+    assert c.box == ((4, 16.0), (3, 16.0), (64, 15.0))
+    with pytest.raises(ValueError):
+        control.parse("BOX:\n4 16.0\n3\nFILEMODEL:\nm\nMODE:\ntau\n")
     with pytest.raises(NotImplementedError):
-        control.parse("BOX:\n1 1\n1 1\n1 1\nFILEMODEL:\nm\nMODE:\ntau\n")
+        control.parse("LINE OPTICAL DEPTH:\ny\nFILEMODEL:\nm\nMODE:\ntau\n")
     # CONTINUUM NORMALIZATION (input_file.f90:1480-1520): T or F on the
     # section's last row, anything else is an error
     assert control.parse("FILEMODEL:\nm\nMODE:\ntau\n").hsra_norm

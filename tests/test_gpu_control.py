@@ -188,7 +188,8 @@ def test_coupled_inversion_control_file(gpu_ctx, tmp_path):
             "STOKES SETUP:\nSTKI 1\nSTKQ 1\nSTKU 1\nSTKV 1\n"
             "INVERSION SETUP:\nTEM 1\nBX 1\nBY 1\nBZ 1\nVLOS 1\n"
             "MISC SETUP:\nDATAPATH ./\nMODPATH ./\nOUTPATH out/\nNOISE 1e-3\n"
-            "COUPLED INVERSION:\nyes\nFILENAME psf.bin\n")
+            "COUPLED INVERSION:\nyes\nFILENAME psf.bin\n"
+            "BOX:\n4 16.0\n3 16.0\n48 15.0\n")
     (tmp_path / "control_file.dat").write_text(text)
     (tmp_path / "out").mkdir()
     c = control.parse(text)
