@@ -11,7 +11,9 @@ reference's output files (DEALLOCATE_AND_WRITE, pre_post_duties.f90:108-355):
   synthesis    OUTPATH/<FILEPROFILE>, OUTPATH/out_<FILEMODEL>, and with
                WRITE RESPONSE FUNCTION: yes OUTPATH/rfout_<FILEMODEL>
   inversion    OUTPATH/out_<FILEPROFILE> (inverted Stokes parameters only),
-               OUTPATH/out_<FILEMODEL>; observations from DATAPATH/<FILEPROFILE>
+               OUTPATH/out_<FILEMODEL>; observations from DATAPATH/<FILEPROFILE>;
+               with COUPLED INVERSION also OUTPATH/cout_<FILEPROFILE>, the
+               synthesis convolved with the PSF
   tau          OUTPATH/out_<FILEMODEL> (log tau5 from the forward's EOS)
   hydrostatic  OUTPATH/out_<FILEMODEL> (Pg, rho, Pe, mu from RK4 from the top)
 
@@ -494,18 +496,25 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
                     raise FileNotFoundError("cannot find PSF file %s in directory %s" % (c.psf_file, c.datapath))
                 ctx.set_psf(modelio.read_psf(ppath))
                 try:
-                    syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz), coupled=True)
+                    syn_c, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz), coupled=True)
                 finally:
                     ctx.set_psf(None)         # END_COUPLED_INVERSION_*
+                # DEALLOCATE_AND_WRITE (pre_post_duties.f90:195-245) writes the
+                # inverted model's own synthesis to out_<FILEPROFILE> and its
+                # PSF convolution to cout_<FILEPROFILE>
+                res = ctx.forward3d(cube)
+                syn3d = res[0] if isinstance(res, tuple) else res
             else:
                 syn3d, chi2, its = ctx.invert3d(cube, obs3d, control_for_inversion(c, nz))
-            st = _stokes4(syn3d, numw, ny, nx)
-            for s in range(4):
-                if not c.invstk[s]:
-                    st[s] = 0.0
-            p = path(c.outpath, "out_" + c.fileprofile)
-            modelio.write_profiles(p, st, wave, index)
-            out["written"].append(p)
+            outs = [("out_", syn3d)] + ([("cout_", syn_c)] if c.coupled else [])
+            for prefix, sy in outs:
+                st = _stokes4(sy, numw, ny, nx)
+                for s in range(4):
+                    if not c.invstk[s]:
+                        st[s] = 0.0
+                p = path(c.outpath, prefix + c.fileprofile)
+                modelio.write_profiles(p, st, wave, index)
+                out["written"].append(p)
             out["chi2"] = chi2
             out["iterations"] = its
         p = path(c.outpath, "out_" + c.filemodel)

@@ -244,8 +244,14 @@ def test_coupled_inversion_control_file(gpu_ctx, tmp_path):
     m = modelio.read_model(str(tmp_path / "out" / "out_start.bin"))
     for n in ("T", "Bx", "By", "Bz", "vz"):
         assert np.array_equal(m[n], cube2[n]), n
-    w, _, _ = modelio.read_profiles(str(tmp_path / "out" / "out_obs.bin"))
+    # cout_: the PSF-convolved synthesis of the inverted model; out_: its own
+    # synthesis (DEALLOCATE_AND_WRITE, pre_post_duties.f90:195-245)
+    w, _, _ = modelio.read_profiles(str(tmp_path / "out" / "cout_obs.bin"))
     assert np.array_equal(w, _st4(syn_d, numw).astype(np.float32))
+    res = gpu_ctx.forward3d(cube2)
+    syn_u = res[0] if isinstance(res, tuple) else res
+    w, _, _ = modelio.read_profiles(str(tmp_path / "out" / "out_obs.bin"))
+    assert np.array_equal(w, _st4(syn_u, numw).astype(np.float32))
     assert float(chi.mean()) < 2.0
 
 
