@@ -24,8 +24,10 @@ RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
-LINE OPTICAL DEPTH, AUTO Stokes weights, FULL_STOKES) raise
-NotImplementedError instead of being ignored.  BOX (three rows "N D" for x,
+LINE OPTICAL DEPTH, AUTO Stokes weights) raise NotImplementedError instead
+of being ignored.  MISC SETUP FULL_STOKES writes, in synthesis mode,
+OUTPATH/fs_<FILEPROFILE> (WRITE_BIN) with the Stokes vector after every
+height, (4, NUMW, NZ, NY, NX).  BOX (three rows "N D" for x,
 y, z) declares the grid instead of the model header (BOX_AVAILABLE,
 input_file.f90:711-731); the runner requires the model file to hold that
 grid.  CONTINUUM NORMALIZATION
@@ -93,6 +95,8 @@ class Control:
     psf_file: Optional[str] = None
     # CONTINUUM NORMALIZATION: T (HSRA, the default) or F (GAIN = 1)
     hsra_norm: bool = True
+    # MISC SETUP FULL_STOKES: the Stokes vector after every height
+    full_stokes: bool = False
     # BOX: ((NX, DX), (NY, DY), (NZ, DZ)) or None (grid from the model header)
     box: Optional[Tuple[Tuple[int, float], ...]] = None
 
@@ -293,7 +297,7 @@ def parse(text: str) -> Control:
                     if _field(row, name, 0)[0]:
                         c.assist |= bit
                 if _field(row, "FULL_STOKES", 0)[0]:
-                    raise NotImplementedError("MISC SETUP FULL_STOKES is not supported by this path")
+                    c.full_stokes = True
                 ok, v = _field(row, "SVDTOL", 0)
                 if ok:
                     c.svdtol = float(v)
@@ -466,6 +470,12 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
                 p = path(c.outpath, c.fileprofile)
                 modelio.write_profiles(p, st, wave, index)
                 out["written"].append(p)
+                if c.full_stokes:
+                    # DEALLOCATE_AND_WRITE (pre_post_duties.f90:265-271):
+                    # fs_<FILEPROFILE>, SYN5D(4, NUMW, NZ, NY, NX)
+                    p = path(c.outpath, "fs_" + c.fileprofile)
+                    modelio.write_bin(p, _full_stokes(ctx, cube))
+                    out["written"].append(p)
             if want_rf:
                 npar = _gpu.NPARAM[rf_mode]
                 # DSYN(NFREQ, IFREEP, NY, NX): NFREQ = Stokes-major blocks of
@@ -528,6 +538,19 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
             ctx.close()
         else:
             ctx.set_hydrostatic(hydro_before)
+
+
+def _full_stokes(ctx, cube):
+    """SYN5D(4, NUMW, NZ, NY, NX) of the cube (FULL_STOKES, forward.f90:980-983)
+    from a device forward; the cube itself is left as it is."""
+    import torch
+    nz, ny, nx = cube["T"].shape
+    atm = torch.from_numpy(modelio.to_device_rows(cube)).cuda()
+    _, _, status, full = ctx.forward_full_torch(atm)
+    if int(status.max()) != 0:
+        raise _gpu.FzError("full Stokes synthesis failed in %d columns" % int((status != 0).sum()))
+    f = full.cpu().numpy().reshape(nx, ny, nz, 4, ctx.numw)        # [x][y][z][s][l], column = y + ny x
+    return np.asfortranarray(f.transpose(3, 4, 2, 1, 0))
 
 
 def _stokes4(syn3d, numw, ny, nx):

@@ -532,6 +532,7 @@ struct RtArgs {
   const RtItem* items;   // this launch's workgroups
   const int64_t* poff;   // [ncol+1] prefix of the pixels' wavelength counts: Stokes at 4*poff[col]
   const double* pop;     // NLTE departure coefficients [2][nz][nreg], or null (LTE)
+  float* fs;             // FULL_STOKES [ncol][nz][4][numw] (Stokes after every height), or null
 };
 
 enum { D_T = 0, D_P, D_R, D_V, D_B, D_G, D_F, NDIR };
@@ -727,7 +728,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
                                                      float* __restrict__ scratch, int ncol, int col0, int ntile,
                                                      int jcol0, const RtItem* __restrict__ items,
                                                      const int64_t* __restrict__ poff,
-                                                     const double* __restrict__ pop) {
+                                                     const double* __restrict__ pop, float* __restrict__ fs) {
   RtArgs a;
   a.ncol = ncol;
   a.col0 = col0;
@@ -743,6 +744,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   a.items = items;
   a.poff = poff;
   a.pop = pop;
+  a.fs = fs;
   // parameters: 5 spherical, 7 Cartesian, 8 with P0 (the top pressure)
   constexpr int NP = (RF == FZ_RF_SPHERICAL || RF == RF_CART_TBV)
                          ? 5
@@ -816,6 +818,14 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
   auto depart = [&](int k) -> float {
     return a.pop ? (float)(a.pop[(long)k * c_cfg.nreg + r] / a.pop[((long)nz + k) * c_cfg.nreg + r]) : 1.0f;
   };
+  // FULL_STOKES (forward.f90:980-983): the Stokes vector after every
+  // height, times GAIN1D, at fs[((col * nz + k) * 4 + s) * numw + l]
+  float* fsp = a.fs ? a.fs + (long)(col0 + ct) * nz * 4 * snumw + lo : nullptr;
+  auto store_fs = [&](int k) {
+    if (!fsp) return;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) fsp[(long)(k * 4 + s) * snumw] = I[s] * a.gain[l * 4 + s];
+  };
   for (int k = 0; k < nz; ++k) {
     const Layer ly = load_layer(a, col, k);
     const float S = c1 * (1.0f / (depart(k) * expf(c2 * ly.iT) - 1.0f));
@@ -825,6 +835,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
     }
     if (a.micro[F_FLAG * ps + k * cs + col] != 0.f) {
       I[0] = S; I[1] = 0.f; I[2] = 0.f; I[3] = 0.f;
+      store_fs(k);
       continue;
     }
     float x7[7];
@@ -859,6 +870,7 @@ __global__ void __launch_bounds__(RT_WAVE) rt_kernel(const float* __restrict__ a
               E.O[i][3] * I[3];
 #pragma unroll
     for (int i = 0; i < 4; ++i) I[i] = In[i];
+    store_fs(k);
   }
   float gn[4];
 #pragma unroll
@@ -1860,7 +1872,7 @@ static int launch_rt(fz_ctx* c, int rf_mode, const RtArgs& a, hipStream_t st, in
       blk(RT_WAVE);
 #define FZ_RT_ARGS                                                                                               \
   a.atm, a.micro, a.gain, a.status, a.stokes, a.jac, a.scratch, a.ncol, a.col0, a.ntile, a.jcol0, a.items, a.poff, \
-      c->pop_on ? c->d_pop : nullptr
+      c->pop_on ? c->d_pop : nullptr, a.fs
   if (pack) {
     if (rf_mode == FZ_RF_NONE) hipLaunchKernelGGL((rt_kernel<FZ_RF_NONE, false, true>), grid, blk, 0, st, FZ_RT_ARGS);
     else if (rf_mode == FZ_RF_SPHERICAL)
@@ -2007,9 +2019,14 @@ int fz_set_tile(fz_ctx* c, int tile, int block) {
   return FZ_OK;
 }
 
-int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac, int32_t* d_status,
-                   void* stream) {
+}  // extern "C"
+
+// FORWARD3D on device data; d_full (or null): the Stokes vector after every
+// height, [ncol][nz][4][numw] (FULL_STOKES)
+static int forward_impl(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
+                        float* d_full, int32_t* d_status, void* stream) {
   if (!c || !c->configured || ncol < 1 || !d_atm || !d_stokes) return FZ_EARG;
+  if (d_full && c->win_on) return set_err(c, FZ_EARG, "per-pixel windows: no full Stokes output");
   if (rf_mode != FZ_RF_NONE && !d_jac) return set_err(c, FZ_EARG, "jacobian output missing");
   if (rf_mode < 0 || rf_mode > 3) return FZ_EARG;
   if (c->win_on) {
@@ -2081,6 +2098,7 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
     a.scratch = c->d_scratch;
     a.items = nullptr;
     a.poff = nullptr;
+    a.fs = d_full;
     int nit = 0;
     if (c->win_on) {
       a.items = c->d_items + c->h_item_first[c0];
@@ -2100,6 +2118,19 @@ int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stok
   }
   if (c->prof) HIPCHK(c, hipEventRecord(c->ev[2], st));
   return FZ_OK;
+}
+
+extern "C" {
+
+int fz_forward_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac, int32_t* d_status,
+                   void* stream) {
+  return forward_impl(c, ncol, d_atm, rf_mode, d_stokes, d_jac, nullptr, d_status, stream);
+}
+
+int fz_forward_full_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac, float* d_full,
+                        int32_t* d_status, void* stream) {
+  if (!d_full) return FZ_EARG;
+  return forward_impl(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_full, d_status, stream);
 }
 
 int fz_set_overlap(fz_ctx* c, int on) {
@@ -2570,6 +2601,7 @@ int fz_forward_nodes_dev(fz_ctx* c, int ncol, float* d_atm, float* d_stokes, flo
     a.scratch = c->d_scratch;
     a.items = nullptr;
     a.poff = nullptr;
+    a.fs = nullptr;
     int nit = 0;
     if (c->win_on) {
       a.items = c->d_items + c->h_item_first[c0];

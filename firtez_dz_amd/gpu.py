@@ -91,6 +91,7 @@ def library():
     L.fz_hydrostatic_dev.argtypes = [vp, ip, vp, vp, vp]
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
+    L.fz_forward_full_dev.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, vp]
     L.fz_set_overlap.argtypes = [vp, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_microturbulence.argtypes = [vp, ctypes.c_double]
@@ -142,7 +143,7 @@ def library():
 def exported_symbols():
     """Names declared in include/firtez_gpu.h."""
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
-            "fz_forward_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
+            "fz_forward_dev", "fz_forward_full_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_set_overlap", "fz_last_timing",
             "fz_set_hydrostatic", "fz_set_microturbulence", "fz_set_lsf", "fz_set_departure", "fz_set_psf", "fz_psf_convolve_dev", "fz_coupled_invert3d_dev", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
@@ -409,6 +410,30 @@ class Context:
         self.forward_dev(ncol, atm.data_ptr(), rf_mode, stokes.data_ptr(), jac.data_ptr() if npar else None,
                          status.data_ptr(), stream)
         return stokes, (jac if npar else None), status
+
+    def forward_full_torch(self, atm, rf_mode=RF_NONE, stokes=None, jac=None, status=None, full=None):
+        """forward_torch plus FULL_STOKES (fz_forward_full_dev): also returns
+        full [ncol, nz, 4, numw], the Stokes vector after every height (bottom
+        first) times GAIN1D."""
+        import torch
+        assert atm.is_cuda and atm.dtype == torch.float32 and atm.is_contiguous()
+        assert atm.shape[0] == 12 and atm.shape[1] == self.nz
+        ncol = atm.shape[2]
+        dev = atm.device
+        if stokes is None:
+            stokes = torch.empty(self._stokes_shape(ncol), dtype=torch.float32, device=dev)
+        npar = NPARAM[rf_mode]
+        if npar and jac is None:
+            jac = torch.empty((ncol, npar, self.nz, 4, self.numw), dtype=torch.float32, device=dev)
+        if status is None:
+            status = torch.zeros(ncol, dtype=torch.int32, device=dev)
+        if full is None:
+            full = torch.empty((ncol, self.nz, 4, self.numw), dtype=torch.float32, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        self._check(self.lib.fz_forward_full_dev(self.h, ncol, atm.data_ptr(), rf_mode, stokes.data_ptr(),
+                                                 jac.data_ptr() if npar else None, full.data_ptr(),
+                                                 status.data_ptr(), stream), "fz_forward_full_dev")
+        return stokes, (jac if npar else None), status, full
 
     def hydrostatic_torch(self, atm, status=None):
         import torch

@@ -24,6 +24,7 @@ PROFILE_MAGIC = 160904
 LSF_MAGIC = 3330904
 RF_MAGIC = 18060904
 PSF_MAGIC = 161906
+BIN_MAGIC = 160206
 VERSION = (3000, 3)      # SS, VV (format 3000, version 3) written as SS+VV, SS-VV (misc.f90:944-953)
 NMAXREC = 1073741784 // 4
 ORDER = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "Pe", "mu", "tau5", "x", "y", "z")
@@ -268,3 +269,27 @@ def read_psf(path):
     if data.size != npx * npy:
         raise ValueError("%s: %d values, header says %d x %d" % (path, data.size, npx, npy))
     return data.reshape(npy, npx).astype(np.float64)
+
+
+def write_bin(path, array):
+    """WRITE_BIN (misc.f90:2352-2418): the header record (SS+VV, SS-VV,
+    160206, records, RK, SH(1..RK)) with format 3000 version 3, then the
+    array's values in Fortran order; array holds the Fortran shape SH."""
+    a = np.asarray(array, np.float32)
+    ss, vv = VERSION
+    header = np.array([ss + vv, ss - vv, BIN_MAGIC, _nrecs(a.size) + 1, a.ndim] + list(a.shape), np.float32)
+    _write_records(path, header, [a.reshape(-1, order="F")])
+
+
+def read_bin(path):
+    """The array of a WRITE_BIN file, with its Fortran shape."""
+    recs = _records(path)
+    hdr = recs[0]
+    if abs(int(round(float(hdr[2]))) - BIN_MAGIC) > 0:
+        raise ValueError("%s is not a WRITE_BIN file" % path)
+    rk = int(hdr[4])
+    shape = tuple(int(v) for v in hdr[5:5 + rk])
+    data = np.concatenate(recs[1:]) if len(recs) > 1 else np.zeros(0, np.float32)
+    if data.size != int(np.prod(shape)):
+        raise ValueError("%s: %d values, header says %s" % (path, data.size, shape))
+    return data.reshape(shape, order="F")

@@ -154,6 +154,14 @@ int fz_window_offsets(const fz_ctx* ctx, int64_t* offsets);
 int fz_forward_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
                    int32_t* d_status, void* stream);
 
+/* fz_forward_dev with MISC SETUP FULL_STOKES (forward.f90:980-983, written
+ * to fs_<FILEPROFILE> by DEALLOCATE_AND_WRITE, pre_post_duties.f90:265-271):
+ * d_full [ncol][nz][4][numw] float receives the Stokes vector after every
+ * height, bottom (k = 0) to top, times GAIN1D; its top height is d_stokes
+ * before any LSF.  Not with per-pixel windows. */
+int fz_forward_full_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
+                        float* d_full, int32_t* d_status, void* stream);
+
 /* FORWARD3D on host arrays in the reference's layout (forward.f90:43-306):
  * cubes(NZ,NY,NX) column-major float (TEM3D..TAU3D5; PG3D, RHO3D, PEL3D,
  * MW3D and TAU3D5 are written back as SPLIT_MODEL3D does, forward.f90:1225-1270),

@@ -99,6 +99,7 @@ class Cpu:
         L.fzc_he.argtypes = [ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _dp, _dp]
         L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
         L.fzc_set_microturbulence.argtypes = [ctypes.c_double]
+        L.fzc_set_full_stokes.argtypes = [ctypes.c_void_p]
         L.fzc_set_lsf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_set_departure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
@@ -206,6 +207,17 @@ class Cpu:
     def set_microturbulence(self, xi_cm_s=0.0):
         """Doppler width sqrt(2kT/m + xi^2) for every line (0: thermal only)."""
         self.lib.fzc_set_microturbulence(float(xi_cm_s))
+
+    def forward1d_full(self, model):
+        """forward1d plus FULL_STOKES: (model, syn, dsyn, kc5, fs [nz][4][numw])."""
+        nz = model.shape[1]
+        fs = np.zeros((nz, 4, self.numw))
+        self.lib.fzc_set_full_stokes(fs.ctypes.data)
+        try:
+            out = self.forward1d(model)
+        finally:
+            self.lib.fzc_set_full_stokes(None)
+        return out + (fs,)
 
     def set_lsf(self, lsf_kernel=None, valid=None):
         """MLSF: LSF_KERNEL(NUMW) (region taps YK/SKI, tap 0 first) and

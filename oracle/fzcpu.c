@@ -44,6 +44,8 @@ static int HYDRO;   /* HYDROSTATIC (and HYDROSTATICDER) */
  * of every line, i.e. the thermal width at T + m xi^2 / (2 k) (0: the
  * reference's thermal width, absorption_matrix.f90:1794-1795) */
 static double XI2 = 0.0;
+/* FULL_STOKES output [nz][4][NUMW] of the next forward1d calls, or NULL */
+static double* FSOUT = NULL;
 static double xi_t(int z) { return XI2 * fz_matom[z] * MAMU / (2.0 * KBOL); }
 static int ATMPAR[8];
 static int PIX0[FZC_MAXREG], PIX1[FZC_MAXREG];
@@ -828,6 +830,7 @@ void fzc_wave(double* w) { memcpy(w, WAVE, sizeof(double) * NUMW); }
 void fzc_set_gain(const float* g) { memcpy(GAIN, g, sizeof(float) * 4 * NUMW); }
 void fzc_set_hydrostatic(int on) { HYDRO = on ? 1 : 0; }
 void fzc_set_microturbulence(double xi) { XI2 = xi * xi; }
+void fzc_set_full_stokes(double* buf) { FSOUT = buf; }
 
 /* lsf_kernel: NUMW doubles, region r's taps YK/SKI at PIX0[r]..PIX1[r]-1 with
  * tap 0 first (LSF_INIT builds them so: pre_post_duties.f90:855-876); valid:
@@ -1555,6 +1558,12 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
         for (int i = 0; i < 4; ++i) o1[i] = o1[i] + st[i];
       }
     }
+    /* FULL_STOKES (forward.f90:980-983): the Stokes vector after height k,
+     * REAL(SYN1D) * GAIN1D */
+    if (FSOUT)
+      for (int l = 0; l < NUMW; ++l)
+        for (int q = 0; q < 4; ++q)
+          FSOUT[((size_t)k * 4 + q) * NUMW + l] = (double)((float)syn[4 * l + q] * GAIN[4 * l + q]);
   }
   if (!err) {
     for (int k = 0; k < nz; ++k) kc5_out[k] = kc5[k];

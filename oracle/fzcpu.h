@@ -61,6 +61,9 @@ void fzc_set_gain(const float* gain);
 void fzc_set_hydrostatic(int on);
 /* microturbulence xi (cm/s) of every line's Doppler width; 0 (the default): thermal only */
 void fzc_set_microturbulence(double xi);
+/* FULL_STOKES: forward1d also writes the Stokes vector after every height,
+ * buf[nz][4][numw] (times GAIN1D); NULL switches it off */
+void fzc_set_full_stokes(double* buf);
 int fzc_set_lsf(const double* lsf_kernel, const int32_t* valid);
 
 /* One column (forward.f90:857-1029, HYDROSTATIC=no):

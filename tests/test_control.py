@@ -137,8 +137,7 @@ def test_reader_errors():
     assert control.parse("CONTINUUM NORMALIZATION:\nF\nTRUE\nFILEMODEL:\nm\nMODE:\ntau\n").hsra_norm
     with pytest.raises(ValueError):
         control.parse("CONTINUUM NORMALIZATION:\nyes\nFILEMODEL:\nm\nMODE:\ntau\n")
-    with pytest.raises(NotImplementedError):
-        control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nFULL_STOKES\n")
+    assert control.parse("FILEMODEL:\nm\nMODE:\ntau\nMISC SETUP:\nFULL_STOKES\n").full_stokes
 
 
 def test_line_spread_function_section(tmp_path):
@@ -194,7 +193,7 @@ def test_nlte_departure_section(tmp_path):
 
 def test_misc_setup_assists():
     """MISC SETUP TEMP / PGAS / BVEC / VLOS set ASSIST_T/P/B/V
-    (input_file.f90:170-176); FULL_STOKES is still refused."""
+    (input_file.f90:170-176); FULL_STOKES (:177) sets the full-Stokes output."""
     from firtez_dz_amd import control, gpu
     base = ("LINES:\n4,5 112 -624.0 20.0\nFILEPROFILE:\np.bin\nFILEMODEL:\nm.bin\nMODE:\ninversion\n"
             "STOKES SETUP:\nSTKI\nSTKQ\nSTKU\nSTKV\nINVERSION SETUP:\nTEM 4\nBZ 1\n")
@@ -202,8 +201,8 @@ def test_misc_setup_assists():
     assert c.assist == gpu.ASSIST_B | gpu.ASSIST_V | gpu.ASSIST_T
     assert control.control_for_inversion(c, 64).assist == c.assist
     assert control.parse(base + "MISC SETUP:\nPGAS\nEND:\n").assist == gpu.ASSIST_P
-    with pytest.raises(NotImplementedError):
-        control.parse(base + "MISC SETUP:\nFULL_STOKES\nEND:\n")
+    assert control.parse(base + "MISC SETUP:\nFULL_STOKES\nEND:\n").full_stokes
+    assert not c.full_stokes
 
 
 def test_coupled_inversion_section(tmp_path):
@@ -234,3 +233,18 @@ def test_coupled_inversion_section(tmp_path):
     assert data[1] == np.float32(psf[0, 1]) and data[5] == np.float32(psf[1, 0])      # x fastest
     back = modelio.read_psf(f)
     assert back.shape == (3, 5) and np.array_equal(back, psf.astype(np.float32).astype(np.float64))
+
+
+def test_write_bin_layout(tmp_path):
+    """WRITE_BIN (misc.f90:2352-2418): header (3003, 2997, 160206, records,
+    rank, shape...) and the values in Fortran order, read back unchanged."""
+    from firtez_dz_amd import modelio
+    a = np.asfortranarray(np.arange(2 * 3 * 4 * 5, dtype=np.float32).reshape(2, 3, 4, 5))
+    f = str(tmp_path / "x.bin")
+    modelio.write_bin(f, a)
+    raw = open(f, "rb").read()
+    hdr = np.frombuffer(raw, np.float32, 9, 4)
+    assert list(hdr) == [3003, 2997, 160206, 2, 4, 2, 3, 4, 5]
+    data = np.frombuffer(raw, np.float32, a.size, 4 + 36 + 4 + 4)
+    assert data[1] == a[1, 0, 0, 0] and data[2] == a[0, 1, 0, 0]         # Fortran order
+    assert np.array_equal(modelio.read_bin(f), a)

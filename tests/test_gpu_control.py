@@ -150,6 +150,22 @@ def test_syn1_without_continuum_normalisation(gpu_ctx, tree):
     report("test_syn1, CONTINUUM NORMALIZATION: F", err, 1e-6)
 
 
+def test_syn1_full_stokes(gpu_ctx, tree):
+    """test_syn1 with MISC SETUP FULL_STOKES: fs_<FILEPROFILE> (WRITE_BIN,
+    SYN5D(4, NUMW, NZ, NY, NX)) holds the Stokes vector after every height;
+    its top height is the written profile."""
+    def full(c):
+        c.full_stokes = True
+        c.fileprofile = "fs_run.bin"
+
+    c, out, wd = run(tree, "test_syn1", gpu_ctx, full)
+    st, _, _ = modelio.read_profiles(os.path.join(wd, "fs_run.bin"))
+    fs = modelio.read_bin(os.path.join(wd, "fs_fs_run.bin"))
+    assert fs.shape == (4, st.shape[1], fs.shape[2], 1, 1) and fs.shape[2] > 1
+    rows = [s for s in range(4) if c.invstk[s]]      # the profile file holds the set-up Stokes rows only
+    report("test_syn1 FULL_STOKES top height vs profile", float(np.max(np.abs(fs[rows, :, -1] - st[rows]))), 0.0)
+
+
 def test_syn1_with_line_spread_function(gpu_ctx, golden, tree):
     """test_syn1 with a LINE SPREAD FUNCTION: section (a Gaussian of sigma
     30 mA centred on 0, NSET_LSF / LSF_INIT): the written profiles equal the
