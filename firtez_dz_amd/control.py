@@ -24,8 +24,9 @@ RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
 not needed).  Fields of features this runner does not implement (BOX,
-LINE OPTICAL DEPTH, AUTO Stokes weights) raise NotImplementedError instead
-of being ignored.  MISC SETUP FULL_STOKES writes, in synthesis mode,
+AUTO Stokes weights) raise NotImplementedError instead of being ignored.
+LINE OPTICAL DEPTH: y writes OUTPATH/tl_<FILEMODEL> (WRITE_LOGTAUS) with
+log10 tau of every wavelength and height in synthesis and inversion modes.  MISC SETUP FULL_STOKES writes, in synthesis mode,
 OUTPATH/fs_<FILEPROFILE> (WRITE_BIN) with the Stokes vector after every
 height, (4, NUMW, NZ, NY, NX).  BOX (three rows "N D" for x,
 y, z) declares the grid instead of the model header (BOX_AVAILABLE,
@@ -55,7 +56,7 @@ from . import modelio
 PARAM_NAMES = {"TEM": "T", "PGAS": "Pg", "RHO": "rho", "BX": "Bx", "BY": "By", "BZ": "Bz", "VLOS": "vz", "P0": "P0"}
 # default weights of VERTICAL REGULARIZATION TYPE / NORM (input_file.f90:1139-1262)
 _REG_DEF = {"TEM": 1, "P0": 0, "PGAS": 1, "RHO": 1, "BX": 0, "BY": 0, "BZ": 1, "VLOS": 1}
-_UNSUPPORTED = ("LINE OPTICAL DEPTH:",)
+_UNSUPPORTED = ()
 
 
 @dataclass
@@ -97,6 +98,8 @@ class Control:
     hsra_norm: bool = True
     # MISC SETUP FULL_STOKES: the Stokes vector after every height
     full_stokes: bool = False
+    # LINE OPTICAL DEPTH: log10 tau of every wavelength (MTAULIN)
+    line_tau: bool = False
     # BOX: ((NX, DX), (NY, DY), (NZ, DZ)) or None (grid from the model header)
     box: Optional[Tuple[Tuple[int, float], ...]] = None
 
@@ -173,7 +176,7 @@ def parse(text: str) -> Control:
                 ids = tuple(int(v) for v in cols[0].split(",") if v)
                 c.lines.append((ids, int(_number(cols[1])), _number(cols[2]), _number(cols[3])))
         elif key in ("FILEPROFILE:", "FILEMODEL:", "MODE:", "HYDROSTATIC:", "HYDROSTATIC DERIVATIVES:",
-                     "WRITE RESPONSE FUNCTION:"):
+                     "WRITE RESPONSE FUNCTION:", "LINE OPTICAL DEPTH:"):
             i += 1
             val = ls[i].strip() if i < len(ls) else ""
             if key == "FILEPROFILE:":
@@ -189,6 +192,9 @@ def parse(text: str) -> Control:
                 c.hydrostatic = val[:1] in ("y", "Y")
             elif key == "HYDROSTATIC DERIVATIVES:":
                 c.hydro_ders = val[:1] not in ("n", "N")
+            elif key == "LINE OPTICAL DEPTH:":
+                # input_file.f90:559-565: y / Y switches MTAULIN on
+                c.line_tau = val[:1] in ("y", "Y")
             else:
                 if val[:1] in ("y", "Y"):
                     c.write_rf = True
@@ -527,6 +533,11 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
                 out["written"].append(p)
             out["chi2"] = chi2
             out["iterations"] = its
+        if c.line_tau and c.mode in ("synthesis", "inversion"):
+            # WRITE_LINTAU (pre_post_duties.f90:168-180): tl_<FILEMODEL>
+            p = path(c.outpath, "tl_" + c.filemodel)
+            modelio.write_logtaus(p, _line_tau(ctx, cube), z, index, wave)
+            out["written"].append(p)
         p = path(c.outpath, "out_" + c.filemodel)
         modelio.write_model(p, cube, x=cube["x"][0, 0, :], y=cube["y"][0, :, 0])
         out["written"].append(p)
@@ -551,6 +562,19 @@ def _full_stokes(ctx, cube):
         raise _gpu.FzError("full Stokes synthesis failed in %d columns" % int((status != 0).sum()))
     f = full.cpu().numpy().reshape(nx, ny, nz, 4, ctx.numw)        # [x][y][z][s][l], column = y + ny x
     return np.asfortranarray(f.transpose(3, 4, 2, 1, 0))
+
+
+def _line_tau(ctx, cube):
+    """TAU3DLIN(NUMW, NZ, NY, NX) of the cube (LINE OPTICAL DEPTH) from the
+    device; the cube itself is left as it is."""
+    import torch
+    nz, ny, nx = cube["T"].shape
+    atm = torch.from_numpy(modelio.to_device_rows(cube)).cuda()
+    tl, status = ctx.line_tau_torch(atm)
+    if int(status.max()) != 0:
+        raise _gpu.FzError("line optical depth failed in %d columns" % int((status != 0).sum()))
+    t = tl.cpu().numpy().reshape(nx, ny, nz, ctx.numw)              # [x][y][z][l], column = y + ny x
+    return np.asfortranarray(t.transpose(3, 2, 1, 0))
 
 
 def _stokes4(syn3d, numw, ny, nx):

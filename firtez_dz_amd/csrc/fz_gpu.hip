@@ -91,6 +91,9 @@ __constant__ Cfg c_cfg;
 __device__ __forceinline__ int fslot(int s, int q) { return F_REG + 4 * c_cfg.nreg + 8 * s + q; }
 __device__ __forceinline__ int freg(int r, int q) { return F_REG + 4 * r + q; }
 __device__ __forceinline__ int fgeo(int q) { return c_cfg.fgeo0 + q; }
+// 1 / (N_H m_u MW) of a height whose opacities are formed, 0 where the height
+// is opaque (LINE OPTICAL DEPTH: KLINTAU = ETAI / N_H / m_u / MW)
+__device__ __forceinline__ int ftlf() { return c_cfg.fgeo0 + NGEO; }
 
 // ---------------------------------------------------------------- microphysics
 // FORWARD1D's per-height physics (forward.f90:911-1006) over a tile of
@@ -355,7 +358,12 @@ __global__ void __launch_bounds__(256) opac_kernel(MicroTile a) {
     const bool bnd = (k == 0) || (kc5 >= 2.0e10);     // forward.f90:884,931,971
     M(F_FLAG) = bnd ? 1.0f : 0.0f;
     M(F_KC5) = (float)kc5;
-    if (bnd) return;
+    // the opacities are formed wherever kc5 is below the threshold, the
+    // bottom height included (its absorption matrix feeds LINE OPTICAL DEPTH,
+    // forward.f90:931-968, while the transfer takes it as the boundary)
+    const bool opaque = kc5 >= 2.0e10;
+    M(ftlf()) = opaque ? 0.f : (float)(1.0 / (nh * MAMU * (double)a.atm[A_MU * ps + k * cs + col]));
+    if (opaque) return;
     for (int r = 0; r < c_cfg.nreg && !err; ++r) {
       err = opac_cont(T, c_cfg.reg_l0main[r], ne, nh, d, a.der != 0, kc);
       for (int q = 0; q < 4; ++q) M(freg(r, q)) = (float)kc[q];
@@ -704,6 +712,45 @@ __device__ __forceinline__ Layer load_layer(const RtArgs& a, int col, int k) {
   ly.g.s2p = mg[fgeo(6) * ps];
   ly.rsT = mg[fgeo(7) * ps];
   return ly;
+}
+
+// LINE OPTICAL DEPTH (forward.f90:966-968, 1008-1015): per (column,
+// wavelength) KLINTAU_k = ETAI_k / N_H / m_u / MW_k from the propagation
+// matrix of every height whose opacities are formed (0 where the height is
+// opaque; the reference keeps the last computed ETAI there), then the REAL(4)
+// trapezoid from the top (tau = 1e-9) down, and log10.
+// out[((col * nz + k) * numw + l]; one lane per (column, wavelength).
+__global__ void __launch_bounds__(256) taulin_kernel(RtArgs a, float* __restrict__ out) {
+  const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int numw = c_cfg.numw, nz = c_cfg.nz;
+  if (g >= (long)a.ntile * numw) return;
+  const int ct = (int)(g / numw), l = (int)(g - (long)ct * numw), col = a.col0 + ct;
+  if (a.status && a.status[col] != 0) return;
+  int r = 0;
+  while (r + 1 < c_cfg.nreg && l >= c_cfg.reg_pix0[r + 1]) ++r;
+  const float wave = (float)(c_cfg.reg_wstart[r] + c_cfg.reg_wstep[r] * (double)(l - c_cfg.reg_pix0[r]));
+  const long ps = (long)nz * a.ncol, cs = a.ncol;
+  float* o = out + (long)col * nz * numw + l;
+  for (int k = 0; k < nz; ++k) {
+    const float fac = a.micro[ftlf() * ps + k * cs + col];
+    float kl = 0.f;
+    if (fac != 0.f) {
+      const Layer ly = load_layer(a, col, k);
+      float x7[7];
+      absorb(a, col, k, r, wave, ly, x7);
+      kl = x7[0] * fac;
+    }
+    o[(long)k * numw] = kl;
+  }
+  float tau = 1.0e-9f, kln = o[(long)(nz - 1) * numw];
+  o[(long)(nz - 1) * numw] = log10f(tau);
+  for (int k = nz - 2; k >= 0; --k) {
+    const float klk = o[(long)k * numw];
+    const float x = (klk * a.atm[A_RHO * ps + k * cs + col] + kln * a.atm[A_RHO * ps + (k + 1) * cs + col]) / 2.0f;
+    tau = (float)((double)tau + (double)x * ((c_cfg.zz[k + 1] - c_cfg.zz[k]) * 1000.0 * 100.0));
+    o[(long)k * numw] = log10f(tau);
+    kln = klk;
+  }
 }
 
 // HE (HYDROSTATIC: yes, response functions on): the T response of height K
@@ -1549,7 +1596,7 @@ int fz_configure(fz_ctx* c, const fz_line* lines, int nlines, const fz_region* r
       ++g.nbpc;
     }
   g.fgeo0 = F_REG + 4 * nreg + 8 * ns;
-  g.nfield = g.fgeo0 + NGEO;
+  g.nfield = g.fgeo0 + NGEO + 1;      // + the line-optical-depth factor
   set_xi(g, matom, c->xi);
   // layer thickness uses the spacing to the next height, the top one the
   // spacing below (rtesolver.f90:439-443); z rounded to REAL(4) like MODEL(:,13)
@@ -2131,6 +2178,32 @@ int fz_forward_full_dev(fz_ctx* c, int ncol, float* d_atm, int rf_mode, float* d
                         int32_t* d_status, void* stream) {
   if (!d_full) return FZ_EARG;
   return forward_impl(c, ncol, d_atm, rf_mode, d_stokes, d_jac, d_full, d_status, stream);
+}
+
+int fz_line_tau_dev(fz_ctx* c, int ncol, float* d_atm, float* d_tau, int32_t* d_status, void* stream) {
+  if (!c || !c->configured || ncol < 1 || !d_atm || !d_tau) return FZ_EARG;
+  if (c->win_on) return set_err(c, FZ_EARG, "per-pixel windows: no line optical depth");
+  const Cfg& g = c->cfg;
+  hipStream_t st = (hipStream_t)stream;
+  HIPCHK(c, hipSetDevice(c->device));
+  int r = upload_cfg(c);
+  if (r) return r;
+  r = ensure(c, &c->d_micro, &c->micro_cap, sizeof(float) * (size_t)g.nfield * g.nz * ncol);
+  if (r) return r;
+  r = run_micro(c, ncol, d_atm, &d_status, false, c->hydro, true, st);
+  if (r) return r;
+  RtArgs a = {};
+  a.ncol = ncol;
+  a.col0 = 0;
+  a.ntile = ncol;
+  a.atm = d_atm;
+  a.micro = c->d_micro;
+  a.gain = c->d_gain;
+  a.status = d_status;
+  const long n = (long)ncol * g.numw;
+  hipLaunchKernelGGL(taulin_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, d_tau);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
 }
 
 int fz_set_overlap(fz_ctx* c, int on) {

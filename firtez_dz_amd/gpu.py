@@ -92,6 +92,7 @@ def library():
     L.fz_forward3d.argtypes = [vp, ip, ip] + [fp] * 12 + [ctypes.POINTER(ctypes.c_int32), ip, dp, fp]
     L.fz_set_tile.argtypes = [vp, ip, ip]
     L.fz_forward_full_dev.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, vp]
+    L.fz_line_tau_dev.argtypes = [vp, ip, vp, vp, vp, vp]
     L.fz_set_overlap.argtypes = [vp, ip]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_microturbulence.argtypes = [vp, ctypes.c_double]
@@ -143,7 +144,7 @@ def library():
 def exported_symbols():
     """Names declared in include/firtez_gpu.h."""
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
-            "fz_forward_dev", "fz_forward_full_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
+            "fz_forward_dev", "fz_forward_full_dev", "fz_line_tau_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_set_overlap", "fz_last_timing",
             "fz_set_hydrostatic", "fz_set_microturbulence", "fz_set_lsf", "fz_set_departure", "fz_set_psf", "fz_psf_convolve_dev", "fz_coupled_invert3d_dev", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
@@ -434,6 +435,18 @@ class Context:
                                                  jac.data_ptr() if npar else None, full.data_ptr(),
                                                  status.data_ptr(), stream), "fz_forward_full_dev")
         return stokes, (jac if npar else None), status, full
+
+    def line_tau_torch(self, atm, status=None):
+        """LINE OPTICAL DEPTH (fz_line_tau_dev): log10 tau [ncol, nz, numw] of
+        every wavelength; atm [12, nz, ncol] float32 cuda (EOS-updated)."""
+        import torch
+        ncol = atm.shape[2]
+        out = torch.empty((ncol, self.nz, self.numw), dtype=torch.float32, device=atm.device)
+        if status is None:
+            status = torch.zeros(ncol, dtype=torch.int32, device=atm.device)
+        self._check(self.lib.fz_line_tau_dev(self.h, ncol, atm.data_ptr(), out.data_ptr(), status.data_ptr(),
+                                             torch.cuda.current_stream(atm.device).cuda_stream), "fz_line_tau_dev")
+        return out, status
 
     def hydrostatic_torch(self, atm, status=None):
         import torch

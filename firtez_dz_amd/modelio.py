@@ -25,6 +25,7 @@ LSF_MAGIC = 3330904
 RF_MAGIC = 18060904
 PSF_MAGIC = 161906
 BIN_MAGIC = 160206
+LOGTAU_MAGIC = 12200904
 VERSION = (3000, 3)      # SS, VV (format 3000, version 3) written as SS+VV, SS-VV (misc.f90:944-953)
 NMAXREC = 1073741784 // 4
 ORDER = ("T", "Pg", "rho", "Bx", "By", "Bz", "vz", "Pe", "mu", "tau5", "x", "y", "z")
@@ -293,3 +294,30 @@ def read_bin(path):
     if data.size != int(np.prod(shape)):
         raise ValueError("%s: %d values, header says %s" % (path, data.size, shape))
     return data.reshape(shape, order="F")
+
+
+def write_logtaus(path, taus, z_km, index, wave_ma):
+    """WRITE_LOGTAUS_V3 (misc.f90:2088-2199): taus (NW, NZ, NY, NX) of
+    log10 tau per wavelength and height; the header record (SS+VV, SS-VV,
+    12200904, records, 4, NX, NY, NZ, NW), then Z, IND and WAVE records and
+    the values with the wavelength fastest."""
+    t = np.asarray(taus, np.float32)
+    nw, nz, ny, nx = t.shape
+    ss, vv = VERSION
+    header = np.array([ss + vv, ss - vv, LOGTAU_MAGIC, _nrecs(t.size) + 4, 4, nx, ny, nz, nw], np.float32)
+    _write_records(path, header, [np.asarray(z_km, np.float64).astype(np.float32),
+                                  np.asarray(index, np.float64).astype(np.float32),
+                                  np.asarray(wave_ma, np.float64).astype(np.float32), t.reshape(-1, order="F")])
+
+
+def read_logtaus(path):
+    """(taus (NW, NZ, NY, NX), z, index, wave) of a WRITE_LOGTAUS file."""
+    recs = _records(path)
+    hdr = recs[0]
+    if abs(int(round(float(hdr[2]))) - LOGTAU_MAGIC) > 0:
+        raise ValueError("%s is not an optical-depth file" % path)
+    nx, ny, nz, nw = (int(v) for v in hdr[5:9])
+    data = np.concatenate(recs[4:]) if len(recs) > 4 else np.zeros(0, np.float32)
+    if data.size != nw * nz * ny * nx:
+        raise ValueError("%s: %d values, header says %d" % (path, data.size, nw * nz * ny * nx))
+    return data.reshape((nw, nz, ny, nx), order="F"), recs[1], recs[2], recs[3]

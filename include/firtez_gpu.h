@@ -162,6 +162,16 @@ int fz_forward_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_st
 int fz_forward_full_dev(fz_ctx* ctx, int ncol, float* d_atm, int rf_mode, float* d_stokes, float* d_jac,
                         float* d_full, int32_t* d_status, void* stream);
 
+/* LINE OPTICAL DEPTH (MTAULIN, forward.f90:966-968, 1008-1015; written to
+ * tl_<FILEMODEL> by WRITE_LINTAU, pre_post_duties.f90:168-180): d_tau
+ * [ncol][nz][numw] float receives log10 of the optical depth at every
+ * wavelength, integrated like log tau5 from the top (tau = 1e-9) with
+ * KLINTAU = ETAI / N_H / m_u / MW per height.  Heights whose 5000 A
+ * continuum opacity reaches the boundary threshold contribute 0 (the
+ * reference reuses the last ETAI it formed there).  Runs the EOS on d_atm
+ * (updated like a forward).  Not with per-pixel windows. */
+int fz_line_tau_dev(fz_ctx* ctx, int ncol, float* d_atm, float* d_tau, int32_t* d_status, void* stream);
+
 /* FORWARD3D on host arrays in the reference's layout (forward.f90:43-306):
  * cubes(NZ,NY,NX) column-major float (TEM3D..TAU3D5; PG3D, RHO3D, PEL3D,
  * MW3D and TAU3D5 are written back as SPLIT_MODEL3D does, forward.f90:1225-1270),

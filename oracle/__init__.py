@@ -100,6 +100,7 @@ class Cpu:
         L.fzc_set_hydrostatic.argtypes = [ctypes.c_int]
         L.fzc_set_microturbulence.argtypes = [ctypes.c_double]
         L.fzc_set_full_stokes.argtypes = [ctypes.c_void_p]
+        L.fzc_set_line_tau.argtypes = [ctypes.c_void_p]
         L.fzc_set_lsf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_set_departure.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.fzc_evol.argtypes = [_dp, ctypes.c_double, _dp, _dp, _dp]
@@ -218,6 +219,17 @@ class Cpu:
         finally:
             self.lib.fzc_set_full_stokes(None)
         return out + (fs,)
+
+    def forward1d_line_tau(self, model):
+        """forward1d plus LINE OPTICAL DEPTH: (model, syn, dsyn, kc5, log10 tau [nz][numw])."""
+        nz = model.shape[1]
+        tl = np.zeros((nz, self.numw))
+        self.lib.fzc_set_line_tau(tl.ctypes.data)
+        try:
+            out = self.forward1d(model)
+        finally:
+            self.lib.fzc_set_line_tau(None)
+        return out + (tl,)
 
     def set_lsf(self, lsf_kernel=None, valid=None):
         """MLSF: LSF_KERNEL(NUMW) (region taps YK/SKI, tap 0 first) and

@@ -46,6 +46,9 @@ static int HYDRO;   /* HYDROSTATIC (and HYDROSTATICDER) */
 static double XI2 = 0.0;
 /* FULL_STOKES output [nz][4][NUMW] of the next forward1d calls, or NULL */
 static double* FSOUT = NULL;
+/* LINE OPTICAL DEPTH output [nz][NUMW] (log10 tau) of the next forward1d
+ * calls, or NULL */
+static double* TLOUT = NULL;
 static double xi_t(int z) { return XI2 * fz_matom[z] * MAMU / (2.0 * KBOL); }
 static int ATMPAR[8];
 static int PIX0[FZC_MAXREG], PIX1[FZC_MAXREG];
@@ -831,6 +834,7 @@ void fzc_set_gain(const float* g) { memcpy(GAIN, g, sizeof(float) * 4 * NUMW); }
 void fzc_set_hydrostatic(int on) { HYDRO = on ? 1 : 0; }
 void fzc_set_microturbulence(double xi) { XI2 = xi * xi; }
 void fzc_set_full_stokes(double* buf) { FSOUT = buf; }
+void fzc_set_line_tau(double* buf) { TLOUT = buf; }
 
 /* lsf_kernel: NUMW doubles, region r's taps YK/SKI at PIX0[r]..PIX1[r]-1 with
  * tap 0 first (LSF_INIT builds them so: pre_post_duties.f90:855-876); valid:
@@ -1471,6 +1475,11 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
       }
       if (err) break;
     }
+    /* KLINTAU (forward.f90:966-968): ETAI / N_H / m_u / MW where the
+     * absorption matrix was formed at this height, 0 where it is opaque */
+    if (TLOUT)
+      for (int l = 0; l < NUMW; ++l)
+        TLOUT[(size_t)k * NUMW + l] = (kc5[k] < THR) ? ETA(A.eta, 0, l) / nh * (1.0 / MAMU) / (double)MW[k] : 0.0;
     for (int l = 0; l < NUMW; ++l) {
       double lam = region_lambda_cm(l);
       const int rg = region_of(l);
@@ -1564,6 +1573,21 @@ int fzc_forward1d(int nz, float* model, double* syn_out, double* dsyn_out, doubl
       for (int l = 0; l < NUMW; ++l)
         for (int q = 0; q < 4; ++q)
           FSOUT[((size_t)k * 4 + q) * NUMW + l] = (double)((float)syn[4 * l + q] * GAIN[4 * l + q]);
+  }
+  if (!err && TLOUT) {
+    /* TAULIN (forward.f90:1008-1015): REAL(4) trapezoid from the top, log10 */
+    for (int l = 0; l < NUMW; ++l) {
+      float tau = 1.0e-9f;
+      float kln = (float)TLOUT[(size_t)(nz - 1) * NUMW + l];
+      TLOUT[(size_t)(nz - 1) * NUMW + l] = (double)log10f(tau);
+      for (int k = nz - 2; k >= 0; --k) {
+        const float klk = (float)TLOUT[(size_t)k * NUMW + l];
+        const float x = (klk * RHO[k] + kln * RHO[k + 1]) / 2.0f;
+        tau = (float)((double)tau + (double)x * ((zz[k + 1] - zz[k]) * 1000.0 * 100.0));
+        TLOUT[(size_t)k * NUMW + l] = (double)log10f(tau);
+        kln = klk;
+      }
+    }
   }
   if (!err) {
     for (int k = 0; k < nz; ++k) kc5_out[k] = kc5[k];

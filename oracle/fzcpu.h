@@ -64,6 +64,9 @@ void fzc_set_microturbulence(double xi);
 /* FULL_STOKES: forward1d also writes the Stokes vector after every height,
  * buf[nz][4][numw] (times GAIN1D); NULL switches it off */
 void fzc_set_full_stokes(double* buf);
+/* LINE OPTICAL DEPTH: forward1d also writes log10 tau at every height and
+ * wavelength, buf[nz][numw]; NULL switches it off */
+void fzc_set_line_tau(double* buf);
 int fzc_set_lsf(const double* lsf_kernel, const int32_t* valid);
 
 /* One column (forward.f90:857-1029, HYDROSTATIC=no):

@@ -128,8 +128,9 @@ def test_reader_errors():
     assert c.box == ((4, 16.0), (3, 16.0), (64, 15.0))
     with pytest.raises(ValueError):
         control.parse("BOX:\n4 16.0\n3\nFILEMODEL:\nm\nMODE:\ntau\n")
-    with pytest.raises(NotImplementedError):
-        control.parse("LINE OPTICAL DEPTH:\ny\nFILEMODEL:\nm\nMODE:\ntau\n")
+    # LINE OPTICAL DEPTH (input_file.f90:559-565): y switches it on
+    assert control.parse("LINE OPTICAL DEPTH:\ny\nFILEMODEL:\nm\nMODE:\ntau\n").line_tau
+    assert not control.parse("LINE OPTICAL DEPTH:\nno\nFILEMODEL:\nm\nMODE:\ntau\n").line_tau
     # CONTINUUM NORMALIZATION (input_file.f90:1480-1520): T or F on the
     # section's last row, anything else is an error
     assert control.parse("FILEMODEL:\nm\nMODE:\ntau\n").hsra_norm
@@ -248,3 +249,17 @@ def test_write_bin_layout(tmp_path):
     data = np.frombuffer(raw, np.float32, a.size, 4 + 36 + 4 + 4)
     assert data[1] == a[1, 0, 0, 0] and data[2] == a[0, 1, 0, 0]         # Fortran order
     assert np.array_equal(modelio.read_bin(f), a)
+
+
+def test_write_logtaus_layout(tmp_path):
+    """WRITE_LOGTAUS_V3 (misc.f90:2088-2199): header (3003, 2997, 12200904,
+    4 + records, 4, NX, NY, NZ, NW), the Z, IND and WAVE records, then the
+    values with the wavelength fastest; read back unchanged."""
+    from firtez_dz_amd import modelio
+    t = np.asfortranarray(np.arange(5 * 4 * 3 * 2, dtype=np.float32).reshape(5, 4, 3, 2))
+    f = str(tmp_path / "tl.bin")
+    modelio.write_logtaus(f, t, np.arange(4.0), np.full(5, 6301.5), np.linspace(-100, 100, 5))
+    raw = open(f, "rb").read()
+    assert list(np.frombuffer(raw, np.float32, 9, 4)) == [3003, 2997, 12200904, 5, 4, 2, 3, 4, 5]
+    back, z, ind, wave = modelio.read_logtaus(f)
+    assert np.array_equal(back, t) and np.array_equal(z, np.arange(4.0, dtype=np.float32))

@@ -166,6 +166,29 @@ def test_syn1_full_stokes(gpu_ctx, tree):
     report("test_syn1 FULL_STOKES top height vs profile", float(np.max(np.abs(fs[rows, :, -1] - st[rows]))), 0.0)
 
 
+def test_syn1_line_optical_depth(gpu_ctx, tree):
+    """test_syn1 with LINE OPTICAL DEPTH: y: tl_<FILEMODEL> (WRITE_LOGTAUS)
+    holds log10 tau of every wavelength and height, -9 at the top, growing
+    downward, identical to fz_line_tau_dev on the written model."""
+    import torch
+
+    def tl(c):
+        c.line_tau = True
+
+    c, out, wd = run(tree, "test_syn1", gpu_ctx, tl)
+    taus, z, ind, wave = modelio.read_logtaus(os.path.join(wd, "tl_" + c.filemodel))
+    nw, nz = taus.shape[:2]
+    assert taus.shape[2:] == (1, 1) and nz == len(z) and nw == len(wave)
+    assert np.all(taus[:, -1] == np.float32(np.log10(np.float32(1e-9))))
+    assert np.all(np.diff(taus[:, :, 0, 0], axis=1) <= 0)
+    m = modelio.read_model(os.path.join(wd, "out_" + c.filemodel))
+    atm = torch.from_numpy(modelio.to_device_rows(m)).cuda()
+    t2, st = gpu_ctx.line_tau_torch(atm)
+    assert int(st.max()) == 0
+    report("test_syn1 LINE OPTICAL DEPTH vs fz_line_tau_dev", float(np.max(np.abs(taus[:, :, 0, 0] -
+                                                                                t2.cpu().numpy()[0].T))), 0.0)
+
+
 def test_syn1_with_line_spread_function(gpu_ctx, golden, tree):
     """test_syn1 with a LINE SPREAD FUNCTION: section (a Gaussian of sigma
     30 mA centred on 0, NSET_LSF / LSF_INIT): the written profiles equal the
