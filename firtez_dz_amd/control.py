@@ -23,8 +23,8 @@ maximum nodes, POW, MAXITER), VERTICAL REGULARIZATION TYPE / NORM, WRITE
 RESPONSE FUNCTION, MISC SETUP (DATAPATH, MODPATH, OUTPATH, SLDPATH, NOISE,
 SVDTOL, LINEAR_SOLVER, PERTURBATION_TYPE).  The line data come from the
 package's table of the reference's lines_database.dat (SLDPATH is read but
-not needed).  Fields of features this runner does not implement (BOX,
-AUTO Stokes weights) raise NotImplementedError instead of being ignored.
+not needed).  STOKES SETUP: AUTO weights every pixel by the spread of its own
+observed Stokes blocks (SET_WEIGHTS3D) in the uncoupled inversion.
 LINE OPTICAL DEPTH: y writes OUTPATH/tl_<FILEMODEL> (WRITE_LOGTAUS) with
 log10 tau of every wavelength and height in synthesis and inversion modes.  MISC SETUP FULL_STOKES writes, in synthesis mode,
 OUTPATH/fs_<FILEPROFILE> (WRITE_BIN) with the Stokes vector after every
@@ -100,6 +100,8 @@ class Control:
     full_stokes: bool = False
     # LINE OPTICAL DEPTH: log10 tau of every wavelength (MTAULIN)
     line_tau: bool = False
+    # STOKES SETUP: AUTO -- every pixel weighted by its own observations
+    auto_weights: bool = False
     # BOX: ((NX, DX), (NY, DY), (NZ, DZ)) or None (grid from the model header)
     box: Optional[Tuple[Tuple[int, float], ...]] = None
 
@@ -208,7 +210,7 @@ def parse(text: str) -> Control:
                         c.wstk[s] = float(w)
                         nstk += 1
                 if _field(row, "AUTO", 1)[0]:
-                    raise NotImplementedError("automatic Stokes weights are not supported by this path")
+                    c.auto_weights = True
         elif key == "INVERSION SETUP:":
             body, i = section(i)
             for row in body:
@@ -454,6 +456,7 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
         if any(f.upper() != "LTE" for _, f in c.nlte):
             ctx.set_departure(*departure_arrays(c, nz, workdir))
         ctx.set_hydrostatic(c.hydrostatic)
+        ctx.set_auto_weights(c.auto_weights)
         if c.hydrostatic and not c.hydro_ders and c.mode == "inversion":
             raise NotImplementedError("HYDROSTATIC DERIVATIVES: no is not supported by this path")
         out = {"written": []}
@@ -549,6 +552,7 @@ def run(c: Control, workdir: str = ".", ctx: Optional[_gpu.Context] = None) -> d
             ctx.close()
         else:
             ctx.set_hydrostatic(hydro_before)
+            ctx.set_auto_weights(False)
 
 
 def _full_stokes(ctx, cube):

@@ -1304,6 +1304,9 @@ struct fz_ctx {
   // default: on the 288x288x96 forward the concurrent kernels slowed the
   // transfer more than they hid (112.5 ms per step against 108)
   bool overlap = false;
+  // STOKES SETUP: AUTO (input_file.f90:1294): per-pixel weights from the
+  // observations in fz_invert3d_dev
+  bool autow = false;
   hipStream_t aux = nullptr;
   std::vector<hipEvent_t> tev;
   // inversion
@@ -2212,6 +2215,74 @@ int fz_set_overlap(fz_ctx* c, int on) {
   return FZ_OK;
 }
 
+int fz_set_auto_weights(fz_ctx* c, int on) {
+  if (!c) return FZ_EARG;
+  c->autow = on != 0;
+  return FZ_OK;
+}
+
+// SET_WEIGHTS3D's per-pixel work (get_dmodel.f90:2051-2170): SET_WEIGHTS1D
+// (:2191-2224) on the pixel's observed row, WSTK = those weights in REAL(4),
+// then SET_ISIGMAP (:356-373).  One lane per pixel; the row is nstk blocks of
+// numw values, the blocks of the inverted Stokes parameters in I, Q, U, V
+// order.  STDDEV (:2228-2251): mean as the sum of x_i / n, then the n - 1
+// variance, both in sequence order.
+__global__ void auto_weights_kernel(int ncol, int numw, int inv_mask, double isigma, const double* __restrict__ obs,
+                                    double* __restrict__ isig) {
+  const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  int nstk = 0;
+  for (int s = 0; s < 4; ++s) nstk += (inv_mask >> s) & 1;
+  const int nf = nstk * numw;
+  const double* o = obs + col * nf;
+  double w[4], minw = 1.0e60;
+  const double dn = (double)numw;
+  for (int s = 0, b = 0; s < 4; ++s) {
+    w[s] = 0.0;
+    if (!((inv_mask >> s) & 1)) continue;
+    const double* x = o + (long)b * numw;
+    double mn = 0.0;
+    for (int i = 0; i < numw; ++i) mn += x[i] / dn;
+    double v = 0.0;
+    for (int i = 0; i < numw; ++i) v += (x[i] - mn) * (x[i] - mn);
+    const double sd = sqrt(v / (dn - 1.0));
+    w[s] = sqrt(1.0 / (sd * sd));
+    if (w[s] < minw) minw = w[s];
+    ++b;
+  }
+  minw *= 10.0;
+  double ss = 0.0;
+  for (int s = 0; s < 4; ++s) {
+    if (w[s] > minw) w[s] = minw;
+    ss += w[s] * w[s];
+  }
+  const double nrm = sqrt(ss) / sqrt((double)nstk);
+  double* out = isig + col * nf;
+  for (int s = 0, b = 0; s < 4; ++s) {
+    if (!((inv_mask >> s) & 1)) continue;
+    const float ws = (float)(w[s] / nrm);
+    const double v = (double)(ws * ws) / (isigma * isigma);
+    for (int i = 0; i < numw; ++i) out[(long)b * numw + i] = v;
+    ++b;
+  }
+}
+
+int fz_auto_weights_dev(fz_ctx* c, int ncol, const double* d_obs, const int32_t* invstk, float noise,
+                        double* d_isigmap, void* stream) {
+  if (!c || !c->configured || ncol < 1 || !d_obs || !invstk || !d_isigmap) return FZ_EARG;
+  if (c->win_on) return set_err(c, FZ_ECONFIG, "automatic Stokes weights take the dense spectral grid");
+  if (c->cfg.numw < 2) return set_err(c, FZ_ECONFIG, "automatic Stokes weights need two wavelengths or more");
+  int mask = 0;
+  for (int s = 0; s < 4; ++s) mask |= invstk[s] ? 1 << s : 0;
+  if (!mask) return set_err(c, FZ_ECONFIG, "no Stokes parameter inverted");
+  HIPCHK(c, hipSetDevice(c->device));
+  const double isigma = (double)(noise > 0 ? noise : 4.0e-3f);
+  hipLaunchKernelGGL(auto_weights_kernel, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, ncol,
+                     c->cfg.numw, mask, isigma, d_obs, d_isigmap);
+  HIPCHK(c, hipGetLastError());
+  return FZ_OK;
+}
+
 int fz_set_profiling(fz_ctx* c, int on) {
   if (!c) return FZ_EARG;
   HIPCHK(c, hipSetDevice(c->device));
@@ -3070,8 +3141,8 @@ int fz_get_dmodel3ds(fz_ctx* c, int nx, int ny, const double* obs3d, const doubl
 // the compacted rows of those pixels, everything else is in pixel space.
 static int lm_step_active(fz_ctx* c, int ncol, int na, const int32_t* pix_of, const double* d_obs,
                           const float* d_stokes_c, double* d_best_syn, double* d_sla, const float* d_eqv_c,
-                          float* d_eqv_best, const double* d_isigmap, float* d_atm, float* d_atm_best, int32_t* d_done,
-                          hipStream_t st) {
+                          float* d_eqv_best, const double* d_isigmap, long isig_stride, float* d_atm,
+                          float* d_atm_best, int32_t* d_done, hipStream_t st) {
   LmArgs a = c->lm;
   a.ncol = ncol;
   a.obs = d_obs;
@@ -3081,6 +3152,7 @@ static int lm_step_active(fz_ctx* c, int ncol, int na, const int32_t* pix_of, co
   a.eqv = d_eqv_c;
   a.eqv_best = d_eqv_best;
   a.isigmap = d_isigmap;
+  a.isig_stride = isig_stride;
   a.atm = d_atm;
   a.atm_best = d_atm_best;
   a.done = d_done;
@@ -3207,7 +3279,11 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   if (npmax > LM_MAXNP_BIG) return set_err(c, FZ_ECONFIG, "more than 2048 free parameters in the last cycle");
   DevBuf b_isig, b_best, b_eqv, b_eqvb, b_bsyn, b_sla, b_done, b_stat, b_t0, b_t1, b_atmc, b_idx, b_cnt;
   const size_t natm = (size_t)12 * nz * ncol;
-  HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq));
+  // STOKES SETUP: AUTO replaces the map-wide weights by every pixel's own
+  // (SET_WEIGHTS, get_dmodel.f90:2024-2047), set once from the observations
+  const long isig_stride = c->autow ? nfreq : 0;
+  if (c->autow && win) return set_err(c, FZ_ECONFIG, "automatic Stokes weights take the dense spectral grid");
+  HIPCHK(c, hipMalloc(&b_isig.p, sizeof(double) * nfreq * (c->autow ? (size_t)ncol : 1)));
   HIPCHK(c, hipMalloc(&b_best.p, sizeof(float) * natm));
   HIPCHK(c, hipMalloc(&b_eqv.p, sizeof(float) * nwtot * npmax * nstk));
   HIPCHK(c, hipMalloc(&b_eqvb.p, sizeof(float) * nwtot * npmax * nstk));
@@ -3222,7 +3298,12 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
   HIPCHK(c, hipMalloc(&b_atmc.p, sizeof(float) * natm));
   HIPCHK(c, hipMalloc(&b_idx.p, sizeof(int32_t) * ncol));
   HIPCHK(c, hipMalloc(&b_cnt.p, sizeof(int32_t)));
-  HIPCHK(c, hipMemcpyAsync(b_isig.p, isig.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, st));
+  if (c->autow) {
+    int r = fz_auto_weights_dev(c, ncol, d_obs, ctl->invstk, ctl->noise, (double*)b_isig.p, stream);
+    if (r) return r;
+  } else {
+    HIPCHK(c, hipMemcpyAsync(b_isig.p, isig.data(), sizeof(double) * nfreq, hipMemcpyHostToDevice, st));
+  }
   float* best = (float*)b_best.p;
   double* sla = (double*)b_sla.p;
   int32_t* done = (int32_t*)b_done.p;
@@ -3289,7 +3370,7 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
         r = fz_forward_nodes_dev(c, ncol, d_atm, d_stokes, (float*)b_eqv.p, nullptr, stream);
         if (r) return r;
         r = lm_step_active(c, ncol, ncol, nullptr, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p,
-                           (float*)b_eqvb.p, (const double*)b_isig.p, d_atm, best, done, st);
+                           (float*)b_eqvb.p, (const double*)b_isig.p, isig_stride, d_atm, best, done, st);
         if (r) return r;
       } else {
         const int32_t* idx = (const int32_t*)b_idx.p;
@@ -3302,7 +3383,7 @@ int fz_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const double* d_obs
         hipLaunchKernelGGL(scatter_cols_kernel, dim3(gc), dim3(256), 0, st, (const float*)atmc, d_atm, idx, na,
                            ncol, 12 * nz);
         r = lm_step_active(c, ncol, na, idx, d_obs, d_stokes, (double*)b_bsyn.p, sla, (const float*)b_eqv.p,
-                           (float*)b_eqvb.p, (const double*)b_isig.p, d_atm, best, done, st);
+                           (float*)b_eqvb.p, (const double*)b_isig.p, isig_stride, d_atm, best, done, st);
         if (r) return r;
       }
       hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(256), 0, st, ncol, sla, done, (double*)b_stat.p);
@@ -3377,6 +3458,7 @@ int fz_coupled_invert3d_dev(fz_ctx* c, int nx, int ny, float* d_atm, const doubl
                             void* stream) {
   if (!c || !c->configured || nx < 1 || ny < 1 || !d_atm || !d_obs || !ctl || !d_stokes) return FZ_EARG;
   if (c->win_on) return set_err(c, FZ_ECONFIG, "the coupled inversion takes the dense spectral grid");
+  if (c->autow) return set_err(c, FZ_ECONFIG, "the coupled inversion takes the map-wide Stokes weights");
   // the map transposes put 32 pixels per workgroup row on the grid's y axis
   if ((long)nx * ny > 32L * 65535) return set_err(c, FZ_EARG, "more than 2097120 pixels in one coupled map");
   const int nz = c->cfg.nz, numw = c->cfg.numw, ncol = nx * ny;

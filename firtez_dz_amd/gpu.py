@@ -94,6 +94,8 @@ def library():
     L.fz_forward_full_dev.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, vp]
     L.fz_line_tau_dev.argtypes = [vp, ip, vp, vp, vp, vp]
     L.fz_set_overlap.argtypes = [vp, ip]
+    L.fz_set_auto_weights.argtypes = [vp, ip]
+    L.fz_auto_weights_dev.argtypes = [vp, ip, vp, vp, ctypes.c_float, vp, vp]
     L.fz_set_hydrostatic.argtypes = [vp, ip]
     L.fz_set_microturbulence.argtypes = [vp, ctypes.c_double]
     L.fz_set_lsf.argtypes = [vp, vp, vp]
@@ -146,6 +148,7 @@ def exported_symbols():
     return ["fz_create", "fz_destroy", "fz_last_error", "fz_configure", "fz_numw", "fz_gain_hsra", "fz_set_gain",
             "fz_forward_dev", "fz_forward_full_dev", "fz_line_tau_dev", "fz_forward3d", "fz_hydrostatic_dev", "fz_malloc", "fz_free", "fz_memcpy_h2d",
             "fz_memcpy_d2h", "fz_synchronize", "fz_set_tile", "fz_set_profiling", "fz_set_overlap", "fz_last_timing",
+            "fz_set_auto_weights", "fz_auto_weights_dev",
             "fz_set_hydrostatic", "fz_set_microturbulence", "fz_set_lsf", "fz_set_departure", "fz_set_psf", "fz_psf_convolve_dev", "fz_coupled_invert3d_dev", "fz_assist_dev", "fz_set_windows", "fz_window_offsets",
             "fz_int_coefs", "fz_set_inversion", "fz_forward_nodes_dev", "fz_lm_step_dev", "fz_invert3d_dev",
             "fz_invert3d", "fz_coupled_invert3d",
@@ -271,6 +274,23 @@ class Context:
         """Microphysics of the next radiative-transfer tile on a second stream
         while the current tile's transfer runs (the library's default is off)."""
         self._check(self.lib.fz_set_overlap(self.h, int(on)), "fz_set_overlap")
+
+    def set_auto_weights(self, on: bool = True):
+        """STOKES SETUP: AUTO -- invert3d weights every pixel by the spread of
+        its own observed Stokes blocks (SET_WEIGHTS3D) instead of wstk."""
+        self._check(self.lib.fz_set_auto_weights(self.h, int(on)), "fz_set_auto_weights")
+
+    def auto_weights_torch(self, obs, invstk, noise: float):
+        """SET_WEIGHTS3D on device: obs [ncol, NFREQ] float64 -> the per-pixel
+        ISIGMAP3D [ncol, NFREQ] float64 (WSTK^2 / ISIGMA^2)."""
+        import torch
+        assert obs.is_cuda and obs.dtype == torch.float64 and obs.is_contiguous() and obs.dim() == 2
+        out = torch.empty_like(obs)
+        inv = (ctypes.c_int32 * 4)(*[int(bool(v)) for v in invstk])
+        stream = torch.cuda.current_stream(obs.device).cuda_stream
+        self._check(self.lib.fz_auto_weights_dev(self.h, obs.shape[0], obs.data_ptr(), inv, float(noise),
+                                                 out.data_ptr(), stream), "fz_auto_weights_dev")
+        return out
 
     def set_hydrostatic(self, on: bool = True):
         """HYDROSTATIC: yes -- Pg re-integrated from the top inside every

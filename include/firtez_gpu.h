@@ -389,6 +389,24 @@ typedef struct {
 
 enum { FZ_ASSIST_B = 1, FZ_ASSIST_V = 2, FZ_ASSIST_T = 4, FZ_ASSIST_P = 8 };
 
+/* STOKES SETUP: AUTO (input_file.f90:1294, SET_WEIGHTS get_dmodel.f90:2024-2047).
+ * With it on, fz_invert3d_dev / fz_invert3d weight every pixel by its own
+ * observations (fz_auto_weights_dev) in place of the control's wstk; the
+ * coupled inversion and per-pixel windows refuse it (FZ_ECONFIG).  Default off. */
+int fz_set_auto_weights(fz_ctx* ctx, int on);
+
+/* SET_WEIGHTS3D (get_dmodel.f90:2051-2170) on device data: for every pixel of
+ * d_obs [ncol][NFREQ] (the inverted Stokes blocks, I Q U V order) the weight
+ * of block s is 1 / STDDEV of that block (n - 1 variance), capped at 10 x
+ * the pixel's smallest, the four normalised to a root mean square of 1 over
+ * the inverted parameters (SET_WEIGHTS1D :2191-2224) and rounded to REAL(4);
+ * d_isigmap [ncol][NFREQ] receives WSTK^2 / ISIGMA^2 (SET_ISIGMAP :356-373),
+ * ISIGMA the REAL(4) noise (0 -> 4e-3).  A block of equal values gives an
+ * infinite weight before the cap, as in the reference.  Needs NUMW >= 2 and
+ * the dense spectral grid.  Asynchronous on stream. */
+int fz_auto_weights_dev(fz_ctx* ctx, int ncol, const double* d_obs, const int32_t* invstk, float noise,
+                        double* d_isigmap, void* stream);
+
 /* SET_TAU's assist initialisers (invert3d.f90:528-700) on device data, for
  * hosts that keep their own cycle loop: from the observations d_obs
  * [ncol][NFREQ] (the inverted Stokes blocks, as fz_invert3d_dev takes them;

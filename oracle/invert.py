@@ -52,6 +52,47 @@ def weights(invstk, wstk, noise, numw):
     return np.concatenate(out)
 
 
+def auto_weights(obs, invstk, noise, numw):
+    """SET_WEIGHTS3D (get_dmodel.f90:2051-2170) on obs [ncol, NFREQ]: per pixel,
+    SET_WEIGHTS1D (:2191-2224) -- weight of inverted block s = SQRT(1/STD**2),
+    STD from STDDEV (:2228-2251: mean as the running sum of x_i/n, n - 1
+    variance); weights above 10 x the smallest cut to it; all divided by
+    SQRT(SUM w**2)/SQRT(NSTKINV) -- then WSTK = REAL(4) of them and SET_ISIGMAP
+    (:356-373): WSTK**2 (REAL(4)) / ISIGMA**2.  Returns [ncol, NFREQ]."""
+    obs = np.asarray(obs, np.float64)
+    isigma = float(np.float32(noise))
+    nstk = int(sum(1 for s in range(4) if invstk[s]))
+    out = np.zeros_like(obs)
+    for c in range(obs.shape[0]):
+        w = [0.0] * 4
+        minw = 1.0e60
+        b = 0
+        for s in range(4):
+            if not invstk[s]:
+                continue
+            x = obs[c, b * numw:(b + 1) * numw]
+            mn = 0.0
+            for v in x:
+                mn += float(v) / float(numw)
+            var = 0.0
+            for v in x:
+                var += (float(v) - mn) * (float(v) - mn)
+            sd = math.sqrt(var / (numw - 1.0))
+            w[s] = math.sqrt(1.0 / (sd * sd)) if sd > 0 else math.inf
+            minw = min(minw, w[s])
+            b += 1
+        minw *= 10.0
+        w = [min(v, minw) for v in w]
+        nrm = math.sqrt(sum(v * v for v in w)) / math.sqrt(nstk)
+        b = 0
+        for s in range(4):
+            if invstk[s]:
+                ws = np.float32(w[s] / nrm)
+                out[c, b * numw:(b + 1) * numw] = float(np.float32(ws * ws)) / (isigma * isigma)
+                b += 1
+    return out
+
+
 def smooth(a, nx, ny):
     """SMOOTHING_B on one parameter: a [nz][ncol] float32 -> float32."""
     nz = a.shape[0]
@@ -73,11 +114,12 @@ def smooth(a, nx, ny):
 
 def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 1, 1, 1), noise=4e-3, cycpow=2,
              maxsteps=5, solver=1, node_expansion=1, svdtol=1e-4, smoothing=True, max_iterations=1000,
-             first_cycle=1, regul=False, pen_typ=None, pen_alp=None):
+             first_cycle=1, regul=False, pen_typ=None, pen_alp=None, auto=False):
     """cpu: oracle.Cpu set up with resp=True and all Cartesian slots (and
     set_hydrostatic for HYDROSTATIC: yes).  max_nodes: {physical index 0..6:
     NSLB_MAX}; pen_typ / pen_alp: {physical index: PEN_TYP / PEN_ALP} of the
-    Tikhonov terms when regul.  Returns (model, stokes [ncol,4,numw], chi2
+    Tikhonov terms when regul; auto: STOKES SETUP AUTO (per-pixel weights of
+    auto_weights in place of wstk).  Returns (model, stokes [ncol,4,numw], chi2
     [ncol], iterations)."""
     cur = np.array(atm, np.float32, copy=True)
     nz, ncol = cur.shape[1], cur.shape[2]
@@ -88,6 +130,7 @@ def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 
     maxiter = 1 if nfreep == nfreev else int(math.floor(math.log(max(vmax.values())) / math.log(cycpow)))
     isig = weights(invstk, wstk, noise, numw)
     nfreq = len(isig)
+    isig_pix = auto_weights(obs, invstk, noise, numw) if auto else [isig] * ncol
     rows = [s for s in range(4) if invstk[s]]
     total = 0
 
@@ -145,7 +188,7 @@ def invert3d(cpu, atm, z, obs, nx, ny, max_nodes, invstk=(1, 1, 1, 1), wstk=(1, 
                 mb[:12] = best[:, :, c]
                 s_rows = syn[c][rows].reshape(-1)
                 mc2, mb2, sl, bs, jb, _ = cpu.lm_step(cfg, coefs, obs[c], s_rows, bsyn[c], sla[c], eq, ebest[c],
-                                                      isig, mc, mb)
+                                                      isig_pix[c], mc, mb)
                 cur[:, :, c] = mc2[:12]
                 best[:, :, c] = mb2[:12]
                 sla[c], bsyn[c], ebest[c] = sl, bs, jb

@@ -204,6 +204,8 @@ def test_misc_setup_assists():
     assert control.parse(base + "MISC SETUP:\nPGAS\nEND:\n").assist == gpu.ASSIST_P
     assert control.parse(base + "MISC SETUP:\nFULL_STOKES\nEND:\n").full_stokes
     assert not c.full_stokes
+    assert not c.auto_weights
+    assert control.parse(base.replace("STKV\n", "STKV\nAUTO\n")).auto_weights
 
 
 def test_coupled_inversion_section(tmp_path):
