@@ -130,6 +130,34 @@ def test_inversion_after_synthesis(gpu_ctx, tree):
     assert np.all(np.isfinite(m["T"])) and np.all(m["T"] >= 2500.0)
 
 
+def test_inversion_auto_weights(gpu_ctx, tree):
+    """test_inv1 with STOKES SETUP AUTO: every pixel weighted by the spread
+    of its own observed Stokes blocks (SET_WEIGHTS3D); the inversion must
+    still lower the misfit."""
+    run(tree, "test_syn1", gpu_ctx)
+
+    def hydro_start(c):
+        c.hydrostatic = True
+        c.fileprofile = "he_start.bin"
+
+    run(tree, "test_syn1", gpu_ctx, hydro_start)
+    out_syn1 = tree / "tests" / "test_syn1" / "out_dir"
+    obs, _, _ = modelio.read_profiles(str(out_syn1 / "syn_profile.bin"))
+    start, _, _ = modelio.read_profiles(str(out_syn1 / "he_start.bin"))
+
+    def auto(c):
+        c.auto_weights = True
+
+    c, out, wd = run(tree, "test_inv1", gpu_ctx, auto)
+    st, _, _ = modelio.read_profiles(os.path.join(wd, "out_syn_profile.bin"))
+    chi = float(np.asarray(out["chi2"]).reshape(-1)[0])
+    e0 = float(np.sqrt(np.mean((start - obs) ** 2)))
+    e1 = float(np.sqrt(np.mean((st - obs) ** 2)))
+    print("[control] test_inv1 AUTO: %d LM iterations, chi2 %.4g; rms misfit start %.3e -> %.3e"
+          % (out["iterations"], chi, e0, e1), flush=True)
+    assert np.isfinite(chi) and e1 < 0.5 * e0
+
+
 def test_syn1_without_continuum_normalisation(gpu_ctx, tree):
     """test_syn1 with CONTINUUM NORMALIZATION: F (GET_NORM without HSRA,
     normalization.f90:746-751: GAIN = 1, V with the opposite sign): the
